@@ -1,0 +1,78 @@
+"""ctypes binding of libfls.so (the C ABI declared in include/fls/fls.h).
+
+The library is built in-tree by `make -C fastlanesgpu_amd`; nothing here falls
+back to another implementation: if libfls.so is missing or a symbol is absent
+the load raises.
+"""
+import ctypes
+import os
+import pathlib
+
+_HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libfls.so"
+
+c_u32p = ctypes.POINTER(ctypes.c_uint32)
+c_u64p = ctypes.POINTER(ctypes.c_uint64)
+c_i64p = ctypes.POINTER(ctypes.c_int64)
+c_vp = ctypes.c_void_p
+c_u32 = ctypes.c_uint32
+c_u64 = ctypes.c_uint64
+c_int = ctypes.c_int
+
+# name -> (restype, argtypes).  Pointers are passed as c_void_p (ints).
+SIGNATURES = {
+    "fls_status_string": (ctypes.c_char_p, [c_int]),
+    "fls_last_hip_error": (c_int, []),
+    "fls_version": (ctypes.c_char_p, []),
+    "fls_pack_u32": (c_int, [c_vp, c_vp, c_u32]),
+    "fls_transpose_i_u32": (c_int, [c_vp, c_vp]),
+    "fls_transpose_o_u32": (c_int, [c_vp, c_vp]),
+    "fls_unrsum_u32": (c_int, [c_vp, c_vp]),
+    "fls_packed_words": (c_u64, [c_u64, c_u32]),
+    "fls_pack_column_u32": (c_int, [c_vp, c_u64, c_u32, c_u32, c_vp]),
+    "fls_delta_encode_column_u32": (c_int, [c_vp, c_u64, c_vp, c_u64, c_vp, c_vp, c_vp]),
+    "fls_unpack_u32": (c_int, [c_vp, c_vp, c_u64, c_u32, c_vp]),
+    "fls_unpack_u32_checksum": (c_int, [c_vp, c_vp, c_u64, c_u32, c_vp, c_vp]),
+    "fls_delta_decode_u32": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp, c_vp]),
+}
+
+_lib = None
+
+
+class FlsError(RuntimeError):
+    def __init__(self, func, status, lib):
+        msg = lib.fls_status_string(status).decode()
+        hip = lib.fls_last_hip_error() if status == -4 else 0
+        super().__init__(f"{func} failed: status {status} ({msg})" + (f", hip error {hip}" if hip else ""))
+        self.status = status
+
+
+def load(path=None):
+    """Load libfls.so and declare every C-ABI signature.  Raises if absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = pathlib.Path(path) if path else pathlib.Path(os.environ.get("FLS_LIB", LIB_PATH))
+    if not p.exists():
+        raise FileNotFoundError(f"{p} is not built: run `make -C {_HERE}` (or __graft_entry__.build())")
+    lib = ctypes.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)  # AttributeError if the export is missing
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(lib, name, status):
+    if status != 0:
+        raise FlsError(name, status, lib)
+
+
+def call(name, *args):
+    """Call a status-returning C-ABI function and raise FlsError on failure."""
+    lib = load()
+    st = getattr(lib, name)(*args)
+    check(lib, name, st)
+    return st

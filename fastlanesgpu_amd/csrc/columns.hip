@@ -1,0 +1,68 @@
+// Column descriptor validation and launch-shape helpers.
+#include "fls_columns.hpp"
+
+namespace fls {
+
+int device_cus(uint32_t* cus) {
+    static int cached[64] = {0};
+    int dev = 0;
+    FLS_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return FLS_ERR_NO_DEVICE;
+    if (!cached[dev]) {
+        int n = 0;
+        FLS_HIP_CHECK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+        if (n <= 0) return FLS_ERR_NO_DEVICE;
+        cached[dev] = n;
+    }
+    *cus = (uint32_t)cached[dev];
+    return FLS_OK;
+}
+
+int persistent_grid(uint64_t work, uint32_t blocks_per_cu, uint32_t* grid) {
+    uint32_t cus = 0;
+    int st = device_cus(&cus);
+    if (st) return st;
+    uint64_t g = (uint64_t)cus * blocks_per_cu;
+    if (work < g) g = work;
+    if (g == 0) g = 1;
+    *grid = (uint32_t)g;
+    return FLS_OK;
+}
+
+int to_dev_column(const fls_column& c, uint64_t n_rows, ColDev* out, bool allow_delta) {
+    const uint64_t n_vec = ceil_div(n_rows, 1024);
+    out->enc = c.encoding;
+    out->bw = c.bw;
+    out->base = c.base;
+    out->transposed = c.transposed;
+    out->words = c.words;
+    out->vmeta = c.vmeta;
+    out->bases = c.bases;
+    switch (c.encoding) {
+        case FLS_ENC_BITPACK:
+            if (c.bw > 32) return FLS_ERR_BW;
+            if (n_rows && c.bw && !c.words) return FLS_ERR_NULL;
+            if (c.n_words < n_vec * 32u * c.bw) return FLS_ERR_SIZE;
+            return FLS_OK;
+        case FLS_ENC_DELTA:
+            if (!allow_delta) return FLS_ERR_ENCODING;
+            if (n_rows && (!c.vmeta || !c.bases)) return FLS_ERR_NULL;
+            if (n_rows && c.n_words && !c.words) return FLS_ERR_NULL;
+            out->transposed = 1;
+            return FLS_OK;
+        default:
+            return FLS_ERR_ENCODING;
+    }
+}
+
+}  // namespace fls
+
+extern "C" uint64_t fls_column_bytes(const fls_column* c, uint64_t n_rows) {
+    if (!c) return 0;
+    const uint64_t n_vec = (n_rows + 1023u) / 1024u;
+    switch (c->encoding) {
+        case FLS_ENC_BITPACK: return n_vec * 32u * c->bw * 4u;
+        case FLS_ENC_DELTA: return c->n_words * 4u + n_vec * (sizeof(fls_vmeta) + 32u * 4u);
+        default: return c->n_words * 4u;
+    }
+}

@@ -1,0 +1,175 @@
+// Fused FastLanes decode + filter + aggregate, Q1.1 shape
+// (reference: QueryKernel_v1 / QueryKernel_v2, fastlanes/src/ssb/fls_q11.cu:29-135,
+// driver query<BT,IPT>() fls_q11.cu:137-171).
+//
+// Design for gfx950:
+//  * persistent grid-stride kernel, 256 work-items = 8 vectors per step; a
+//    wave decodes two vectors (work-item t: lane t%32 of vector t/32);
+//  * every column is decoded straight into registers (fused form); the
+//    FastLanes delta chain of a lane only touches that lane's own rows, so the
+//    orderdate rsum needs no LDS;
+//  * predicates become per-row lane masks that the compiler keeps in SGPR
+//    pairs, so AND-ing predicates is scalar work;
+//  * the int64 revenue and the row count are reduced per wave with DPP/shuffle,
+//    per block through LDS, and flushed with one 64-bit atomic each per block.
+#include "fls/device/unpack.hpp"
+#include "fls/device/wave.hpp"
+#include "fls_runtime.hpp"
+#include "fls_columns.hpp"
+
+namespace fls {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kVecPerBlock = kBlock / 32;
+
+struct Q11Dev {
+    ColDev date, qty, disc, price;
+    uint64_t n_rows, n_vec, n_groups;
+    uint32_t date_gt, date_lt, qty_lt, disc_lo, disc_hi;
+    int64_t* revenue;
+    uint64_t* count;
+};
+
+// Rows of this work-item that exist (only the last vector can be partial).
+__device__ __forceinline__ void valid_rows(uint64_t v, const Q11Dev& a, bool transposed, bool (&ok)[32]) {
+    const uint64_t first = v * 1024u;
+    const uint32_t l = dev::lane32();
+    if (first + 1024u <= a.n_rows) {
+#pragma unroll
+        for (int r = 0; r < 32; ++r) ok[r] = true;
+    } else {
+        const uint32_t live = (uint32_t)(a.n_rows - first);
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            const uint32_t p = l + 32u * r;
+            ok[r] = (transposed ? dev::untranspose_index(p) : p) < live;
+        }
+    }
+}
+
+template <int QBW, int DBW, int PBW, bool DATE_DELTA, bool PRED>
+__global__ __launch_bounds__(kBlock) void q11_kernel(const Q11Dev a) {
+    uint64_t acc = 0;
+    uint64_t cnt = 0;  // wave-level count, identical in every lane of the wave
+    const bool transposed = a.price.transposed != 0;
+    for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
+        const uint64_t v = g * kVecPerBlock + (threadIdx.x >> 5);
+        if (v >= a.n_vec) continue;
+        bool f[32];
+        valid_rows(v, a, transposed, f);
+
+        uint32_t x[32];
+        if constexpr (DATE_DELTA) decode_delta(a.date, v, x);
+        else decode_fixed<-1>(a.date, v, x);
+        // date_gt < x < date_lt  <=>  (x - date_gt - 1) < (date_lt - date_gt - 1)  (unsigned)
+        const uint32_t dlo = a.date_gt + 1u, dspan = a.date_lt - a.date_gt - 1u;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) f[r] = f[r] && (x[r] - dlo) < dspan && a.date_lt > a.date_gt;
+        if constexpr (PRED) {
+            bool any = false;
+#pragma unroll
+            for (int r = 0; r < 32; ++r) any |= f[r];
+            if (!__any(any)) continue;
+        }
+
+        decode_fixed<QBW>(a.qty, v, x);
+#pragma unroll
+        for (int r = 0; r < 32; ++r) f[r] = f[r] && x[r] < a.qty_lt;
+        if constexpr (PRED) {
+            bool any = false;
+#pragma unroll
+            for (int r = 0; r < 32; ++r) any |= f[r];
+            if (!__any(any)) continue;
+        }
+
+        uint32_t d[32];
+        decode_fixed<DBW>(a.disc, v, d);
+        const uint32_t span = a.disc_hi - a.disc_lo;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) f[r] = f[r] && (d[r] - a.disc_lo) <= span && a.disc_hi >= a.disc_lo;
+        if constexpr (PRED) {
+            bool any = false;
+#pragma unroll
+            for (int r = 0; r < 32; ++r) any |= f[r];
+            if (!__any(any)) continue;
+        }
+
+        decode_fixed<PBW>(a.price, v, x);
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            acc += (uint64_t)x[r] * (uint64_t)(f[r] ? d[r] : 0u);
+            cnt += __popcll(__ballot(f[r]));
+        }
+    }
+
+    // block reduction: wave sums -> LDS -> one atomic pair per block
+    __shared__ uint64_t s_acc[kBlock / 64];
+    __shared__ uint64_t s_cnt[kBlock / 64];
+    acc = dev::wave_sum(acc);
+    const int wave = threadIdx.x >> 6;
+    if (dev::is_wave_leader()) {
+        s_acc[wave] = acc;
+        s_cnt[wave] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t ta = 0, tc = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) {
+            ta += s_acc[w];
+            tc += s_cnt[w];
+        }
+        if (ta) atomicAdd((unsigned long long*)a.revenue, (unsigned long long)ta);
+        if (a.count && tc) atomicAdd((unsigned long long*)a.count, (unsigned long long)tc);
+    }
+}
+
+template <int QBW, int DBW, int PBW, bool DATE_DELTA>
+int launch(const Q11Dev& a, bool pred, uint32_t grid, hipStream_t s) {
+    if (pred) hipLaunchKernelGGL((q11_kernel<QBW, DBW, PBW, DATE_DELTA, true>), dim3(grid), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL((q11_kernel<QBW, DBW, PBW, DATE_DELTA, false>), dim3(grid), dim3(kBlock), 0, s, a);
+    return check_launch();
+}
+
+}  // namespace
+}  // namespace fls
+
+extern "C" int fls_query_q11(const fls_column* orderdate, const fls_column* quantity, const fls_column* discount,
+                             const fls_column* price, uint64_t n_rows, const fls_q11_params* p, uint32_t flags,
+                             int64_t* d_revenue, uint64_t* d_count, void* stream) {
+    using namespace fls;
+    FLS_RETURN_IF(!orderdate || !quantity || !discount || !price || !p || !d_revenue, FLS_ERR_NULL);
+    Q11Dev a{};
+    int st;
+    if ((st = to_dev_column(*orderdate, n_rows, &a.date, true)) != FLS_OK) return st;
+    if ((st = to_dev_column(*quantity, n_rows, &a.qty, false)) != FLS_OK) return st;
+    if ((st = to_dev_column(*discount, n_rows, &a.disc, false)) != FLS_OK) return st;
+    if ((st = to_dev_column(*price, n_rows, &a.price, false)) != FLS_OK) return st;
+    const uint32_t tr = price->transposed != 0;
+    FLS_RETURN_IF((orderdate->encoding == FLS_ENC_DELTA ? 1u : (orderdate->transposed != 0)) != tr ||
+                      (quantity->transposed != 0) != tr || (discount->transposed != 0) != tr,
+                  FLS_ERR_ARG);
+    a.n_rows = n_rows;
+    a.n_vec = ceil_div(n_rows, 1024);
+    a.n_groups = ceil_div(a.n_vec, kVecPerBlock);
+    a.date_gt = p->date_gt;
+    a.date_lt = p->date_lt;
+    a.qty_lt = p->qty_lt;
+    a.disc_lo = p->disc_lo;
+    a.disc_hi = p->disc_hi;
+    a.revenue = d_revenue;
+    a.count = d_count;
+    hipStream_t s = as_stream(stream);
+    if ((st = zero_u64(reinterpret_cast<uint64_t*>(d_revenue), 1, s)) != FLS_OK) return st;
+    if (d_count && (st = zero_u64(d_count, 1, s)) != FLS_OK) return st;
+    if (n_rows == 0) return FLS_OK;
+    uint32_t grid = 0;
+    if ((st = persistent_grid(a.n_groups, 8, &grid)) != FLS_OK) return st;
+    const bool pred = (flags & FLS_Q_PRED_LOAD) != 0;
+    const bool delta = a.date.enc == FLS_ENC_DELTA;
+    if (delta && a.qty.bw == 6 && a.disc.bw == 4 && a.price.bw == 24)
+        return launch<6, 4, 24, true>(a, pred, grid, s);
+    if (delta) return launch<-1, -1, -1, true>(a, pred, grid, s);
+    return launch<-1, -1, -1, false>(a, pred, grid, s);
+}

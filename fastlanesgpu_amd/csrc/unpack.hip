@@ -1,0 +1,137 @@
+// Grid-level FastLanes decode kernels (plain bit-unpack and FastLanes delta)
+// and their C-ABI launchers.
+//
+// One wave decodes two vectors (work-item t: lane t%32 of vector t/32 of the
+// wave); a 256-work-item block covers 8 vectors.  Each work-item issues bw
+// coalesced dword loads (two 128-byte runs per wave instruction) and 32
+// coalesced dword stores.
+#include "fls/device/unpack.hpp"
+#include "fls/device/wave.hpp"
+#include "fls_runtime.hpp"
+
+namespace fls {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kVecPerBlock = kBlock / 32;
+
+// Decode [+ write] [+ checksum sum_i (i+1)*v_i] of fixed-width vectors.
+template <int BW, bool WRITE, bool CHECKSUM>
+__global__ __launch_bounds__(kBlock) void unpack_global_kernel(const uint32_t* __restrict in,
+                                                               uint32_t* __restrict out, uint64_t n_vec,
+                                                               uint64_t n_values, uint64_t* __restrict checksum) {
+    const uint64_t v = (uint64_t)blockIdx.x * kVecPerBlock + (threadIdx.x >> 5);
+    const uint32_t l = dev::lane32();
+    uint64_t acc = 0;
+    if (v < n_vec) {
+        uint32_t vals[32];
+        dev::unpack_lane<BW>(in + v * (32u * BW), vals);
+        if constexpr (WRITE) {
+            uint32_t* o = out + v * 1024u + l;
+#pragma unroll
+            for (int r = 0; r < 32; ++r) __builtin_nontemporal_store(vals[r], o + 32 * r);
+        }
+        if constexpr (CHECKSUM) {
+            // sum_r (first + 32r) * v_r with first = v*1024 + l + 1
+            //   = first * S0 + 32 * S1,  S0 = sum v_r,  S1 = sum r*v_r   (mod 2^64)
+            const uint64_t first = v * 1024u + l + 1u;
+            const uint64_t vec_end = (v + 1) * 1024u;
+            uint64_t s0 = 0, s1 = 0;
+            if (vec_end <= n_values) {
+#pragma unroll
+                for (int r = 0; r < 32; ++r) {
+                    s0 += vals[r];
+                    s1 += (uint64_t)vals[r] * (uint32_t)r;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 32; ++r) {
+                    const bool live = first + 32u * r <= n_values;
+                    const uint32_t x = live ? vals[r] : 0u;
+                    s0 += x;
+                    s1 += (uint64_t)x * (uint32_t)r;
+                }
+            }
+            acc = first * s0 + 32u * s1;
+        }
+    }
+    if constexpr (CHECKSUM) {
+        acc = dev::wave_sum(acc);
+        if (dev::is_wave_leader() && acc != 0) atomicAdd((unsigned long long*)checksum, (unsigned long long)acc);
+    }
+}
+
+// FastLanes delta decode: per-vector (offset, bw), 32 bases per vector.
+__global__ __launch_bounds__(kBlock) void delta_decode_kernel(const uint32_t* __restrict words,
+                                                              const fls_vmeta* __restrict vmeta,
+                                                              const uint32_t* __restrict bases, uint64_t n_vec,
+                                                              uint32_t* __restrict out) {
+    const uint64_t v = (uint64_t)blockIdx.x * kVecPerBlock + (threadIdx.x >> 5);
+    const uint32_t l = dev::lane32();
+    if (v >= n_vec) return;
+    const fls_vmeta m = vmeta[v];
+    uint32_t vals[32];
+    dev::unpack_device(words + FLS_VMETA_OFFSET(m), vals, FLS_VMETA_BW(m));
+    dev::rsum_lane(vals, bases[v * 32u + l]);
+    uint32_t* o = out + v * 1024u + l;
+#pragma unroll
+    for (int r = 0; r < 32; ++r) __builtin_nontemporal_store(vals[r], o + 32 * r);
+}
+
+template <bool WRITE, bool CHECKSUM>
+struct UnpackTable {
+    using Fn = void (*)(const uint32_t*, uint32_t*, uint64_t, uint64_t, uint64_t*);
+    template <int... B>
+    static constexpr auto make(std::integer_sequence<int, B...>) {
+        return std::array<Fn, sizeof...(B)>{&unpack_global_kernel<B, WRITE, CHECKSUM>...};
+    }
+    static constexpr auto table = make(std::make_integer_sequence<int, 33>{});
+};
+
+template <bool WRITE, bool CHECKSUM>
+int launch_unpack(const uint32_t* in, uint32_t* out, uint64_t n_vec, uint64_t n_values, uint32_t bw,
+                  uint64_t* checksum, hipStream_t stream) {
+    if (n_vec == 0) return FLS_OK;
+    const uint64_t blocks = ceil_div(n_vec, kVecPerBlock);
+    FLS_RETURN_IF(blocks > 0x7fffffffu, FLS_ERR_SIZE);
+    auto fn = UnpackTable<WRITE, CHECKSUM>::table[bw];
+    hipLaunchKernelGGL(fn, dim3((uint32_t)blocks), dim3(kBlock), 0, stream, in, out, n_vec, n_values, checksum);
+    return check_launch();
+}
+
+}  // namespace
+}  // namespace fls
+
+extern "C" {
+
+int fls_unpack_u32(const uint32_t* d_in, uint32_t* d_out, uint64_t n_vec, uint32_t bw, void* stream) {
+    FLS_RETURN_IF(bw > 32, FLS_ERR_BW);
+    FLS_RETURN_IF(n_vec && (!d_out || (bw && !d_in)), FLS_ERR_NULL);
+    return fls::launch_unpack<true, false>(d_in, d_out, n_vec, n_vec * 1024u, bw, nullptr, fls::as_stream(stream));
+}
+
+int fls_unpack_u32_checksum(const uint32_t* d_in, uint32_t* d_out, uint64_t n_values, uint32_t bw,
+                            uint64_t* d_checksum, void* stream) {
+    FLS_RETURN_IF(bw > 32, FLS_ERR_BW);
+    FLS_RETURN_IF(!d_checksum, FLS_ERR_NULL);
+    FLS_RETURN_IF(n_values && bw && !d_in, FLS_ERR_NULL);
+    hipStream_t s = fls::as_stream(stream);
+    int st = fls::zero_u64(d_checksum, 1, s);
+    if (st) return st;
+    const uint64_t n_vec = fls::ceil_div(n_values, 1024);
+    if (d_out) return fls::launch_unpack<true, true>(d_in, d_out, n_vec, n_values, bw, d_checksum, s);
+    return fls::launch_unpack<false, true>(d_in, nullptr, n_vec, n_values, bw, d_checksum, s);
+}
+
+int fls_delta_decode_u32(const uint32_t* d_words, const fls_vmeta* d_vmeta, const uint32_t* d_bases,
+                         uint64_t n_vec, uint32_t* d_out, void* stream) {
+    if (n_vec == 0) return FLS_OK;
+    FLS_RETURN_IF(!d_words || !d_vmeta || !d_bases || !d_out, FLS_ERR_NULL);
+    const uint64_t blocks = fls::ceil_div(n_vec, fls::kVecPerBlock);
+    FLS_RETURN_IF(blocks > 0x7fffffffu, FLS_ERR_SIZE);
+    hipLaunchKernelGGL(fls::delta_decode_kernel, dim3((uint32_t)blocks), dim3(fls::kBlock), 0,
+                       fls::as_stream(stream), d_words, d_vmeta, d_bases, n_vec, d_out);
+    return fls::check_launch();
+}
+
+}  // extern "C"

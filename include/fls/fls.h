@@ -1,0 +1,161 @@
+/*
+ * fls.h -- C ABI of the MI355X (gfx950) FastLanes decode + query library.
+ *
+ * Every entry point returns an int status (FLS_OK = 0, negative on error) and
+ * never exits the process.  Device pointers are caller-owned and must stay
+ * valid until the work queued on `stream` completes; `stream` is a hipStream_t
+ * passed as void* (NULL = the null stream).  Kernels never allocate, except
+ * where a function says it owns a workspace.  Calls on one stream are ordered;
+ * calls on different streams that write the same output race.
+ *
+ * Data unit: the FastLanes vector of 1024 uint32 values, 32 lanes x 32 rows,
+ * value 32*r + l is row r of lane l; lane l's rows form an LSB-first bit
+ * stream whose word w sits at packed[32*w + l]  (fastlanes/src/pack.cpp:2970,
+ * fastlanes/src/include/fls_gen/unpack/unpack.cuh:877-971).  A vector packed
+ * at bit width bw occupies 32*bw words; vectors of a fixed-width column are
+ * contiguous at v*32*bw.
+ *
+ * Each declaration cites the reference interface it replaces (paths relative
+ * to the reference repository root).
+ */
+#ifndef FLS_FLS_H
+#define FLS_FLS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+enum {
+    FLS_OK = 0,
+    FLS_ERR_BW = -1,        /* bit width outside 0..32 */
+    FLS_ERR_SIZE = -2,      /* count or buffer size out of range */
+    FLS_ERR_NULL = -3,      /* required pointer is NULL */
+    FLS_ERR_HIP = -4,       /* a HIP runtime call failed (fls_last_hip_error) */
+    FLS_ERR_ENCODING = -5,  /* unknown or unsupported encoding */
+    FLS_ERR_ARG = -6,       /* other invalid argument */
+    FLS_ERR_CAPACITY = -7,  /* output buffer too small */
+    FLS_ERR_NO_DEVICE = -8  /* no usable gfx950 device */
+};
+
+#define FLS_VECTOR_SIZE 1024u
+#define FLS_LANES 32u
+
+/* Human-readable text for a status code. */
+const char* fls_status_string(int status);
+/* Last HIP error code seen by the library on this thread (0 if none). */
+int fls_last_hip_error(void);
+/* Library version, e.g. "fls-mi355x 0.1". */
+const char* fls_version(void);
+
+/* ========================================================= host encoders (CPU)
+ * One 1024-value vector.  Replaces
+ * generated::pack::fallback::scalar::pack(const uint32_t*, uint32_t*, uint8_t)
+ * (fastlanes/src/include/fls_gen/pack/pack.hpp:8-23; bodies pack.cpp:29607).
+ * Values are masked to bw bits; writes 32*bw words.                          */
+int fls_pack_u32(const uint32_t* in, uint32_t* out, uint32_t bw);
+/* generated::transpose::fallback::scalar::transpose_i (transpose.hpp:8-20,
+ * transpose.cpp:4110).                                                        */
+int fls_transpose_i_u32(const uint32_t* in, uint32_t* out);
+/* Inverse of fls_transpose_i_u32.  The reference's transpose_o
+ * (transpose.cpp:5136) repeats transpose_i's permutation; this one is the
+ * untranspose its name promises.                                              */
+int fls_transpose_o_u32(const uint32_t* in, uint32_t* out);
+/* generated::unrsum::fallback::scalar::unrsum (unrsum.hpp:8, unrsum.cpp:120). */
+int fls_unrsum_u32(const uint32_t* in, uint32_t* out);
+
+/* Column-level host encoders.  A column of n values becomes ceil(n/1024)
+ * vectors; the tail of the last vector is padded (0 for bit-packing, the last
+ * value for delta/RLE).
+ * FLS_ENC_FLAG_TRANSPOSE applies transpose_i to each vector before packing,
+ * which puts a bit-packed column in the same row order as a delta- or
+ * RLE-decoded one (FastLanes' unified transposed layout).                     */
+#define FLS_ENC_FLAG_TRANSPOSE 1u
+uint64_t fls_packed_words(uint64_t n, uint32_t bw);
+int fls_pack_column_u32(const uint32_t* in, uint64_t n, uint32_t bw, uint32_t flags, uint32_t* out);
+
+/* Per-vector metadata word: bits 0..7 bit width, bits 8..63 word offset of the
+ * vector's packed data (cf. the per-vector {bw, offset} tables of
+ * fastlanes/src/ssb/fls_q21_bitpacked_opt_v4.cu:293-331).                     */
+typedef uint64_t fls_vmeta;
+#define FLS_VMETA(offset, bw) (((uint64_t)(offset) << 8) | (uint64_t)(bw))
+#define FLS_VMETA_BW(m) ((uint32_t)((m)&0xffu))
+#define FLS_VMETA_OFFSET(m) ((uint64_t)(m) >> 8)
+
+/* FastLanes delta of a column (fastlanes/example/fastlanes_bench_delta.cu:143-155:
+ * transpose_i, bases = first 32 transposed values, unrsum, pack at the
+ * smallest bw of each vector).  Writes vmeta[n_vec], bases[32*n_vec] and the
+ * packed words (capacity out_cap words); *out_words gets the words used.     */
+int fls_delta_encode_column_u32(const uint32_t* in, uint64_t n, uint32_t* out, uint64_t out_cap,
+                                fls_vmeta* vmeta, uint32_t* bases, uint64_t* out_words);
+
+/* ========================================================== GPU decode (gfx950)
+ * Grid-level decode of n_vec fixed-width vectors into normal order
+ * out[v*1024 + 32*r + l].  Replaces __global__ unpack_global(in, out, bw)
+ * launched <<<n_vec, 32>>> (unpack.cuh:3436-3442, unpack_fused.cuh:3444-3453). */
+int fls_unpack_u32(const uint32_t* d_in, uint32_t* d_out, uint64_t n_vec, uint32_t bw, void* stream);
+/* Same decode that also produces the order-sensitive checksum
+ * sum_i (i+1)*value_i mod 2^64 over the first n_values values into *d_checksum
+ * (overwritten).  d_out may be NULL to decode without materialising.          */
+int fls_unpack_u32_checksum(const uint32_t* d_in, uint32_t* d_out, uint64_t n_values, uint32_t bw,
+                            uint64_t* d_checksum, void* stream);
+/* FastLanes delta decode: unpack at each vector's bw + d_rsum_32 with the
+ * vector's 32 bases (rsum.cuh:3-107, delta_shared_memory.cu:41-45).  Output is
+ * in transposed order (as fastlanes_bench_delta.cu:183-191 checks it).        */
+int fls_delta_decode_u32(const uint32_t* d_words, const fls_vmeta* d_vmeta, const uint32_t* d_bases,
+                         uint64_t n_vec, uint32_t* d_out, void* stream);
+
+/* ================================================================ columns
+ * A compressed column on the device.  All columns read by one query must use
+ * the same row order inside each vector: either normal order, or FastLanes'
+ * transposed order (transposed = 1), which delta and RLE always produce.     */
+enum {
+    FLS_ENC_BITPACK = 0, /* fixed bw for every vector, + frame of reference `base` */
+    FLS_ENC_DELTA = 1,   /* FastLanes delta, per-vector (offset,bw) + 32 bases     */
+    FLS_ENC_RLE = 2      /* FastLanes RLE (run values + delta-coded run index)     */
+};
+
+typedef struct fls_column {
+    uint32_t encoding;       /* FLS_ENC_* */
+    uint32_t bw;             /* BITPACK: width of every vector */
+    uint32_t base;           /* BITPACK: value = decoded + base (FOR) */
+    uint32_t transposed;     /* rows in FastLanes transposed order */
+    uint64_t n_words;        /* packed words in `words` (for sizes/bounds) */
+    const uint32_t* words;   /* packed words */
+    const fls_vmeta* vmeta;  /* DELTA / RLE: per-vector (offset, bw); else NULL */
+    const uint32_t* bases;   /* DELTA: 32 bases per vector; else NULL */
+} fls_column;
+
+/* ==================================================== fused decode + query
+ * Q1.1 shape, fls_q11.cu:29-81 (QueryKernel_v1) and its driver
+ * query<BT,IPT>() (fls_q11.cu:137-171): rows with
+ *   date_gt < orderdate < date_lt  AND  quantity < qty_lt  AND
+ *   disc_lo <= discount <= disc_hi
+ * contribute price*discount (64-bit product) to *d_revenue; *d_count (may be
+ * NULL) gets the number of qualifying rows.  Both are overwritten.
+ * orderdate may be DELTA or BITPACK; the others BITPACK.                     */
+typedef struct fls_q11_params {
+    uint32_t date_gt, date_lt, qty_lt, disc_lo, disc_hi;
+} fls_q11_params;
+
+/* Skip loading (not only decoding) a wave's later columns once none of its
+ * rows can qualify -- the predicated-load variant (Crystal-opt BlockPredLoad,
+ * crystal-opt/load.cuh:25-97; fls_q11.cu:83-135).  Without it every byte of
+ * every column is read (full scan).                                          */
+#define FLS_Q_PRED_LOAD 1u
+
+int fls_query_q11(const fls_column* orderdate, const fls_column* quantity, const fls_column* discount,
+                  const fls_column* price, uint64_t n_rows, const fls_q11_params* params, uint32_t flags,
+                  int64_t* d_revenue, uint64_t* d_count, void* stream);
+
+/* Encoded bytes a full scan of a column reads for n_rows (packed words, plus
+ * per-vector metadata and bases for DELTA).                                   */
+uint64_t fls_column_bytes(const fls_column* col, uint64_t n_rows);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLS_FLS_H */

@@ -1,0 +1,328 @@
+/*
+ * CPU oracle for the FastLanes decode + Crystal query hot path.
+ * TEST INFRASTRUCTURE ONLY: the checker for the product library, never part of it.
+ * Declarations, citations and the pinning status of each function: oracle.h.
+ *
+ * Style is intentionally naive (one bit at a time, one row at a time) so that
+ * it shares no structure with the product's word-parallel encoders/decoders.
+ */
+#include "oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+/* ===================================================================== layout
+ * The FastLanes vector: 1024 values as 32 lanes x 32 rows; value index 32*r+l
+ * is row r of lane l.  Lane l's values form an LSB-first bit stream, row r at
+ * bits [r*bw, r*bw+bw); bit p of lane l lives in word 32*(p/32)+l, bit p%32.
+ * (fastlanes/src/pack.cpp:2970-3123 writes exactly these words; the reference
+ * masks every input value to bw bits first, pack.cpp:2975.)                  */
+void orc_pack_u32(const uint32_t* in, uint32_t* out, unsigned bw) {
+    if (bw == 0) return; /* pack_0bit_32ow writes nothing */
+    memset(out, 0, sizeof(uint32_t) * 32u * bw);
+    for (unsigned l = 0; l < 32; ++l) {
+        for (unsigned r = 0; r < 32; ++r) {
+            uint64_t v = in[32 * r + l];
+            for (unsigned b = 0; b < bw; ++b) {
+                if ((v >> b) & 1u) {
+                    unsigned p = r * bw + b;
+                    out[32 * (p / 32) + l] |= 1u << (p % 32);
+                }
+            }
+        }
+    }
+}
+
+void orc_unpack_u32(const uint32_t* in, uint32_t* out, unsigned bw) {
+    for (unsigned l = 0; l < 32; ++l) {
+        for (unsigned r = 0; r < 32; ++r) {
+            uint32_t v = 0;
+            for (unsigned b = 0; b < bw; ++b) {
+                unsigned p = r * bw + b;
+                uint32_t bit = (in[32 * (p / 32) + l] >> (p % 32)) & 1u;
+                v |= bit << b;
+            }
+            out[32 * r + l] = v;
+        }
+    }
+}
+
+/* =================================================================== transpose
+ * transpose_i (transpose.cpp:4110-5135) moves input k = 64a + 8b + c
+ * (a<16, b<8, c<8) to output a + 16*rev3(b) + 128*c, where rev3 reverses the
+ * three bits of b (k=8 -> 64, k=16 -> 32, k=32 -> 16, k=64 -> 1).           */
+static unsigned rev3(unsigned b) { return ((b & 1u) << 2) | (b & 2u) | ((b >> 2) & 1u); }
+
+static unsigned transpose_pos(unsigned k) {
+    unsigned a = k / 64, b = (k / 8) % 8, c = k % 8;
+    return a + 16 * rev3(b) + 128 * c;
+}
+
+void orc_transpose_i_u32(const uint32_t* in, uint32_t* out) {
+    for (unsigned k = 0; k < 1024; ++k) out[transpose_pos(k)] = in[k];
+}
+
+void orc_transpose_o_u32(const uint32_t* in, uint32_t* out) {
+    for (unsigned k = 0; k < 1024; ++k) out[k] = in[transpose_pos(k)];
+}
+
+/* ======================================================================= rsum
+ * rsum.cuh:8-106 walks, for lane l, positions l + O_j with
+ * O = 0,128,...,896, 64,...,960, 32,...,928, 96,...,992.                    */
+unsigned orc_rsum_offset(unsigned j) {
+    static const unsigned grp[4] = {0, 64, 32, 96};
+    return 128 * (j % 8) + grp[j / 8];
+}
+
+void orc_unrsum_u32(const uint32_t* in, uint32_t* out) {
+    for (unsigned l = 0; l < 32; ++l) {
+        out[l + orc_rsum_offset(0)] = 0; /* unrsum.cpp:130 */
+        for (unsigned j = 1; j < 32; ++j) {
+            unsigned cur = l + orc_rsum_offset(j), prev = l + orc_rsum_offset(j - 1);
+            out[cur] = in[cur] - in[prev];
+        }
+    }
+}
+
+void orc_rsum_u32(const uint32_t* in, uint32_t* out, const uint32_t* base) {
+    for (unsigned l = 0; l < 32; ++l) {
+        uint32_t acc = base[l];
+        for (unsigned j = 0; j < 32; ++j) {
+            unsigned p = l + orc_rsum_offset(j);
+            acc += in[p];
+            out[p] = acc;
+        }
+    }
+}
+
+/* ========================================================================= bw */
+unsigned orc_bit_width(uint32_t x) {
+    unsigned w = 0;
+    while (x) {
+        ++w;
+        x >>= 1;
+    }
+    return w;
+}
+
+unsigned orc_find_bw4(const uint32_t* in, unsigned n) {
+    uint32_t mx = 0;
+    for (unsigned i = 0; i < n; ++i)
+        if (in[i] > mx) mx = in[i];
+    unsigned w = orc_bit_width(mx);
+    return (w + 3) / 4 * 4; /* ssb_utils.h:226-236: 0 stays 0, else next multiple of 4 */
+}
+
+/* ====================================================================== delta */
+unsigned orc_delta_encode_u32(const uint32_t* in, uint32_t* packed, uint32_t* bases) {
+    uint32_t t[1024], d[1024];
+    orc_transpose_i_u32(in, t);
+    for (unsigned l = 0; l < 32; ++l) bases[l] = t[l];
+    orc_unrsum_u32(t, d);
+    uint32_t mx = 0;
+    for (unsigned i = 0; i < 1024; ++i)
+        if (d[i] > mx) mx = d[i];
+    unsigned bw = orc_bit_width(mx);
+    orc_pack_u32(d, packed, bw);
+    return bw;
+}
+
+void orc_delta_decode_u32(const uint32_t* packed, unsigned bw, const uint32_t* bases, uint32_t* out_t) {
+    uint32_t d[1024];
+    orc_unpack_u32(packed, d, bw);
+    orc_rsum_u32(d, out_t, bases);
+}
+
+/* ======================================================================== RLE */
+uint32_t orc_linear_extract(const uint32_t* words, uint64_t k, unsigned bw) {
+    uint32_t v = 0;
+    for (unsigned b = 0; b < bw; ++b) {
+        uint64_t p = k * bw + b;
+        v |= ((words[p / 32] >> (p % 32)) & 1u) << b;
+    }
+    return v;
+}
+
+static void linear_put(uint32_t* words, uint64_t k, unsigned bw, uint32_t v) {
+    for (unsigned b = 0; b < bw; ++b) {
+        uint64_t p = k * bw + b;
+        if ((v >> b) & 1u) words[p / 32] |= 1u << (p % 32);
+    }
+}
+
+void orc_rle_encode_u32(const uint32_t* in, unsigned n, orc_rle_vec* hdr, uint32_t* values_out,
+                        uint32_t* idx_out) {
+    uint32_t full[1024], idx[1024], runv[1024];
+    unsigned nr = 0;
+    for (unsigned i = 0; i < 1024; ++i) full[i] = in[i < n ? i : (n ? n - 1 : 0)];
+    if (n == 0)
+        for (unsigned i = 0; i < 1024; ++i) full[i] = 0;
+    for (unsigned i = 0; i < 1024; ++i) {
+        if (i == 0 || full[i] != full[i - 1]) runv[nr++] = full[i];
+        idx[i] = nr - 1;
+    }
+    uint32_t mn = runv[0], mx = runv[0];
+    for (unsigned k = 0; k < nr; ++k) {
+        if (runv[k] < mn) mn = runv[k];
+        if (runv[k] > mx) mx = runv[k];
+    }
+    hdr->n_runs = nr;
+    hdr->vbase = mn;
+    hdr->vbw = orc_bit_width(mx - mn);
+    memset(values_out, 0, sizeof(uint32_t) * ((nr * hdr->vbw + 31) / 32));
+    for (unsigned k = 0; k < nr; ++k) linear_put(values_out, k, hdr->vbw, runv[k] - mn);
+    hdr->ibw = orc_delta_encode_u32(idx, idx_out, hdr->bases);
+}
+
+void orc_rle_decode_u32(const orc_rle_vec* hdr, const uint32_t* values, const uint32_t* idx,
+                        uint32_t* out_t) {
+    uint32_t id_t[1024];
+    orc_delta_decode_u32(idx, hdr->ibw, hdr->bases, id_t);
+    for (unsigned p = 0; p < 1024; ++p) out_t[p] = hdr->vbase + orc_linear_extract(values, id_t[p], hdr->vbw);
+}
+
+/* ================================================================== checksum */
+uint64_t orc_checksum_u32(const uint32_t* v, uint64_t n) {
+    uint64_t s = 0;
+    for (uint64_t i = 0; i < n; ++i) s += (i + 1) * (uint64_t)v[i];
+    return s;
+}
+
+/* ======================================================================= RNG
+ * splitmix64 finaliser (Steele, Lea, Flood, "Fast splittable pseudorandom
+ * number generators", OOPSLA 2014), keyed by (seed, stream) and counter i.   */
+uint64_t orc_mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+uint64_t orc_rand64(uint64_t seed, uint64_t stream, uint64_t i) {
+    uint64_t key = orc_mix64(seed ^ (stream * 0x9E3779B97F4A7C15ULL));
+    return orc_mix64(key + (i + 1) * 0x9E3779B97F4A7C15ULL);
+}
+
+/* uniform in [0, range), range <= 2^32: high 32 bits scaled by range */
+uint32_t orc_uniform(uint64_t seed, uint64_t stream, uint64_t i, uint64_t range) {
+    uint64_t hi = orc_rand64(seed, stream, i) >> 32;
+    return (uint32_t)((hi * range) >> 32);
+}
+
+static int is_leap(unsigned y) { return (y % 4 == 0 && y % 100 != 0) || y % 400 == 0; }
+
+uint32_t orc_yyyymmdd(uint32_t day) {
+    static const unsigned mdays[12] = {31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31};
+    unsigned y = 1992;
+    for (;;) {
+        unsigned len = is_leap(y) ? 366 : 365;
+        if (day < len) break;
+        day -= len;
+        ++y;
+    }
+    unsigned m = 0;
+    for (;;) {
+        unsigned len = mdays[m] + (m == 1 && is_leap(y) ? 1 : 0);
+        if (day < len) break;
+        day -= len;
+        ++m;
+    }
+    return y * 10000 + (m + 1) * 100 + (day + 1);
+}
+
+/* Streams: one per synthetic column, fixed so every consumer agrees. */
+enum {
+    ST_C1 = 1,
+    ST_QTY = 11, ST_DISC = 12, ST_PRICE = 13,
+    ST_DFK = 21, ST_SFK = 22, ST_PFK = 23, ST_CFK = 24, ST_REV = 25,
+    ST_SREG = 31, ST_SNAT = 32, ST_PCAT = 33, ST_PBRAND = 34, ST_CREG = 35, ST_CNAT = 36
+};
+
+void orc_gen_c1(uint64_t seed, uint64_t n, unsigned bw, uint32_t* v) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) v[i] = orc_uniform(seed, ST_C1, (uint64_t)i, 1ULL << bw);
+}
+
+#define C2_DAYS 2557u /* 1992-01-01 .. 1998-12-31 */
+
+void orc_gen_c2(uint64_t seed, uint64_t n, uint32_t* orderdate, uint32_t* quantity, uint32_t* discount,
+                uint32_t* price) {
+    uint32_t table[C2_DAYS];
+    for (unsigned d = 0; d < C2_DAYS; ++d) table[d] = orc_yyyymmdd(d);
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        uint64_t i = (uint64_t)ii;
+        /* sorted and uniform over the days: row i falls on day floor(i*D/n) */
+        orderdate[i] = table[(i * C2_DAYS) / n];
+        quantity[i] = 1 + orc_uniform(seed, ST_QTY, i, 50);
+        discount[i] = orc_uniform(seed, ST_DISC, i, 11);
+        price[i] = 90000 + orc_uniform(seed, ST_PRICE, i, 10000000 - 90000);
+    }
+}
+
+void orc_gen_c3_fact(uint64_t seed, uint64_t n, uint32_t* datefk, uint32_t* suppfk, uint32_t* partfk,
+                     uint32_t* custfk, uint32_t* revenue) {
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        uint64_t i = (uint64_t)ii;
+        datefk[i] = orc_uniform(seed, ST_DFK, i, 2556);
+        suppfk[i] = orc_uniform(seed, ST_SFK, i, 40000);
+        partfk[i] = orc_uniform(seed, ST_PFK, i, 800000);
+        if (custfk) custfk[i] = orc_uniform(seed, ST_CFK, i, 600000);
+        revenue[i] = orc_uniform(seed, ST_REV, i, 10000000);
+    }
+}
+
+void orc_gen_c3_dims(uint64_t seed, uint32_t n_date, uint32_t n_supp, uint32_t n_part, uint32_t n_cust,
+                     uint32_t* d_year, uint32_t* s_region, uint32_t* s_nation, uint32_t* p_category,
+                     uint32_t* p_brand, uint32_t* c_region, uint32_t* c_nation) {
+    for (uint32_t d = 0; d < n_date; ++d) d_year[d] = orc_yyyymmdd(d) / 10000;
+    for (uint32_t s = 0; s < n_supp; ++s) {
+        s_region[s] = 1 + orc_uniform(seed, ST_SREG, s, 5);
+        s_nation[s] = 1 + orc_uniform(seed, ST_SNAT, s, 25);
+    }
+    for (uint32_t p = 0; p < n_part; ++p) {
+        p_category[p] = 1 + orc_uniform(seed, ST_PCAT, p, 25);
+        p_brand[p] = 1 + orc_uniform(seed, ST_PBRAND, p, 1000);
+    }
+    for (uint32_t c = 0; c < n_cust; ++c) {
+        c_region[c] = 1 + orc_uniform(seed, ST_CREG, c, 5);
+        c_nation[c] = 1 + orc_uniform(seed, ST_CNAT, c, 25);
+    }
+}
+
+/* ==================================================================== queries */
+int64_t orc_q11(const uint32_t* orderdate, const uint32_t* quantity, const uint32_t* discount,
+                const uint32_t* price, uint64_t n, uint32_t date_gt, uint32_t date_lt, uint32_t qty_lt,
+                uint32_t disc_lo, uint32_t disc_hi, uint64_t* count) {
+    int64_t sum = 0;
+    uint64_t cnt = 0;
+#pragma omp parallel for reduction(+ : sum, cnt) schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+        uint64_t i = (uint64_t)ii;
+        if (orderdate[i] > date_gt && orderdate[i] < date_lt && quantity[i] < qty_lt && discount[i] >= disc_lo &&
+            discount[i] <= disc_hi) {
+            sum += (int64_t)price[i] * (int64_t)discount[i];
+            cnt += 1;
+        }
+    }
+    if (count) *count = cnt;
+    return sum;
+}
+
+#define Q21_SLOTS 7000u
+
+void orc_q21(const uint32_t* datefk, const uint32_t* suppfk, const uint32_t* partfk, const uint32_t* revenue,
+             uint64_t n, const uint32_t* d_year, const uint32_t* s_region, const uint32_t* p_category,
+             const uint32_t* p_brand, uint32_t cat, uint32_t reg, int64_t* sums, uint64_t* counts) {
+    memset(sums, 0, sizeof(int64_t) * Q21_SLOTS);
+    memset(counts, 0, sizeof(uint64_t) * Q21_SLOTS);
+    for (uint64_t i = 0; i < n; ++i) {
+        if (s_region[suppfk[i]] != reg) continue;
+        if (p_category[partfk[i]] != cat) continue;
+        uint32_t year = d_year[datefk[i]];
+        uint32_t brand = p_brand[partfk[i]];
+        uint32_t slot = (brand * 7 + (year - 1992)) % Q21_SLOTS; /* fls_q21.cu:71 */
+        sums[slot] += (int64_t)revenue[i];
+        counts[slot] += 1;
+    }
+}

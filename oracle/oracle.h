@@ -1,0 +1,131 @@
+/*
+ * CPU oracle for the FastLanes decode + Crystal query hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by the
+ * tests, by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py,
+ * and by nothing else.  The product library (fastlanesgpu_amd/libfls.so) never
+ * links, loads or calls it.
+ *
+ * Every function restates one piece of the reference's algorithm in plain,
+ * deliberately naive C (bit-by-bit packing, closed-form permutations, scalar
+ * query loops).  The citation on each declaration names the reference code it
+ * follows (paths relative to the reference repository root).
+ *
+ * Pinning status (what independently checks each function):
+ *   orc_pack_u32 / orc_unpack_u32      pinned: golden helper::rand_arr_* vectors
+ *                                      packed by the reference's own pack.cpp
+ *                                      (tests/golden), oracle/_ref build, bw 0/32
+ *                                      special cases, round trip.
+ *   orc_transpose_{i,o}_u32            pinned: oracle/_ref transpose.cpp build,
+ *                                      committed 1024-entry permutation fixture.
+ *   orc_unrsum_u32 / orc_rsum_u32      pinned: oracle/_ref unrsum.cpp build,
+ *                                      committed unrsum fixtures, rsum∘unrsum = id.
+ *   orc_delta_*                        pinned through the three above.
+ *   orc_find_bw4 / orc_bit_width       pinned: closed form + brute force.
+ *   orc_rle_*                          format defined by this project (the
+ *                                      reference's RLE is a CPU sketch):
+ *                                      round trip + brute-force run counting.
+ *   orc_q11 / orc_q21                  pinned: textbook semantics checked against
+ *                                      numpy brute force on small tables.
+ *   orc_gen_* / orc_rand*              definitions of the synthetic inputs.
+ *   orc_mlp_*                          pinned: finite differences in double.
+ */
+#ifndef FLS_ORACLE_H
+#define FLS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- bit layout */
+/* fastlanes/src/pack.cpp:2970-3123 (pack_11bit_32ow), dispatch pack.cpp:29607-29709 */
+void orc_pack_u32(const uint32_t* in, uint32_t* out, unsigned bw);
+/* fastlanes/src/include/fls_gen/unpack/unpack.cuh:877-971 (per-lane decode), :3330-3432 */
+void orc_unpack_u32(const uint32_t* in, uint32_t* out, unsigned bw);
+
+/* fastlanes/src/transpose.cpp:4110-5135 (transpose_i u32), :5136-6161 (transpose_o u32) */
+void orc_transpose_i_u32(const uint32_t* in, uint32_t* out);
+void orc_transpose_o_u32(const uint32_t* in, uint32_t* out);
+/* fastlanes/src/unrsum.cpp:120-256 */
+void orc_unrsum_u32(const uint32_t* in, uint32_t* out);
+/* fastlanes/src/include/fls_gen/rsum/rsum.cuh:3-107 */
+void orc_rsum_u32(const uint32_t* in, uint32_t* out, const uint32_t* base);
+/* position order of the rsum chain (rsum.cuh:8-106): returns O_j for j in [0,32) */
+unsigned orc_rsum_offset(unsigned j);
+
+/* ---------------------------------------------------------------- FOR / bw */
+unsigned orc_bit_width(uint32_t x);
+/* fastlanes/src/include/ssb_utils.h:222-239 (find_bw: round up to a multiple of 4) */
+unsigned orc_find_bw4(const uint32_t* in, unsigned n);
+
+/* ---------------------------------------------------------------- delta (a8)
+ * Encode one 1024-value vector (original order) following
+ * fastlanes/example/fastlanes_bench_delta.cu:143-155: transpose_i, bases = the
+ * first 32 transposed values, unrsum, pack at the smallest bw.  Returns bw.
+ * `packed` needs room for 32*32 words.                                        */
+unsigned orc_delta_encode_u32(const uint32_t* in, uint32_t* packed, uint32_t* bases);
+/* unpack + rsum; output is in transposed order (fastlanes_bench_delta.cu:183-191) */
+void orc_delta_decode_u32(const uint32_t* packed, unsigned bw, const uint32_t* bases, uint32_t* out_t);
+
+/* ---------------------------------------------------------------- RLE (a9)
+ * Project-defined FastLanes RLE of one vector of n <= 1024 values (original
+ * order; rows >= n are padding that repeats the last value):
+ *   run values (FOR base + linear LSB-first bit-packing at width vbw),
+ *   run index idx[i] delta-encoded exactly like orc_delta_encode_u32.
+ * Decoding yields transposed order, like delta.                               */
+typedef struct orc_rle_vec {
+    uint32_t n_runs;
+    uint32_t vbase;      /* min run value */
+    uint32_t vbw;        /* width of (run value - vbase) */
+    uint32_t ibw;        /* width of the run-index deltas: 0 or 1 */
+    uint32_t bases[32];  /* run index at each lane's chain start */
+} orc_rle_vec;
+/* values_out needs ceil(1024*32/32) words, idx_out 32*32 words. */
+void orc_rle_encode_u32(const uint32_t* in, unsigned n, orc_rle_vec* hdr, uint32_t* values_out,
+                        uint32_t* idx_out);
+void orc_rle_decode_u32(const orc_rle_vec* hdr, const uint32_t* values, const uint32_t* idx,
+                        uint32_t* out_t);
+uint32_t orc_linear_extract(const uint32_t* words, uint64_t k, unsigned bw);
+
+/* ---------------------------------------------------------------- checksums */
+/* sum over i of (i+1)*v[i] modulo 2^64 (order sensitive) */
+uint64_t orc_checksum_u32(const uint32_t* v, uint64_t n);
+
+/* ---------------------------------------------------------------- RNG + data */
+uint64_t orc_mix64(uint64_t z);
+uint64_t orc_rand64(uint64_t seed, uint64_t stream, uint64_t i);
+uint32_t orc_uniform(uint64_t seed, uint64_t stream, uint64_t i, uint64_t range);
+/* days since 1992-01-01 -> yyyymmdd */
+uint32_t orc_yyyymmdd(uint32_t day);
+
+/* fused filter+sum fact table (config C2 shape), n rows */
+void orc_gen_c1(uint64_t seed, uint64_t n, unsigned bw, uint32_t* v);
+void orc_gen_c2(uint64_t seed, uint64_t n, uint32_t* orderdate, uint32_t* quantity, uint32_t* discount,
+                uint32_t* price);
+/* probe+group-by tables (config C3 shape); fks are 0-based dense keys */
+void orc_gen_c3_fact(uint64_t seed, uint64_t n, uint32_t* datefk, uint32_t* suppfk, uint32_t* partfk,
+                     uint32_t* custfk, uint32_t* revenue);
+void orc_gen_c3_dims(uint64_t seed, uint32_t n_date, uint32_t n_supp, uint32_t n_part, uint32_t n_cust,
+                     uint32_t* d_year, uint32_t* s_region, uint32_t* s_nation, uint32_t* p_category,
+                     uint32_t* p_brand, uint32_t* c_region, uint32_t* c_nation);
+
+/* ---------------------------------------------------------------- queries
+ * Scalar Q1.1 shape, fastlanes/src/ssb/fls_q11.cu:29-81: rows with
+ * date_gt < orderdate < date_lt, quantity < qty_lt, disc_lo <= discount <= disc_hi;
+ * returns sum(price*discount) as int64 and the qualifying row count.           */
+int64_t orc_q11(const uint32_t* orderdate, const uint32_t* quantity, const uint32_t* discount,
+                const uint32_t* price, uint64_t n, uint32_t date_gt, uint32_t date_lt, uint32_t qty_lt,
+                uint32_t disc_lo, uint32_t disc_hi, uint64_t* count);
+/* Scalar Q2.1 shape, fastlanes/src/ssb/fls_q21.cu:21-78,197-263: keep rows whose
+ * part has category==cat and supplier has region==reg; group slot
+ * (brand*7 + year-1992) % 7000; sums[7000] int64 and counts[7000].            */
+void orc_q21(const uint32_t* datefk, const uint32_t* suppfk, const uint32_t* partfk, const uint32_t* revenue,
+             uint64_t n, const uint32_t* d_year, const uint32_t* s_region, const uint32_t* p_category,
+             const uint32_t* p_brand, uint32_t cat, uint32_t reg, int64_t* sums, uint64_t* counts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,179 @@
+"""Shared helpers for the tests: the CPU oracle (ctypes), the reference's own
+encoders built into oracle/_ref (when present), and device plumbing."""
+import ctypes
+import pathlib
+
+import numpy as np
+import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+ORACLE_SO = ROOT / "oracle" / "liboracle.so"
+REF_SO = ROOT / "oracle" / "_ref" / "libflsref.so"
+GOLDEN = ROOT / "tests" / "golden"
+
+U32P = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C")
+U64P = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C")
+I64P = np.ctypeslib.ndpointer(dtype=np.int64, flags="C")
+
+
+class OrcRle(ctypes.Structure):
+    _fields_ = [("n_runs", ctypes.c_uint32), ("vbase", ctypes.c_uint32), ("vbw", ctypes.c_uint32),
+                ("ibw", ctypes.c_uint32), ("bases", ctypes.c_uint32 * 32)]
+
+
+_oracle = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is not None:
+        return _oracle
+    if not ORACLE_SO.exists():
+        raise FileNotFoundError(f"{ORACLE_SO} missing: run `make -C oracle`")
+    o = ctypes.CDLL(str(ORACLE_SO))
+    u, u64, c_u = ctypes.c_uint, ctypes.c_uint64, ctypes.c_uint32
+    o.orc_pack_u32.argtypes = [U32P, U32P, u]
+    o.orc_unpack_u32.argtypes = [U32P, U32P, u]
+    for n in ("orc_transpose_i_u32", "orc_transpose_o_u32", "orc_unrsum_u32"):
+        getattr(o, n).argtypes = [U32P, U32P]
+    o.orc_rsum_u32.argtypes = [U32P, U32P, U32P]
+    o.orc_rsum_offset.argtypes = [u]
+    o.orc_rsum_offset.restype = u
+    o.orc_bit_width.argtypes = [c_u]
+    o.orc_bit_width.restype = u
+    o.orc_find_bw4.argtypes = [U32P, u]
+    o.orc_find_bw4.restype = u
+    o.orc_delta_encode_u32.argtypes = [U32P, U32P, U32P]
+    o.orc_delta_encode_u32.restype = u
+    o.orc_delta_decode_u32.argtypes = [U32P, u, U32P, U32P]
+    o.orc_rle_encode_u32.argtypes = [U32P, u, ctypes.POINTER(OrcRle), U32P, U32P]
+    o.orc_rle_decode_u32.argtypes = [ctypes.POINTER(OrcRle), U32P, U32P, U32P]
+    o.orc_linear_extract.argtypes = [U32P, u64, u]
+    o.orc_linear_extract.restype = c_u
+    o.orc_checksum_u32.argtypes = [U32P, u64]
+    o.orc_checksum_u32.restype = u64
+    o.orc_mix64.argtypes = [u64]
+    o.orc_mix64.restype = u64
+    o.orc_rand64.argtypes = [u64, u64, u64]
+    o.orc_rand64.restype = u64
+    o.orc_uniform.argtypes = [u64, u64, u64, u64]
+    o.orc_uniform.restype = c_u
+    o.orc_yyyymmdd.argtypes = [c_u]
+    o.orc_yyyymmdd.restype = c_u
+    o.orc_gen_c1.argtypes = [u64, u64, u, U32P]
+    o.orc_gen_c2.argtypes = [u64, u64, U32P, U32P, U32P, U32P]
+    o.orc_gen_c3_fact.argtypes = [u64, u64, U32P, U32P, U32P, ctypes.c_void_p, U32P]
+    o.orc_gen_c3_dims.argtypes = [u64, c_u, c_u, c_u, c_u] + [U32P] * 7
+    o.orc_q11.argtypes = [U32P, U32P, U32P, U32P, u64, c_u, c_u, c_u, c_u, c_u, ctypes.POINTER(u64)]
+    o.orc_q11.restype = ctypes.c_int64
+    o.orc_q21.argtypes = [U32P, U32P, U32P, U32P, u64, U32P, U32P, U32P, U32P, c_u, c_u, I64P, U64P]
+    _oracle = o
+    return o
+
+
+def ref():
+    """The reference's own encoders (oracle/_ref), or skip when not built."""
+    if not REF_SO.exists():
+        pytest.skip("oracle/_ref not built (reference checkout absent)")
+    r = ctypes.CDLL(str(REF_SO))
+    r.ref_pack_u32.argtypes = [U32P, U32P, ctypes.c_uint]
+    for n in ("ref_transpose_i_u32", "ref_transpose_o_u32", "ref_unrsum_u32"):
+        getattr(r, n).argtypes = [U32P, U32P]
+    return r
+
+
+def golden():
+    return np.load(GOLDEN / "fls_layout_golden.npz")
+
+
+# ----------------------------------------------------------------- oracle wrappers
+def o_pack(x, bw):
+    out = np.zeros(1024, np.uint32)
+    oracle().orc_pack_u32(np.ascontiguousarray(x, np.uint32), out, bw)
+    return out[: 32 * bw]
+
+
+def o_unpack(words, bw):
+    buf = np.zeros(1024, np.uint32)
+    buf[: 32 * bw] = words[: 32 * bw]
+    out = np.zeros(1024, np.uint32)
+    oracle().orc_unpack_u32(buf, out, bw)
+    return out
+
+
+def o_perm(name, x):
+    out = np.zeros(1024, np.uint32)
+    getattr(oracle(), name)(np.ascontiguousarray(x, np.uint32), out)
+    return out
+
+
+def o_delta_encode(x):
+    packed = np.zeros(1024, np.uint32)
+    bases = np.zeros(32, np.uint32)
+    bw = oracle().orc_delta_encode_u32(np.ascontiguousarray(x, np.uint32), packed, bases)
+    return packed[: 32 * bw], bases, bw
+
+
+def o_delta_decode(packed, bw, bases):
+    buf = np.zeros(1024, np.uint32)
+    buf[: 32 * bw] = packed[: 32 * bw]
+    out = np.zeros(1024, np.uint32)
+    oracle().orc_delta_decode_u32(buf, bw, np.ascontiguousarray(bases, np.uint32), out)
+    return out
+
+
+def o_gen_c1(seed, n, bw):
+    v = np.zeros(n, np.uint32)
+    oracle().orc_gen_c1(seed, n, bw, v)
+    return v
+
+
+def o_gen_c2(seed, n):
+    cols = [np.zeros(n, np.uint32) for _ in range(4)]
+    oracle().orc_gen_c2(seed, n, *cols)
+    return cols
+
+
+def o_checksum(v):
+    return int(oracle().orc_checksum_u32(np.ascontiguousarray(v, np.uint32), v.size))
+
+
+def transpose_vectors(col):
+    """Apply the oracle transpose_i to every 1024-value vector of a padded column."""
+    n_vec = col.size // 1024
+    out = np.empty_like(col)
+    for v in range(n_vec):
+        out[v * 1024:(v + 1) * 1024] = o_perm("orc_transpose_i_u32", col[v * 1024:(v + 1) * 1024])
+    return out
+
+
+# ----------------------------------------------------------------- device helpers
+def torch_cuda():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def to_dev(a):
+    torch = torch_cuda()
+    a = np.ascontiguousarray(a)
+    if a.dtype == np.uint32:
+        a = a.view(np.int32)
+    elif a.dtype == np.uint64:
+        a = a.view(np.int64)
+    return torch.from_numpy(a.copy()).cuda()
+
+
+def dev_empty(n, dtype="u32"):
+    torch = torch_cuda()
+    dt = {"u32": torch.int32, "u64": torch.int64, "i64": torch.int64}[dtype]
+    return torch.empty(max(1, n), dtype=dt, device="cuda")
+
+
+def from_dev(t, dtype=np.uint32):
+    import torch
+
+    torch.cuda.synchronize()
+    return t.cpu().numpy().view(dtype)

@@ -1,0 +1,33 @@
+"""libfls.so loads and exports every function that include/fls/*.h declares."""
+import re
+
+import fastlanesgpu_amd as fls
+import fls_testlib as T
+
+
+def declared_functions():
+    names = set()
+    for h in (T.ROOT / "include" / "fls").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(fls_\w+)\s*\(", text, re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_every_declared_symbol_is_exported():
+    lib = fls.load()
+    names = declared_functions()
+    assert len(names) >= 10
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_every_declared_symbol_has_a_python_signature():
+    from fastlanesgpu_amd._lib import SIGNATURES
+
+    missing = sorted(declared_functions() - set(SIGNATURES))
+    assert not missing, missing
+
+
+def test_version():
+    assert fls.load().fls_version().startswith(b"fls-mi355x")

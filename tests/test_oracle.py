@@ -1,0 +1,242 @@
+"""Pins the CPU oracle before anything is checked against it.
+
+Each check is independent of the oracle's own formulas: the reference's golden
+vectors and its compiled encoders (oracle/_ref), closed forms a mistake would
+break, Python's datetime, and brute force.
+"""
+import ctypes
+import datetime
+
+import numpy as np
+import pytest
+
+import fls_testlib as T
+
+RNG = np.random.default_rng(7)
+
+
+@pytest.mark.parametrize("variant", ["normal", "fused"])
+def test_pack_matches_reference_golden(variant):
+    g = T.golden()
+    rand, packed = g[f"rand_{variant}"], g[f"packed_{variant}"]
+    for bw in range(33):
+        np.testing.assert_array_equal(T.o_pack(rand[bw], bw), packed[bw][: 32 * bw], err_msg=f"bw={bw}")
+
+
+@pytest.mark.parametrize("variant", ["normal", "fused"])
+def test_unpack_golden_is_reference_kat(variant):
+    # cuda_*_unpack_test.cu:37-366: unpack(pack(rand_arr_bw)) == rand_arr_bw
+    g = T.golden()
+    rand, packed = g[f"rand_{variant}"], g[f"packed_{variant}"]
+    for bw in range(33):
+        mask = (1 << bw) - 1
+        assert (rand[bw].astype(np.uint64) & mask == rand[bw]).all()
+        np.testing.assert_array_equal(T.o_unpack(packed[bw], bw), rand[bw], err_msg=f"bw={bw}")
+
+
+def test_pack_matches_reference_build_random():
+    r = T.ref()
+    for bw in range(33):
+        for _ in range(3):
+            x = RNG.integers(0, 2**32, 1024, dtype=np.uint64).astype(np.uint32)  # masking exercised
+            want = np.zeros(1024, np.uint32)
+            r.ref_pack_u32(x, want, bw)
+            np.testing.assert_array_equal(T.o_pack(x, bw), want[: 32 * bw], err_msg=f"bw={bw}")
+
+
+def test_bw0_and_bw32_special_cases():
+    x = RNG.integers(0, 2**32, 1024, dtype=np.uint64).astype(np.uint32)
+    assert T.o_pack(x, 0).size == 0
+    assert (T.o_unpack(np.zeros(0, np.uint32), 0) == 0).all()
+    np.testing.assert_array_equal(T.o_pack(x, 32), x)  # unpack.cuh:3256-3329 is a copy
+    np.testing.assert_array_equal(T.o_unpack(x, 32), x)
+
+
+@pytest.mark.parametrize("bw", [8, 16])
+def test_byte_and_halfword_widths(bw):
+    # at bw 8 / 16 the rows of a lane are the bytes / halfwords of its words
+    x = RNG.integers(0, 2**bw, 1024).astype(np.uint32)
+    packed = T.o_pack(x, bw)
+    per = 32 // bw
+    for lane in (0, 5, 31):
+        for w in range(bw):
+            word = int(packed[32 * w + lane])
+            for k in range(per):
+                assert (word >> (bw * k)) & ((1 << bw) - 1) == x[32 * (per * w + k) + lane]
+
+
+def test_transpose_i_matches_reference():
+    g = T.golden()
+    ar = np.arange(1024, dtype=np.uint32)
+    np.testing.assert_array_equal(T.o_perm("orc_transpose_i_u32", ar), g["transpose_i_perm"])
+    # textbook statement: input 64a+8b+c lands at a + 16*bitrev3(b) + 128*c
+    perm = g["transpose_i_perm"]
+    for k in (0, 1, 7, 8, 16, 24, 32, 63, 64, 65, 1023):
+        a, b, c = k // 64, (k // 8) % 8, k % 8
+        rb = int(f"{b:03b}"[::-1], 2)
+        assert perm[a + 16 * rb + 128 * c] == k
+
+
+def test_transpose_o_is_inverse_and_reference_defect():
+    x = RNG.integers(0, 2**32, 1024, dtype=np.uint64).astype(np.uint32)
+    t = T.o_perm("orc_transpose_i_u32", x)
+    np.testing.assert_array_equal(T.o_perm("orc_transpose_o_u32", t), x)
+    np.testing.assert_array_equal(T.o_perm("orc_transpose_i_u32", T.o_perm("orc_transpose_o_u32", x)), x)
+    # The reference's transpose_o (transpose.cpp:5136) repeats transpose_i: it
+    # is not an untranspose, so the oracle deliberately differs from it.
+    g = T.golden()
+    np.testing.assert_array_equal(g["transpose_o_ref"], g["transpose_i_perm"])
+
+
+def test_unrsum_matches_reference_golden():
+    g = T.golden()
+    for k in range(g["unrsum_in"].shape[0]):
+        np.testing.assert_array_equal(T.o_perm("orc_unrsum_u32", g["unrsum_in"][k]), g["unrsum_out"][k])
+
+
+def test_unrsum_matches_reference_build_random():
+    r = T.ref()
+    x = RNG.integers(0, 2**32, 1024, dtype=np.uint64).astype(np.uint32)
+    want = np.zeros(1024, np.uint32)
+    r.ref_unrsum_u32(x, want)
+    np.testing.assert_array_equal(T.o_perm("orc_unrsum_u32", x), want)
+
+
+def test_rsum_order_matches_reference():
+    order = [T.oracle().orc_rsum_offset(j) for j in range(32)]
+    np.testing.assert_array_equal(order, T.golden()["rsum_order"])
+
+
+def test_rsum_inverts_unrsum():
+    for x in (RNG.integers(0, 2**32, 1024, dtype=np.uint64).astype(np.uint32),
+              np.sort(RNG.integers(0, 2**31, 1024)).astype(np.uint32)):
+        d = T.o_perm("orc_unrsum_u32", x)
+        out = np.zeros(1024, np.uint32)
+        T.oracle().orc_rsum_u32(d, out, np.ascontiguousarray(x[:32]))
+        np.testing.assert_array_equal(out, x)
+
+
+def test_delta_encode_matches_reference_bench():
+    # fastlanes_bench_delta.cu:131-155 (value i%1024) encoded by the reference
+    g = T.golden()
+    x = np.arange(1024, dtype=np.uint32)
+    packed, bases, bw = T.o_delta_encode(x)
+    assert bw == int(g["delta_i1024_bw"][0])
+    np.testing.assert_array_equal(bases, g["delta_i1024_bases"])
+    np.testing.assert_array_equal(packed, g["delta_i1024_packed"][: 32 * bw])
+    # decode is in transposed order (fastlanes_bench_delta.cu:183-191)
+    np.testing.assert_array_equal(T.o_delta_decode(packed, bw, bases), T.o_perm("orc_transpose_i_u32", x))
+
+
+def test_delta_roundtrip_sorted_and_wrapping():
+    for x in (np.sort(RNG.integers(0, 2**32, 1024, dtype=np.uint64)).astype(np.uint32),
+              RNG.integers(0, 2**32, 1024, dtype=np.uint64).astype(np.uint32),
+              np.full(1024, 77, np.uint32)):
+        packed, bases, bw = T.o_delta_encode(x)
+        np.testing.assert_array_equal(T.o_delta_decode(packed, bw, bases), T.o_perm("orc_transpose_i_u32", x))
+    packed, bases, bw = T.o_delta_encode(np.full(1024, 77, np.uint32))
+    assert bw == 0
+
+
+def test_bit_width_and_find_bw4():
+    o = T.oracle()
+    for x in (0, 1, 2, 3, 15, 16, 255, 256, 2**31 - 1, 2**31, 2**32 - 1):
+        assert o.orc_bit_width(x) == int(x).bit_length()
+    # ssb_utils.h:222-239: 0 -> 0, 1..4 -> 4, 5..8 -> 8, ..., 29..32 -> 32
+    for w in range(33):
+        arr = np.zeros(1024, np.uint32)
+        if w:
+            arr[3] = (1 << (w - 1))
+        want = 0 if w == 0 else 4 * ((w + 3) // 4)
+        assert o.orc_find_bw4(arr, 1024) == want, w
+
+
+def test_checksum_brute_force():
+    v = RNG.integers(0, 2**32, 5000, dtype=np.uint64).astype(np.uint32)
+    want = sum((i + 1) * int(x) for i, x in enumerate(v)) % 2**64
+    assert T.o_checksum(v) == want
+
+
+def test_yyyymmdd_against_datetime():
+    o = T.oracle()
+    d0 = datetime.date(1992, 1, 1)
+    for d in range(2557):
+        want = int((d0 + datetime.timedelta(days=d)).strftime("%Y%m%d"))
+        assert o.orc_yyyymmdd(d) == want
+    assert o.orc_yyyymmdd(2556) == 19981231
+
+
+def test_uniform_range_and_determinism():
+    o = T.oracle()
+    vals = [o.orc_uniform(5, 3, i, 11) for i in range(20000)]
+    assert min(vals) == 0 and max(vals) == 10
+    counts = np.bincount(vals, minlength=11)
+    assert counts.min() > 1500  # roughly uniform
+    assert vals[:50] == [o.orc_uniform(5, 3, i, 11) for i in range(50)]
+    assert o.orc_uniform(5, 3, 0, 2**32) == o.orc_rand64(5, 3, 0) >> 32
+
+
+def test_gen_c2_columns():
+    n = 300_000
+    od, q, d, p = T.o_gen_c2(1, n)
+    assert (np.diff(od.astype(np.int64)) >= 0).all() and od[0] == 19920101 and od[-1] == 19981231
+    assert q.min() == 1 and q.max() == 50 and d.min() == 0 and d.max() == 10
+    assert p.min() >= 90000 and p.max() < 10**7
+    assert int(p.max()).bit_length() == 24 and int(q.max()).bit_length() == 6 and int(d.max()).bit_length() == 4
+
+
+def test_q11_brute_force():
+    n = 50_000
+    od, q, d, p = T.o_gen_c2(3, n)
+    c = ctypes.c_uint64(0)
+    s = T.oracle().orc_q11(od, q, d, p, n, 19930000, 19940000, 25, 1, 3, ctypes.byref(c))
+    m = (od > 19930000) & (od < 19940000) & (q < 25) & (d >= 1) & (d <= 3)
+    assert s == int((p[m].astype(np.int64) * d[m].astype(np.int64)).sum())
+    assert c.value == int(m.sum()) and c.value > 0
+
+
+def test_q21_brute_force():
+    o = T.oracle()
+    n, nd, ns, np_, nc = 40_000, 2556, 400, 8000, 600
+    fk = [np.zeros(n, np.uint32) for _ in range(4)]
+    o.orc_gen_c3_fact(9, n, fk[0], fk[1], fk[2], None, fk[3])
+    datefk, suppfk, partfk, rev = fk
+    suppfk %= ns
+    partfk %= np_
+    dims = [np.zeros(k, np.uint32) for k in (nd, ns, ns, np_, np_, nc, nc)]
+    o.orc_gen_c3_dims(9, nd, ns, np_, nc, *dims)
+    d_year, s_region, _, p_cat, p_brand, _, _ = dims
+    sums = np.zeros(7000, np.int64)
+    counts = np.zeros(7000, np.uint64)
+    o.orc_q21(datefk, suppfk, partfk, rev, n, d_year, s_region, p_cat, p_brand, 12, 2, sums, counts)
+    want = {}
+    for i in range(n):
+        if s_region[suppfk[i]] == 2 and p_cat[partfk[i]] == 12:
+            key = (int(d_year[datefk[i]]), int(p_brand[partfk[i]]))
+            want[key] = want.get(key, 0) + int(rev[i])
+    got = {}
+    for slot in np.nonzero(counts)[0]:
+        year, brand = 1992 + slot % 7, slot // 7 if slot >= 7 else 1000
+        got[(year, int(brand))] = int(sums[slot])
+    assert got == want and len(want) > 10
+    assert set(d_year.tolist()) == set(range(1992, 1999))
+
+
+def test_rle_roundtrip_and_run_count():
+    o = T.oracle()
+    for n in (1024, 1000, 1, 513):
+        starts = RNG.random(1024) < 1 / 8
+        starts[0] = True
+        vals = RNG.integers(0, 2**13, 1024).astype(np.uint32)
+        x = vals[np.cumsum(starts) - 1].astype(np.uint32)
+        hdr = T.OrcRle()
+        values = np.zeros(1024, np.uint32)
+        idx = np.zeros(1024, np.uint32)
+        o.orc_rle_encode_u32(x, n, hdr, values, idx)
+        full = np.concatenate([x[:n], np.full(1024 - n, x[n - 1], np.uint32)])
+        runs = 1 + int((full[1:] != full[:-1]).sum())
+        assert hdr.n_runs == runs
+        assert hdr.ibw == (1 if runs > 1 else 0)
+        out = np.zeros(1024, np.uint32)
+        o.orc_rle_decode_u32(hdr, values, idx, out)
+        np.testing.assert_array_equal(out, T.o_perm("orc_transpose_i_u32", full))
