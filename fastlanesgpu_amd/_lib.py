@@ -36,6 +36,31 @@ SIGNATURES = {
     "fls_delta_decode_u32": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp, c_vp]),
 }
 
+
+
+class FlsColumn(ctypes.Structure):
+    """struct fls_column (include/fls/fls.h)."""
+    _fields_ = [("encoding", c_u32), ("bw", c_u32), ("base", c_u32), ("transposed", c_u32),
+                ("n_words", c_u64), ("words", c_vp), ("vmeta", c_vp), ("bases", c_vp)]
+
+
+class Q11Params(ctypes.Structure):
+    """struct fls_q11_params."""
+    _fields_ = [("date_gt", c_u32), ("date_lt", c_u32), ("qty_lt", c_u32), ("disc_lo", c_u32), ("disc_hi", c_u32)]
+
+
+COLP = ctypes.POINTER(FlsColumn)
+
+SIGNATURES.update({
+    "fls_query_q11": (c_int, [COLP, COLP, COLP, COLP, c_u64, ctypes.POINTER(Q11Params), c_u32, c_vp, c_vp, c_vp]),
+    "fls_column_bytes": (c_u64, [COLP, c_u64]),
+    "fls_gen_c1": (c_int, [c_u64, c_u64, c_u32, c_vp, c_vp]),
+    "fls_copy_bytes": (c_int, [c_vp, c_vp, c_u64, c_vp]),
+    "fls_gen_c2": (c_int, [c_u64, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "fls_gen_c3_fact": (c_int, [c_u64, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "fls_gen_c3_dims": (c_int, [c_u64, c_u32, c_u32, c_u32, c_u32] + [c_vp] * 8),
+})
+
 _lib = None
 
 

@@ -47,4 +47,68 @@ __device__ __forceinline__ void decode_delta(const ColDev& c, uint64_t v, uint32
     dev::rsum_lane(x, c.bases[v * 32u + dev::lane32()]);
 }
 
+// Predicate-fused FastLanes delta walk of one lane at compile-time width BW:
+// the running sum visits the rows in chain order (rsum.cuh:8-106) and each
+// decoded value is reduced to one bit of the returned row mask at once, so
+// only the BW packed words stay live.
+template <int BW, typename Pred>
+__device__ __forceinline__ uint32_t delta_mask_lane(const uint32_t* __restrict lane_in, uint32_t base, Pred pred) {
+    uint32_t w[BW > 0 ? BW : 1];
+    dev::load_lane<BW>(lane_in, w);
+    if constexpr (BW == 0) {
+        return pred(base) ? 0xffffffffu : 0u;
+    } else {
+        uint32_t acc = base, m = 0;
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+            const int r = dev::rsum_row(j);
+            acc += dev::extract_at<BW>(w, r);
+            m |= (uint32_t)pred(acc) << r;
+        }
+        return m;
+    }
+}
+
+// Run-time-width delta walk of one lane: each row reads the (at most two)
+// words it straddles, which the first touch brings into the cache, and
+// extracts with one funnel shift.  Partially unrolled so that only a few row
+// loads are in flight and few VGPRs are held, whatever the width.
+template <typename Pred>
+__device__ __forceinline__ uint32_t delta_mask_walk(const uint32_t* __restrict in, uint32_t bw, uint32_t base,
+                                                    Pred pred) {
+    const uint32_t mask = bw >= 32 ? 0xffffffffu : ((1u << bw) - 1u);
+    uint32_t acc = base, m = 0;
+#pragma unroll 4
+    for (int j = 0; j < 32; ++j) {
+        const int r = dev::rsum_row(j);
+        const uint32_t bit = (uint32_t)r * bw, k = bit >> 5, s = bit & 31u;
+        const uint32_t lo = in[32u * k];
+        const uint32_t hi = (k + 1u < bw) ? in[32u * (k + 1u)] : 0u;
+        acc += __builtin_amdgcn_alignbit(hi, lo, s) & mask;
+        m |= (uint32_t)pred(acc) << r;
+    }
+    return m;
+}
+
+// Row mask {r : pred(value of row r)} of vector v of a DELTA column.  Widths
+// up to 16 take a fully unrolled per-width walk (all words in registers, one
+// extract per row); wider vectors take the run-time walk, which bounds the
+// VGPRs a kernel must reserve for this column at about 16 + a few.
+template <typename Pred>
+__device__ __forceinline__ uint32_t delta_mask(const ColDev& c, uint64_t v, Pred pred) {
+    const fls_vmeta m = c.vmeta[v];
+    const uint32_t* in = c.words + FLS_VMETA_OFFSET(m) + dev::lane32();
+    const uint32_t base = c.bases[v * 32u + dev::lane32()];
+    const uint32_t bw = FLS_VMETA_BW(m);
+    switch (bw) {
+#define FLS_DM_CASE(B) \
+    case B: return delta_mask_lane<B>(in, base, pred);
+        FLS_DM_CASE(0) FLS_DM_CASE(1) FLS_DM_CASE(2) FLS_DM_CASE(3) FLS_DM_CASE(4) FLS_DM_CASE(5) FLS_DM_CASE(6)
+        FLS_DM_CASE(7) FLS_DM_CASE(8) FLS_DM_CASE(9) FLS_DM_CASE(10) FLS_DM_CASE(11) FLS_DM_CASE(12)
+        FLS_DM_CASE(13) FLS_DM_CASE(14) FLS_DM_CASE(15) FLS_DM_CASE(16)
+#undef FLS_DM_CASE
+        default: return delta_mask_walk(in, bw, base, pred);
+    }
+}
+
 }  // namespace fls

@@ -125,10 +125,114 @@ __global__ __launch_bounds__(kBlock) void q11_kernel(const Q11Dev a) {
     }
 }
 
+// Row mask of the existing rows of this work-item (bit r = row r).
+__device__ __forceinline__ uint32_t valid_mask(uint64_t v, const Q11Dev& a, bool transposed) {
+    const uint64_t first = v * 1024u;
+    if (first + 1024u <= a.n_rows) return 0xffffffffu;
+    const uint32_t live = (uint32_t)(a.n_rows - first), l = dev::lane32();
+    uint32_t m = 0;
+#pragma unroll
+    for (int r = 0; r < 32; ++r) {
+        const uint32_t p = l + 32u * r;
+        m |= (uint32_t)((transposed ? dev::untranspose_index(p) : p) < live) << r;
+    }
+    return m;
+}
+
+// Make the compiler treat the loaded words as consumed here, so their loads
+// are issued (and complete) before any data-dependent branch: a full scan
+// reads every byte even when the ALU work of a wave is skipped.
+template <int N>
+__device__ __forceinline__ void pin_words(uint32_t (&w)[N]) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) asm volatile("" : "+v"(w[k]));
+}
+
+// Fast path: compile-time widths for quantity/discount/price, FastLanes delta
+// orderdate.  Live state per work-item: the packed words of the three
+// fixed-width columns (QBW+DBW+PBW VGPRs), one 32-bit date mask, accumulators.
+// FULL: every column's words are loaded for every vector; waves whose date
+// mask is empty skip only the extraction/aggregation ALU work.
+// PRED: later columns are not even loaded for such waves.
+template <int QBW, int DBW, int PBW, bool PRED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void q11_fast_kernel(const Q11Dev a) {
+    uint64_t acc = 0;
+    uint32_t cnt = 0;
+    const uint32_t l = dev::lane32();
+    const uint32_t dlo = a.date_gt + 1u, dspan = a.date_lt - a.date_gt - 1u;
+    const bool date_ok = a.date_lt > a.date_gt, disc_ok = a.disc_hi >= a.disc_lo;
+    const uint32_t span = a.disc_hi - a.disc_lo;
+    // rows of the (possibly partial) last vector, computed once: one VGPR
+    const uint32_t tail = valid_mask(a.n_vec - 1, a, true);
+    for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
+        const uint64_t v = g * kVecPerBlock + (threadIdx.x >> 5);
+        if (v >= a.n_vec) continue;
+        // orderdate first: its per-vector width makes this a (half-wave
+        // divergent) switch whose widest case holds 32 words, so nothing else
+        // is kept live across it.
+        uint32_t m = delta_mask(a.date, v, [=](uint32_t x) { return (x - dlo) < dspan; });
+        if (v == a.n_vec - 1) m &= tail;
+        if (!date_ok || !disc_ok) m = 0;
+        uint32_t qw[QBW > 0 ? QBW : 1], dw[DBW > 0 ? DBW : 1], pw[PBW > 0 ? PBW : 1];
+        if constexpr (!PRED) {
+            dev::load_lane<QBW>(a.qty.words + v * (32u * QBW) + l, qw);
+            dev::load_lane<DBW>(a.disc.words + v * (32u * DBW) + l, dw);
+            dev::load_lane<PBW>(a.price.words + v * (32u * PBW) + l, pw);
+            pin_words(qw);
+            pin_words(dw);
+            pin_words(pw);
+            if (!__any(m != 0)) continue;
+        } else {
+            if (!__any(m != 0)) continue;
+            dev::load_lane<QBW>(a.qty.words + v * (32u * QBW) + l, qw);
+            dev::load_lane<DBW>(a.disc.words + v * (32u * DBW) + l, dw);
+            dev::load_lane<PBW>(a.price.words + v * (32u * PBW) + l, pw);
+        }
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            // branch-free: every term is a 0/1 integer, so the row loop stays
+            // one straight-line block
+            const uint32_t q = dev::extract_at<QBW>(qw, r) + a.qty.base;
+            const uint32_t d = dev::extract_at<DBW>(dw, r) + a.disc.base;
+            const uint32_t f = ((m >> r) & 1u) & (uint32_t)(q < a.qty_lt) & (uint32_t)((d - a.disc_lo) <= span);
+            const uint32_t p = dev::extract_at<PBW>(pw, r) + a.price.base;
+            acc += (uint64_t)p * (uint64_t)(d * f);
+            cnt += f;
+        }
+    }
+    __shared__ uint64_t s_acc[kBlock / 64];
+    __shared__ uint64_t s_cnt[kBlock / 64];
+    acc = dev::wave_sum(acc);
+    const uint64_t c64 = dev::wave_sum((uint64_t)cnt);
+    const int wave = threadIdx.x >> 6;
+    if (dev::is_wave_leader()) {
+        s_acc[wave] = acc;
+        s_cnt[wave] = c64;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t ta = 0, tc = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) {
+            ta += s_acc[w];
+            tc += s_cnt[w];
+        }
+        if (ta) atomicAdd((unsigned long long*)a.revenue, (unsigned long long)ta);
+        if (a.count && tc) atomicAdd((unsigned long long*)a.count, (unsigned long long)tc);
+    }
+}
+
 template <int QBW, int DBW, int PBW, bool DATE_DELTA>
 int launch(const Q11Dev& a, bool pred, uint32_t grid, hipStream_t s) {
     if (pred) hipLaunchKernelGGL((q11_kernel<QBW, DBW, PBW, DATE_DELTA, true>), dim3(grid), dim3(kBlock), 0, s, a);
     else hipLaunchKernelGGL((q11_kernel<QBW, DBW, PBW, DATE_DELTA, false>), dim3(grid), dim3(kBlock), 0, s, a);
+    return check_launch();
+}
+
+template <int QBW, int DBW, int PBW>
+int launch_fast(const Q11Dev& a, bool pred, uint32_t grid, hipStream_t s) {
+    if (pred) hipLaunchKernelGGL((q11_fast_kernel<QBW, DBW, PBW, true>), dim3(grid), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL((q11_fast_kernel<QBW, DBW, PBW, false>), dim3(grid), dim3(kBlock), 0, s, a);
     return check_launch();
 }
 
@@ -161,15 +265,16 @@ extern "C" int fls_query_q11(const fls_column* orderdate, const fls_column* quan
     a.revenue = d_revenue;
     a.count = d_count;
     hipStream_t s = as_stream(stream);
-    if ((st = zero_u64(reinterpret_cast<uint64_t*>(d_revenue), 1, s)) != FLS_OK) return st;
-    if (d_count && (st = zero_u64(d_count, 1, s)) != FLS_OK) return st;
+    if (!(flags & FLS_Q_ACCUMULATE)) {
+        if ((st = zero_u64(reinterpret_cast<uint64_t*>(d_revenue), 1, s)) != FLS_OK) return st;
+        if (d_count && (st = zero_u64(d_count, 1, s)) != FLS_OK) return st;
+    }
     if (n_rows == 0) return FLS_OK;
     uint32_t grid = 0;
     if ((st = persistent_grid(a.n_groups, 8, &grid)) != FLS_OK) return st;
     const bool pred = (flags & FLS_Q_PRED_LOAD) != 0;
     const bool delta = a.date.enc == FLS_ENC_DELTA;
-    if (delta && a.qty.bw == 6 && a.disc.bw == 4 && a.price.bw == 24)
-        return launch<6, 4, 24, true>(a, pred, grid, s);
+    if (delta && a.qty.bw == 6 && a.disc.bw == 4 && a.price.bw == 24) return launch_fast<6, 4, 24>(a, pred, grid, s);
     if (delta) return launch<-1, -1, -1, true>(a, pred, grid, s);
     return launch<-1, -1, -1, false>(a, pred, grid, s);
 }

@@ -50,6 +50,22 @@ __device__ __forceinline__ uint32_t extract_row(const uint32_t (&w)[BW > 0 ? BW 
     }
 }
 
+// Same extraction with the row as an ordinary argument: inside a fully
+// unrolled row loop r is a constant and this folds to one v_bfe_u32 or one
+// v_alignbit_b32 + mask.
+template <int BW>
+__device__ __forceinline__ uint32_t extract_at(const uint32_t (&w)[BW > 0 ? BW : 1], int r) {
+    if constexpr (BW == 0) {
+        return 0u;
+    } else if constexpr (BW == 32) {
+        return w[r];
+    } else {
+        const int bit = r * BW, k = bit >> 5, s = bit & 31;
+        if (s + BW <= 32) return __builtin_amdgcn_ubfe(w[k], s, BW);
+        return __builtin_amdgcn_alignbit(w[k + 1], w[k], s) & Mask<BW>::value;
+    }
+}
+
 template <int BW, int R = 0>
 __device__ __forceinline__ void extract_rows(const uint32_t (&w)[BW > 0 ? BW : 1], uint32_t (&out)[32]) {
     if constexpr (R < 32) {

@@ -50,6 +50,10 @@ const char* fls_status_string(int status);
 int fls_last_hip_error(void);
 /* Library version, e.g. "fls-mi355x 0.1". */
 const char* fls_version(void);
+/* Plain streaming copy of n_bytes (multiple of 16, 16-byte aligned): the
+ * calibration kernel for bandwidth and HBM-counter measurements.  No
+ * counterpart in the reference.                                              */
+int fls_copy_bytes(const void* d_src, void* d_dst, uint64_t n_bytes, void* stream);
 
 /* ========================================================= host encoders (CPU)
  * One 1024-value vector.  Replaces
@@ -135,7 +139,8 @@ typedef struct fls_column {
  *   date_gt < orderdate < date_lt  AND  quantity < qty_lt  AND
  *   disc_lo <= discount <= disc_hi
  * contribute price*discount (64-bit product) to *d_revenue; *d_count (may be
- * NULL) gets the number of qualifying rows.  Both are overwritten.
+ * NULL) gets the number of qualifying rows.  Both are overwritten (zeroed by a
+ * small kernel first) unless flags has FLS_Q_ACCUMULATE.
  * orderdate may be DELTA or BITPACK; the others BITPACK.                     */
 typedef struct fls_q11_params {
     uint32_t date_gt, date_lt, qty_lt, disc_lo, disc_hi;
@@ -146,6 +151,10 @@ typedef struct fls_q11_params {
  * crystal-opt/load.cuh:25-97; fls_q11.cu:83-135).  Without it every byte of
  * every column is read (full scan).                                          */
 #define FLS_Q_PRED_LOAD 1u
+/* Add into *d_revenue / *d_count instead of overwriting them: the caller
+ * zeroes them first, as query<>() does with cudaMemset (fls_q11.cu:146-148).
+ * The call is then exactly one kernel launch.                               */
+#define FLS_Q_ACCUMULATE 2u
 
 int fls_query_q11(const fls_column* orderdate, const fls_column* quantity, const fls_column* discount,
                   const fls_column* price, uint64_t n_rows, const fls_q11_params* params, uint32_t flags,
@@ -154,6 +163,28 @@ int fls_query_q11(const fls_column* orderdate, const fls_column* quantity, const
 /* Encoded bytes a full scan of a column reads for n_rows (packed words, plus
  * per-vector metadata and bases for DELTA).                                   */
 uint64_t fls_column_bytes(const fls_column* col, uint64_t n_rows);
+
+/* ====================================================== synthetic workloads
+ * Deterministic generators computed on the GPU (no host data moves).  Every
+ * value is a pure function of (seed, column, row).  They have no counterpart
+ * in the reference, whose drivers read dbgen files (crystal_ssb_utils.h:108).
+ *   c1: n values uniform in [0, 2^bw).
+ *   c2: fact table of n rows: orderdate sorted and uniform over 1992-01-01 ..
+ *       1998-12-31 (row i on day floor(i*2557/n)), quantity U{1..50},
+ *       discount U{0..10}, price U[90000, 10^7).
+ *   c3: fact foreign keys as 0-based dense keys, date U[0,2556),
+ *       supplier U[0,40000), part U[0,800000), customer U[0,600000) (may be
+ *       NULL), revenue U[0,10^7); dimensions: year of each date row, supplier
+ *       region U{1..5} / nation U{1..25}, part category U{1..25} / brand
+ *       U{1..1000}, customer region / nation.                                */
+int fls_gen_c1(uint64_t seed, uint64_t n, uint32_t bw, uint32_t* d_out, void* stream);
+int fls_gen_c2(uint64_t seed, uint64_t n, uint32_t* d_orderdate, uint32_t* d_quantity, uint32_t* d_discount,
+               uint32_t* d_price, void* stream);
+int fls_gen_c3_fact(uint64_t seed, uint64_t n, uint32_t* d_datefk, uint32_t* d_suppfk, uint32_t* d_partfk,
+                    uint32_t* d_custfk, uint32_t* d_revenue, void* stream);
+int fls_gen_c3_dims(uint64_t seed, uint32_t n_date, uint32_t n_supp, uint32_t n_part, uint32_t n_cust,
+                    uint32_t* d_year, uint32_t* d_s_region, uint32_t* d_s_nation, uint32_t* d_p_category,
+                    uint32_t* d_p_brand, uint32_t* d_c_region, uint32_t* d_c_nation, void* stream);
 
 #ifdef __cplusplus
 }

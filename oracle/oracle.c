@@ -58,6 +58,13 @@ static unsigned transpose_pos(unsigned k) {
     return a + 16 * rev3(b) + 128 * c;
 }
 
+/* original index of transposed position p (inverse of transpose_pos; rev3 is
+ * its own inverse) */
+static unsigned transpose_pos_inv(unsigned p) {
+    unsigned a = p % 16, rb = (p / 16) % 8, c = p / 128;
+    return 64 * a + 8 * rev3(rb) + c;
+}
+
 void orc_transpose_i_u32(const uint32_t* in, uint32_t* out) {
     for (unsigned k = 0; k < 1024; ++k) out[transpose_pos(k)] = in[k];
 }
@@ -325,4 +332,61 @@ void orc_q21(const uint32_t* datefk, const uint32_t* suppfk, const uint32_t* par
         sums[slot] += (int64_t)revenue[i];
         counts[slot] += 1;
     }
+}
+
+/* ================================================================ CPU baseline */
+void orc_unpack_lanewise_u32(const uint32_t* in, uint32_t* out, unsigned bw) {
+    const uint32_t mask = bw >= 32 ? 0xffffffffu : ((1u << bw) - 1u);
+    for (unsigned l = 0; l < 32; ++l) {
+        for (unsigned r = 0; r < 32; ++r) {
+            if (bw == 0) {
+                out[32 * r + l] = 0;
+                continue;
+            }
+            unsigned bit = r * bw, w = bit / 32, s = bit % 32;
+            uint32_t x = in[32 * w + l] >> s;
+            if (s + bw > 32) x |= in[32 * (w + 1) + l] << (32 - s);
+            out[32 * r + l] = x & mask;
+        }
+    }
+}
+
+void orc_decode_vector(const orc_col* c, uint64_t v, uint32_t* out) {
+    if (c->encoding == 1) {
+        uint64_t m = c->vmeta[v];
+        uint32_t d[1024];
+        orc_unpack_lanewise_u32(c->words + (m >> 8), d, (unsigned)(m & 0xff));
+        orc_rsum_u32(d, out, c->bases + 32 * v);
+    } else {
+        orc_unpack_lanewise_u32(c->words + v * 32 * c->bw, out, c->bw);
+        if (c->base)
+            for (unsigned i = 0; i < 1024; ++i) out[i] += c->base;
+    }
+}
+
+int64_t orc_q11_encoded(const orc_col* od, const orc_col* q, const orc_col* d, const orc_col* p, uint64_t n_rows,
+                        uint64_t v_begin, uint64_t v_end, uint32_t date_gt, uint32_t date_lt, uint32_t qty_lt,
+                        uint32_t disc_lo, uint32_t disc_hi, int n_threads, uint64_t* count) {
+    int64_t sum = 0;
+    uint64_t cnt = 0;
+    const int tr = p->transposed != 0;
+#pragma omp parallel for num_threads(n_threads) reduction(+ : sum, cnt) schedule(static)
+    for (int64_t vv = (int64_t)v_begin; vv < (int64_t)v_end; ++vv) {
+        uint64_t v = (uint64_t)vv;
+        uint32_t a[1024], b[1024], c[1024], e[1024];
+        orc_decode_vector(od, v, a);
+        orc_decode_vector(q, v, b);
+        orc_decode_vector(d, v, c);
+        orc_decode_vector(p, v, e);
+        for (unsigned k = 0; k < 1024; ++k) {
+            uint64_t row = v * 1024 + (tr ? (uint64_t)transpose_pos_inv(k) : k);
+            if (row >= n_rows) continue;
+            if (a[k] > date_gt && a[k] < date_lt && b[k] < qty_lt && c[k] >= disc_lo && c[k] <= disc_hi) {
+                sum += (int64_t)e[k] * (int64_t)c[k];
+                cnt += 1;
+            }
+        }
+    }
+    if (count) *count = cnt;
+    return sum;
 }

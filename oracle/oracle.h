@@ -125,6 +125,28 @@ void orc_q21(const uint32_t* datefk, const uint32_t* suppfk, const uint32_t* par
              uint64_t n, const uint32_t* d_year, const uint32_t* s_region, const uint32_t* p_category,
              const uint32_t* p_brand, uint32_t cat, uint32_t reg, int64_t* sums, uint64_t* counts);
 
+/* ---------------------------------------------------------------- CPU baseline
+ * Decode + query over ENCODED columns, the CPU counterpart of the fused GPU
+ * kernels.  Per-lane scalar extraction in the style of the generated decoder
+ * (unpack.cuh:877-971: a word window, shift and mask per row), FastLanes rsum
+ * for delta, scalar predicate/aggregate loop; OpenMP over vector ranges.
+ * Pinned against the bit-by-bit functions above.                             */
+typedef struct orc_col {
+    uint32_t encoding;   /* 0 bitpack(+FOR base), 1 delta */
+    uint32_t bw;
+    uint32_t base;
+    uint32_t transposed;
+    const uint32_t* words;
+    const uint64_t* vmeta; /* (offset << 8) | bw */
+    const uint32_t* bases;
+} orc_col;
+
+void orc_unpack_lanewise_u32(const uint32_t* in, uint32_t* out, unsigned bw);
+void orc_decode_vector(const orc_col* c, uint64_t v, uint32_t* out);
+int64_t orc_q11_encoded(const orc_col* od, const orc_col* q, const orc_col* d, const orc_col* p, uint64_t n_rows,
+                        uint64_t v_begin, uint64_t v_end, uint32_t date_gt, uint32_t date_lt, uint32_t qty_lt,
+                        uint32_t disc_lo, uint32_t disc_hi, int n_threads, uint64_t* count);
+
 #ifdef __cplusplus
 }
 #endif

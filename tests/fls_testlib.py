@@ -177,3 +177,41 @@ def from_dev(t, dtype=np.uint32):
 
     torch.cuda.synchronize()
     return t.cpu().numpy().view(dtype)
+
+
+# ----------------------------------------------------------------- CPU baseline (oracle)
+class OrcCol(ctypes.Structure):
+    _fields_ = [("encoding", ctypes.c_uint32), ("bw", ctypes.c_uint32), ("base", ctypes.c_uint32),
+                ("transposed", ctypes.c_uint32), ("words", ctypes.c_void_p), ("vmeta", ctypes.c_void_p),
+                ("bases", ctypes.c_void_p)]
+
+
+def orc_col(hc):
+    """oracle view (orc_col) of a fastlanesgpu_amd.HostColumn; keeps arrays alive via the returned tuple."""
+    keep = [a for a in (hc.words, hc.vmeta, hc.bases) if a is not None]
+    ptr = lambda a: None if a is None else a.ctypes.data  # noqa: E731
+    c = OrcCol(hc.encoding, hc.bw, hc.base, int(hc.transposed), ptr(hc.words), ptr(hc.vmeta), ptr(hc.bases))
+    return c, keep
+
+
+def o_q11_encoded(cols, n_rows, v_begin=0, v_end=None, threads=8, params=(19930000, 19940000, 25, 1, 3)):
+    o = oracle()
+    if not hasattr(o.orc_q11_encoded, "_set"):
+        P = ctypes.POINTER(OrcCol)
+        o.orc_q11_encoded.argtypes = [P, P, P, P, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64] + \
+            [ctypes.c_uint32] * 5 + [ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]
+        o.orc_q11_encoded.restype = ctypes.c_int64
+        o.orc_q11_encoded._set = True
+    if v_end is None:
+        v_end = (n_rows + 1023) // 1024
+    cs = [orc_col(c) for c in cols]
+    cnt = ctypes.c_uint64(0)
+    s = o.orc_q11_encoded(*(ctypes.byref(c[0]) for c in cs), n_rows, v_begin, v_end, *params, threads,
+                          ctypes.byref(cnt))
+    return int(s), int(cnt.value)
+
+
+def o_q11(cols, n, params=(19930000, 19940000, 25, 1, 3)):
+    c = ctypes.c_uint64(0)
+    s = oracle().orc_q11(*[np.ascontiguousarray(x, np.uint32) for x in cols], n, *params, ctypes.byref(c))
+    return int(s), int(c.value)

@@ -87,3 +87,13 @@ def test_delta_decode(kind):
     # the first vector once more through the oracle's own delta decode
     bw = int(vmeta[0]) & 0xFF
     np.testing.assert_array_equal(T.from_dev(d_out)[:1024], T.o_delta_decode(words[:32 * bw], bw, bases[:32]))
+
+
+def test_copy_kernel():
+    torch = T.torch_cuda()
+    src = torch.arange(1 << 20, dtype=torch.int32, device="cuda")
+    dst = torch.zeros_like(src)
+    fls.call("fls_copy_bytes", src.data_ptr(), dst.data_ptr(), src.numel() * 4, fls.current_stream())
+    torch.cuda.synchronize()
+    assert torch.equal(src, dst)
+    assert fls.load().fls_copy_bytes(src.data_ptr(), dst.data_ptr(), 10, None) == -6
