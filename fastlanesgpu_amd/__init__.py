@@ -259,3 +259,136 @@ def encode_c2_table(orderdate, quantity, discount, price):
     bit-packed at their smallest widths after transpose_i (so their rows align
     with the delta-decoded orderdate)."""
     return [delta_column(orderdate)] + [bitpack_column(c, transposed=True) for c in (quantity, discount, price)]
+
+
+# ----------------------------------------------------------- Q2.1 shape
+from ._lib import Q21Dims, Q21Params, Q21Pht  # noqa: E402
+
+C3_DIMS = {"n_date": 2556, "n_supp": 40000, "n_part": 800000, "n_cust": 600000}
+
+
+def q21_params(category=12, region=2, year_base=1992, n_years=7, n_slots=7000):
+    return Q21Params(category, region, year_base, n_years, n_slots)
+
+
+def gen_c3_fact(seed, n, with_customer=False, stream=None):
+    """C3 fact columns on the device (0-based dense keys) -> dict of int32 tensors."""
+    import torch
+
+    names = ["datefk", "suppfk", "partfk", "custfk", "revenue"]
+    cols = {k: torch.empty(n, dtype=torch.int32, device="cuda") for k in names if with_customer or k != "custfk"}
+    call("fls_gen_c3_fact", seed, n, cols["datefk"].data_ptr(), cols["suppfk"].data_ptr(),
+         cols["partfk"].data_ptr(), cols["custfk"].data_ptr() if with_customer else None,
+         cols["revenue"].data_ptr(), _stream(stream))
+    return cols
+
+
+def gen_c3_dims(seed, n_date=2556, n_supp=40000, n_part=800000, n_cust=600000, stream=None):
+    """C3 dimension columns on the device -> dict of int32 tensors."""
+    import torch
+
+    e = lambda k: torch.empty(max(1, k), dtype=torch.int32, device="cuda")  # noqa: E731
+    d = {"d_year": e(n_date), "s_region": e(n_supp), "s_nation": e(n_supp), "p_category": e(n_part),
+         "p_brand": e(n_part), "c_region": e(n_cust), "c_nation": e(n_cust)}
+    call("fls_gen_c3_dims", seed, n_date, n_supp, n_part, n_cust,
+         *(d[k].data_ptr() for k in ("d_year", "s_region", "s_nation", "p_category", "p_brand", "c_region",
+                                     "c_nation")), _stream(stream))
+    d.update(n_date=n_date, n_supp=n_supp, n_part=n_part, n_cust=n_cust)
+    return d
+
+
+def key_column(fk0, n_keys):
+    """Frame-of-reference column of 1-based keys (key = fk0 + 1, base 1) at
+    bw = ceil(log2 n_keys): the packed values are the 0-based keys."""
+    fk0 = _u32(fk0)
+    bw = max(1, int(n_keys - 1).bit_length())
+    return HostColumn(ENC_BITPACK, fk0.size, pack_column(fk0, bw), bw=bw, base=1)
+
+
+def encode_c3_table(datefk, suppfk, partfk, revenue, n_date=2556, n_supp=40000, n_part=800000):
+    """C3 fact table: FOR-packed keys at ceil(log2 n) bits, revenue at its width."""
+    return [key_column(datefk, n_date), key_column(suppfk, n_supp), key_column(partfk, n_part),
+            bitpack_column(revenue)]
+
+
+def q21_dims_struct(d, keys_min=1):
+    return Q21Dims(d["s_region"].data_ptr(), d["n_supp"], d["p_category"].data_ptr(), d["p_brand"].data_ptr(),
+                   d["n_part"], d["d_year"].data_ptr(), d["n_date"], keys_min)
+
+
+def q21_build(dims, params=None, stream=None):
+    """Build the compact dimension workspace (device tensor) for query_q21."""
+    import torch
+
+    params = params or q21_params()
+    nbytes = int(load().fls_q21_workspace_bytes(dims["n_supp"], dims["n_part"], dims["n_date"]))
+    ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    ds = q21_dims_struct(dims)
+    call("fls_q21_build", ctypes.byref(ds), ctypes.byref(params), ws.data_ptr(), nbytes, _stream(stream))
+    return ws
+
+
+def _q21_out(params, d_out):
+    import torch
+
+    if d_out is None:
+        d_out = torch.zeros(2 * params.n_slots, dtype=torch.int64, device="cuda")
+    return d_out, d_out.data_ptr(), d_out.data_ptr() + 8 * params.n_slots
+
+
+def query_q21(date, supp, part, rev, n_rows, dims, ws, params=None, accumulate=False, d_out=None, stream=None):
+    """Fused decode + semi-joins + group-by over the compact workspace.
+    Returns the [sums(n_slots) | counts(n_slots)] int64 device tensor."""
+    params = params or q21_params()
+    d_out, ps, pc = _q21_out(params, d_out)
+    cols = [c.struct() for c in (date, supp, part, rev)]
+    ds = q21_dims_struct(dims)
+    call("fls_query_q21", *(ctypes.byref(c) for c in cols), n_rows, ctypes.byref(ds), ws.data_ptr(),
+         ctypes.byref(params), ps, pc, Q_ACCUMULATE if accumulate else 0, _stream(stream))
+    return d_out
+
+
+def build_pht(d_filter, filter_eq, d_keys, d_vals, n, d_ht, ht_len, keys_min, stream=None):
+    call("fls_build_pht_u32", dptr(d_filter), filter_eq, dptr(d_keys), dptr(d_vals), n, dptr(d_ht), ht_len,
+         keys_min, _stream(stream))
+
+
+def q21_build_pht(dims, params=None, stream=None):
+    """Reference-layout hash tables (build_hashtable_s/p/d) -> (Q21Pht, tensors)."""
+    import torch
+
+    params = params or q21_params()
+    z = lambda k: torch.zeros(max(1, k), dtype=torch.int32, device="cuda")  # noqa: E731
+    keys = {k: torch.arange(1, dims[k] + 1, dtype=torch.int32, device="cuda") for k in ("n_supp", "n_part", "n_date")}
+    ht_s, ht_p, ht_d = z(dims["n_supp"]), z(2 * dims["n_part"]), z(2 * dims["n_date"])
+    build_pht(dims["s_region"], params.region, keys["n_supp"], None, dims["n_supp"], ht_s, dims["n_supp"], 1, stream)
+    build_pht(dims["p_category"], params.category, keys["n_part"], dims["p_brand"], dims["n_part"], ht_p,
+              dims["n_part"], 1, stream)
+    build_pht(None, 0, keys["n_date"], dims["d_year"], dims["n_date"], ht_d, dims["n_date"], 1, stream)
+    pht = Q21Pht(ht_s.data_ptr(), dims["n_supp"], 1, ht_p.data_ptr(), dims["n_part"], 1, ht_d.data_ptr(),
+                 dims["n_date"], 1)
+    return pht, (ht_s, ht_p, ht_d)
+
+
+def query_q21_pht(date, supp, part, rev, n_rows, pht, params=None, accumulate=False, d_out=None, stream=None):
+    params = params or q21_params()
+    d_out, ps, pc = _q21_out(params, d_out)
+    cols = [c.struct() for c in (date, supp, part, rev)]
+    call("fls_query_q21_pht", *(ctypes.byref(c) for c in cols), n_rows, ctypes.byref(pht), ctypes.byref(params),
+         ps, pc, Q_ACCUMULATE if accumulate else 0, _stream(stream))
+    return d_out
+
+
+def q21_groups(d_out, params=None):
+    """Result rows (year, brand, sum) of the non-empty slots in slot order --
+    the compaction of runQuery (fls_q21.cu:341-347)."""
+    params = params or q21_params()
+    h = d_out.cpu().numpy()
+    sums, counts = h[: params.n_slots], h[params.n_slots:]
+    rows = []
+    for slot in np.nonzero(counts)[0]:
+        brand = int(slot) // params.n_years
+        if brand == 0:  # brand n_slots/n_years wraps onto slot 0..n_years-1
+            brand = params.n_slots // params.n_years
+        rows.append((params.year_base + int(slot) % params.n_years, brand, int(sums[slot])))
+    return rows

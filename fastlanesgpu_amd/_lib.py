@@ -101,3 +101,33 @@ def call(name, *args):
     st = getattr(lib, name)(*args)
     check(lib, name, st)
     return st
+
+
+class Q21Params(ctypes.Structure):
+    """struct fls_q21_params."""
+    _fields_ = [("category", c_u32), ("region", c_u32), ("year_base", c_u32), ("n_years", c_u32),
+                ("n_slots", c_u32)]
+
+
+class Q21Pht(ctypes.Structure):
+    """struct fls_q21_pht."""
+    _fields_ = [("ht_s", c_vp), ("s_len", c_u32), ("s_min", c_u32), ("ht_p", c_vp), ("p_len", c_u32),
+                ("p_min", c_u32), ("ht_d", c_vp), ("d_len", c_u32), ("d_min", c_u32)]
+
+
+class Q21Dims(ctypes.Structure):
+    """struct fls_q21_dims."""
+    _fields_ = [("s_region", c_vp), ("n_supp", c_u32), ("p_category", c_vp), ("p_brand", c_vp), ("n_part", c_u32),
+                ("d_year", c_vp), ("n_date", c_u32), ("keys_min", c_u32)]
+
+
+_Q21P = ctypes.POINTER(Q21Params)
+SIGNATURES.update({
+    "fls_build_pht_u32": (c_int, [c_vp, c_u32, c_vp, c_vp, c_u32, c_vp, c_u32, c_u32, c_vp]),
+    "fls_query_q21_pht": (c_int, [COLP, COLP, COLP, COLP, c_u64, ctypes.POINTER(Q21Pht), _Q21P, c_vp, c_vp, c_u32,
+                                  c_vp]),
+    "fls_q21_workspace_bytes": (c_u64, [c_u32, c_u32, c_u32]),
+    "fls_q21_build": (c_int, [ctypes.POINTER(Q21Dims), _Q21P, c_vp, c_u64, c_vp]),
+    "fls_query_q21": (c_int, [COLP, COLP, COLP, COLP, c_u64, ctypes.POINTER(Q21Dims), c_vp, _Q21P, c_vp, c_vp,
+                              c_u32, c_vp]),
+})

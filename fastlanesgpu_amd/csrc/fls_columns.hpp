@@ -24,6 +24,27 @@ int persistent_grid(uint64_t work, uint32_t blocks_per_cu, uint32_t* grid);
 // Number of compute units of the current device (cached per device).
 int device_cus(uint32_t* cus);
 
+// Grid of a persistent kernel: every block resident at once (CUs x the
+// kernel's occupancy at `block` work-items), capped by the work.  A larger
+// grid would leave the excess blocks to run as a tail after the first ones
+// retire.  The occupancy is queried once per kernel.
+template <typename Kernel>
+int resident_grid(Kernel kernel, int block, uint64_t work, uint32_t* grid) {
+    static int per_cu = 0;
+    if (!per_cu) {
+        int n = 0;
+        FLS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, block, 0));
+        per_cu = n > 0 ? n : 1;
+    }
+    uint32_t cus = 0;
+    int st = device_cus(&cus);
+    if (st) return st;
+    uint64_t g = (uint64_t)cus * (uint64_t)per_cu;
+    if (work < g) g = work;
+    *grid = (uint32_t)(g ? g : 1);
+    return FLS_OK;
+}
+
 // Fixed-width (BITPACK + FOR) column: BW >= 0 is a compile-time width that
 // must equal c.bw; BW < 0 dispatches on c.bw at run time.
 template <int BW>
@@ -45,6 +66,47 @@ __device__ __forceinline__ void decode_delta(const ColDev& c, uint64_t v, uint32
     const fls_vmeta m = c.vmeta[v];
     dev::unpack_device(c.words + FLS_VMETA_OFFSET(m), x, FLS_VMETA_BW(m));
     dev::rsum_lane(x, c.bases[v * 32u + dev::lane32()]);
+}
+
+// Keep the asm-visible value of each word live here: the loads that produced
+// them are then issued (and complete) before any data-dependent branch that
+// follows, so a full scan reads every byte even when a wave skips its ALU work.
+template <int N>
+__device__ __forceinline__ void pin_words(uint32_t (&w)[N]) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) asm volatile("" : "+v"(w[k]));
+}
+
+// One lane of one vector of a fixed-width column, held for row-by-row use.
+// BW >= 0: the packed words (BW registers), each row extracted on demand.
+// BW < 0: width known only at run time; the 32 rows are decoded up front.
+template <int BW>
+struct LaneCol {
+    uint32_t w[BW > 0 ? BW : (BW == 0 ? 1 : 32)];
+    uint32_t base;
+    __device__ __forceinline__ void load(const ColDev& c, uint64_t v) {
+        base = c.base;
+        if constexpr (BW >= 0) dev::load_lane<BW>(c.words + v * (32u * BW) + dev::lane32(), w);
+        else dev::unpack_device(c.words + v * (32u * c.bw), w, c.bw);
+    }
+    __device__ __forceinline__ uint32_t get(int r) const {
+        if constexpr (BW >= 0) return dev::extract_at<BW>(w, r) + base;
+        else return w[r] + base;
+    }
+    __device__ __forceinline__ void pin() { pin_words(w); }
+};
+
+// Row mask (bit r) of the rows of this lane that exist in a vector holding
+// `live` < 1024 rows, in normal or transposed row order.
+__device__ __forceinline__ uint32_t live_row_mask(uint32_t live, bool transposed) {
+    const uint32_t l = dev::lane32();
+    uint32_t m = 0;
+#pragma unroll
+    for (int r = 0; r < 32; ++r) {
+        const uint32_t p = l + 32u * r;
+        m |= (uint32_t)((transposed ? dev::untranspose_index(p) : p) < live) << r;
+    }
+    return m;
 }
 
 // Predicate-fused FastLanes delta walk of one lane at compile-time width BW:

@@ -139,15 +139,6 @@ __device__ __forceinline__ uint32_t valid_mask(uint64_t v, const Q11Dev& a, bool
     return m;
 }
 
-// Make the compiler treat the loaded words as consumed here, so their loads
-// are issued (and complete) before any data-dependent branch: a full scan
-// reads every byte even when the ALU work of a wave is skipped.
-template <int N>
-__device__ __forceinline__ void pin_words(uint32_t (&w)[N]) {
-#pragma unroll
-    for (int k = 0; k < N; ++k) asm volatile("" : "+v"(w[k]));
-}
-
 // Fast path: compile-time widths for quantity/discount/price, FastLanes delta
 // orderdate.  Live state per work-item: the packed words of the three
 // fixed-width columns (QBW+DBW+PBW VGPRs), one 32-bit date mask, accumulators.
@@ -222,18 +213,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     }
 }
 
-template <int QBW, int DBW, int PBW, bool DATE_DELTA>
-int launch(const Q11Dev& a, bool pred, uint32_t grid, hipStream_t s) {
-    if (pred) hipLaunchKernelGGL((q11_kernel<QBW, DBW, PBW, DATE_DELTA, true>), dim3(grid), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL((q11_kernel<QBW, DBW, PBW, DATE_DELTA, false>), dim3(grid), dim3(kBlock), 0, s, a);
+template <typename Kernel>
+int launch_resident(Kernel k, const Q11Dev& a, hipStream_t s) {
+    uint32_t grid = 0;
+    int st = resident_grid(k, kBlock, a.n_groups, &grid);
+    if (st) return st;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, a);
     return check_launch();
 }
 
+template <int QBW, int DBW, int PBW, bool DATE_DELTA>
+int launch(const Q11Dev& a, bool pred, hipStream_t s) {
+    if (pred) return launch_resident(q11_kernel<QBW, DBW, PBW, DATE_DELTA, true>, a, s);
+    return launch_resident(q11_kernel<QBW, DBW, PBW, DATE_DELTA, false>, a, s);
+}
+
 template <int QBW, int DBW, int PBW>
-int launch_fast(const Q11Dev& a, bool pred, uint32_t grid, hipStream_t s) {
-    if (pred) hipLaunchKernelGGL((q11_fast_kernel<QBW, DBW, PBW, true>), dim3(grid), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL((q11_fast_kernel<QBW, DBW, PBW, false>), dim3(grid), dim3(kBlock), 0, s, a);
-    return check_launch();
+int launch_fast(const Q11Dev& a, bool pred, hipStream_t s) {
+    if (pred) return launch_resident(q11_fast_kernel<QBW, DBW, PBW, true>, a, s);
+    return launch_resident(q11_fast_kernel<QBW, DBW, PBW, false>, a, s);
 }
 
 }  // namespace
@@ -270,11 +268,9 @@ extern "C" int fls_query_q11(const fls_column* orderdate, const fls_column* quan
         if (d_count && (st = zero_u64(d_count, 1, s)) != FLS_OK) return st;
     }
     if (n_rows == 0) return FLS_OK;
-    uint32_t grid = 0;
-    if ((st = persistent_grid(a.n_groups, 8, &grid)) != FLS_OK) return st;
     const bool pred = (flags & FLS_Q_PRED_LOAD) != 0;
     const bool delta = a.date.enc == FLS_ENC_DELTA;
-    if (delta && a.qty.bw == 6 && a.disc.bw == 4 && a.price.bw == 24) return launch_fast<6, 4, 24>(a, pred, grid, s);
-    if (delta) return launch<-1, -1, -1, true>(a, pred, grid, s);
-    return launch<-1, -1, -1, false>(a, pred, grid, s);
+    if (delta && a.qty.bw == 6 && a.disc.bw == 4 && a.price.bw == 24) return launch_fast<6, 4, 24>(a, pred, s);
+    if (delta) return launch<-1, -1, -1, true>(a, pred, s);
+    return launch<-1, -1, -1, false>(a, pred, s);
 }

@@ -1,0 +1,378 @@
+// Fused FastLanes decode + hash-probe + group-by, Q2.1 shape
+// (reference: probe_v1 + build_hashtable_s/p/d, fastlanes/src/ssb/fls_q21.cu:21-78,
+// 197-263, driver runQuery<BT,IPT>() fls_q21.cu:265-358).
+//
+// Two probe paths with identical results:
+//  * PHT path: the reference's perfect hash tables (crystal/join.cuh layout:
+//    PHT_1 = key per slot, PHT_2 = [key, value] pairs), built and probed with
+//    the Crystal primitives; drop-in for callers that already hold such tables.
+//  * compact path (MI355X-native): the build reduces each dimension to what
+//    the probe needs -- a bitmap of qualifying suppliers, a bitmap of
+//    qualifying parts, a 16-bit brand per part, a 16-bit year per date row --
+//    and the probe stages the bitmaps and the year table in LDS (about 110 KB
+//    at 40k suppliers / 800k parts / 2556 dates), so the two selective
+//    semi-joins cost an LDS read per row instead of a random HBM/L2 access.
+//    Brand and year are read only for rows that survive both filters.
+//
+// Both probe kernels are persistent, issue every fact column's loads before
+// any data-dependent branch (full scan), and aggregate with 64-bit atomics on
+// the n_slots-entry group table: slot = (brand*n_years + year-year_base) % n_slots
+// (fls_q21.cu:71).
+#include <string.h>
+
+#include "fls/device/crystal.hpp"
+#include "fls/device/unpack.hpp"
+#include "fls/device/wave.hpp"
+#include "fls_columns.hpp"
+#include "fls_runtime.hpp"
+
+namespace fls {
+namespace {
+
+// ------------------------------------------------------------ PHT builds
+constexpr int kBuildThreads = 128;
+constexpr int kBuildItems = 4;
+
+// build_hashtable_s / _p / _d (fls_q21.cu:197-263) in one kernel: optional
+// equality filter, PHT_1 (vals == nullptr) or PHT_2.
+__global__ __launch_bounds__(kBuildThreads) void build_pht_kernel(const uint32_t* __restrict filter, uint32_t eq,
+                                                                  const uint32_t* __restrict keys,
+                                                                  const uint32_t* __restrict vals, uint32_t n,
+                                                                  uint32_t* __restrict ht, uint32_t ht_len,
+                                                                  uint32_t keys_min) {
+    constexpr int tile = kBuildThreads * kBuildItems;
+    const uint32_t off = blockIdx.x * tile;
+    const int num = (int)min<uint32_t>(tile, n - off);
+    uint32_t items[kBuildItems], items2[kBuildItems];
+    int flags[kBuildItems];
+    InitFlags<kBuildThreads, kBuildItems>(flags);
+    if (filter) {
+        BlockLoad<uint32_t, kBuildThreads, kBuildItems>(filter + off, items, num);
+        BlockPredEQ<uint32_t, kBuildThreads, kBuildItems>(items, eq, flags, num);
+    }
+    BlockLoad<uint32_t, kBuildThreads, kBuildItems>(keys + off, items, num);
+    if (vals) {
+        BlockLoad<uint32_t, kBuildThreads, kBuildItems>(vals + off, items2, num);
+        BlockBuildSelectivePHT_2<uint32_t, uint32_t, kBuildThreads, kBuildItems>(items, items2, flags, ht,
+                                                                                 (int)ht_len, keys_min, num);
+    } else {
+        BlockBuildSelectivePHT_1<uint32_t, kBuildThreads, kBuildItems>(items, flags, ht, (int)ht_len, keys_min, num);
+    }
+}
+
+// ----------------------------------------------------------- compact build
+struct Q21Layout {
+    uint64_t supp_bm, part_bm, brand, year, total;  // byte offsets in the workspace
+    uint32_t supp_words, part_words;
+};
+
+Q21Layout q21_layout(uint32_t n_supp, uint32_t n_part, uint32_t n_date) {
+    auto up = [](uint64_t x) { return (x + 255u) & ~(uint64_t)255u; };
+    Q21Layout L{};
+    L.supp_words = (n_supp + 31u) / 32u;
+    L.part_words = (n_part + 31u) / 32u;
+    L.supp_bm = 0;
+    L.part_bm = up(L.supp_bm + 4ull * L.supp_words);
+    L.brand = up(L.part_bm + 4ull * L.part_words);
+    L.year = up(L.brand + 2ull * n_part);
+    L.total = up(L.year + 2ull * n_date);
+    return L;
+}
+
+__global__ void q21_zero_words(uint32_t* __restrict w, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) w[i] = 0u;
+}
+
+__global__ void q21_build_compact(const uint32_t* __restrict s_region, uint32_t n_supp,
+                                  const uint32_t* __restrict p_category, const uint32_t* __restrict p_brand,
+                                  uint32_t n_part, const uint32_t* __restrict d_year, uint32_t n_date, uint32_t region,
+                                  uint32_t category, uint32_t* __restrict supp_bm, uint32_t* __restrict part_bm,
+                                  uint16_t* __restrict brand, uint16_t* __restrict year) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_part; i += stride) {
+        const bool q = p_category[i] == category;
+        brand[i] = q ? (uint16_t)p_brand[i] : (uint16_t)0;
+        if (q) atomicOr(&part_bm[i >> 5], 1u << (i & 31u));
+    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_supp; i += stride)
+        if (s_region[i] == region) atomicOr(&supp_bm[i >> 5], 1u << (i & 31u));
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_date; i += stride) year[i] = (uint16_t)d_year[i];
+}
+
+// ------------------------------------------------------------ probe kernels
+struct Q21Dev {
+    ColDev date, supp, part, rev;
+    uint64_t n_rows, n_vec, n_groups;
+    // PHT path
+    const uint32_t *ht_s, *ht_p, *ht_d;
+    uint32_t s_len, p_len, d_len, s_min, p_min, d_min;
+    // compact path
+    const uint32_t *supp_bm, *part_bm;
+    const uint16_t *brand, *year;
+    uint32_t n_supp, n_part, n_date, keys_min;
+    uint32_t supp_words, part_words;
+    // group-by
+    uint32_t year_base, n_years, n_slots;
+    int64_t* sums;
+    uint64_t* counts;
+};
+
+__device__ __forceinline__ void group_add(const Q21Dev& a, uint32_t brand, uint32_t year, uint32_t revenue) {
+    const uint32_t slot = (brand * a.n_years + (year - a.year_base)) % a.n_slots;
+    atomicAdd((unsigned long long*)&a.sums[slot], (unsigned long long)revenue);
+    atomicAdd((unsigned long long*)&a.counts[slot], 1ull);
+}
+
+// Reference-layout probe (probe_v1, fls_q21.cu:21-78) through the Crystal
+// primitives, one 32-lane tile per FastLanes vector.
+template <int DBW, int SBW, int PBW, int RBW>
+__global__ __launch_bounds__(256) void q21_pht_kernel(const Q21Dev a) {
+    for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
+        const uint64_t v = g * 8u + (threadIdx.x >> 5);
+        if (v >= a.n_vec) continue;
+        const int num = (int)min<uint64_t>(1024u, a.n_rows - v * 1024u);
+        uint32_t items[32], brand[32], year[32];
+        int flags[32];
+        InitFlags<32, 32>(flags);
+        decode_fixed<SBW>(a.supp, v, items);
+        BlockProbeAndPHT_1<uint32_t, 32, 32>(items, flags, a.ht_s, (int)a.s_len, a.s_min, num);
+        decode_fixed<PBW>(a.part, v, items);
+        BlockProbeAndPHT_2<uint32_t, uint32_t, 32, 32>(items, brand, flags, a.ht_p, (int)a.p_len, a.p_min, num);
+        decode_fixed<DBW>(a.date, v, items);
+        BlockProbeAndPHT_2<uint32_t, uint32_t, 32, 32>(items, year, flags, a.ht_d, (int)a.d_len, a.d_min, num);
+        decode_fixed<RBW>(a.rev, v, items);
+        const int tid = dev::lane32();
+#pragma unroll
+        for (int i = 0; i < 32; ++i)
+            if (tid + 32 * i < num && flags[i]) group_add(a, brand[i], year[i], items[i]);
+    }
+}
+
+// Compact-table probe.  1024 work-items = 32 vectors per step; LDS holds the
+// supplier and part bitmaps and the year table when they fit (USE_LDS).
+template <int DBW, int SBW, int PBW, int RBW, bool USE_LDS>
+__global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t* supp_bm = a.supp_bm;
+    const uint32_t* part_bm = a.part_bm;
+    const uint16_t* year_tab = a.year;
+    if constexpr (USE_LDS) {
+        uint32_t* s = lds;
+        uint32_t* p = lds + a.supp_words;
+        uint16_t* y = reinterpret_cast<uint16_t*>(p + a.part_words);
+        for (uint32_t i = threadIdx.x; i < a.supp_words; i += blockDim.x) s[i] = a.supp_bm[i];
+        for (uint32_t i = threadIdx.x; i < a.part_words; i += blockDim.x) p[i] = a.part_bm[i];
+        for (uint32_t i = threadIdx.x; i < a.n_date; i += blockDim.x) y[i] = a.year[i];
+        __syncthreads();
+        supp_bm = s;
+        part_bm = p;
+        year_tab = y;
+    }
+    // rows of the (possibly partial) last vector, computed once
+    const uint32_t tail =
+        live_row_mask((uint32_t)(a.n_rows - (a.n_vec - 1) * 1024u), a.rev.transposed != 0);
+    for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
+        const uint64_t v = g * 32u + (threadIdx.x >> 5);
+        if (v >= a.n_vec) continue;
+        LaneCol<SBW> sc;
+        LaneCol<PBW> pc;
+        LaneCol<DBW> dc;
+        LaneCol<RBW> rc;
+        sc.load(a.supp, v);
+        pc.load(a.part, v);
+        dc.load(a.date, v);
+        rc.load(a.rev, v);
+        // supplier semi-join on every row
+        uint32_t m = 0;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            const uint32_t s = sc.get(r) - a.keys_min;
+            const uint32_t hit = s < a.n_supp ? (supp_bm[s >> 5] >> (s & 31u)) & 1u : 0u;
+            m |= hit << r;
+        }
+        if (v == a.n_vec - 1) m &= tail;
+        pc.pin();
+        dc.pin();
+        rc.pin();
+        if (!__any(m != 0)) continue;
+        // part semi-join on the surviving rows (rows no lane of the wave
+        // still holds are skipped as a whole)
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            if (__any((m >> r) & 1u)) {
+                const uint32_t p = pc.get(r) - a.keys_min;
+                const bool live = (m >> r) & 1u;
+                const uint32_t hit = (live && p < a.n_part) ? (part_bm[p >> 5] >> (p & 31u)) & 1u : 0u;
+                if (!hit) m &= ~(1u << r);
+            }
+        }
+        if (!__any(m != 0)) continue;
+        // group-by on the rows that passed both
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            if ((m >> r) & 1u) {
+                const uint32_t p = pc.get(r) - a.keys_min;
+                const uint32_t d = dc.get(r) - a.keys_min;
+                if (d < a.n_date) group_add(a, a.brand[p], year_tab[d], rc.get(r));
+            }
+        }
+    }
+}
+
+template <typename Kernel>
+int launch_resident_q21(Kernel k, int block, size_t lds, const Q21Dev& a, hipStream_t s) {
+    int per_cu = 0;
+    FLS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, lds));
+    if (per_cu < 1) per_cu = 1;
+    uint32_t cus = 0;
+    int st = device_cus(&cus);
+    if (st) return st;
+    uint64_t grid = (uint64_t)cus * per_cu;
+    if (grid > a.n_groups) grid = a.n_groups;
+    if (!grid) grid = 1;
+    hipLaunchKernelGGL(k, dim3((uint32_t)grid), dim3(block), lds, s, a);
+    return check_launch();
+}
+
+template <int DBW, int SBW, int PBW, int RBW>
+int launch_compact(const Q21Dev& a, bool in_lds, size_t lds, hipStream_t s) {
+    if (in_lds) {
+        auto k = q21_compact_kernel<DBW, SBW, PBW, RBW, true>;
+        FLS_HIP_CHECK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        return launch_resident_q21(k, 1024, lds, a, s);
+    }
+    return launch_resident_q21(q21_compact_kernel<DBW, SBW, PBW, RBW, false>, 1024, 0, a, s);
+}
+
+int fill_fact(Q21Dev& a, const fls_column* date, const fls_column* supp, const fls_column* part,
+              const fls_column* rev, uint64_t n_rows) {
+    if (!date || !supp || !part || !rev) return FLS_ERR_NULL;
+    int st;
+    if ((st = to_dev_column(*date, n_rows, &a.date, false)) != FLS_OK) return st;
+    if ((st = to_dev_column(*supp, n_rows, &a.supp, false)) != FLS_OK) return st;
+    if ((st = to_dev_column(*part, n_rows, &a.part, false)) != FLS_OK) return st;
+    if ((st = to_dev_column(*rev, n_rows, &a.rev, false)) != FLS_OK) return st;
+    if (date->transposed != supp->transposed || date->transposed != part->transposed ||
+        date->transposed != rev->transposed)
+        return FLS_ERR_ARG;
+    a.n_rows = n_rows;
+    a.n_vec = ceil_div(n_rows, 1024);
+    return FLS_OK;
+}
+
+int fill_groups(Q21Dev& a, const fls_q21_params* p, int64_t* d_sums, uint64_t* d_counts, uint32_t flags,
+                hipStream_t s) {
+    if (!p || !d_sums || !d_counts) return FLS_ERR_NULL;
+    if (p->n_slots == 0 || p->n_years == 0) return FLS_ERR_ARG;
+    a.year_base = p->year_base;
+    a.n_years = p->n_years;
+    a.n_slots = p->n_slots;
+    a.sums = d_sums;
+    a.counts = d_counts;
+    if (!(flags & FLS_Q_ACCUMULATE)) {
+        int st = zero_u64(reinterpret_cast<uint64_t*>(d_sums), p->n_slots, s);
+        if (st) return st;
+        return zero_u64(d_counts, p->n_slots, s);
+    }
+    return FLS_OK;
+}
+
+}  // namespace
+}  // namespace fls
+
+extern "C" {
+
+int fls_build_pht_u32(const uint32_t* d_filter, uint32_t filter_eq, const uint32_t* d_keys, const uint32_t* d_vals,
+                      uint32_t n, uint32_t* d_ht, uint32_t ht_len, uint32_t keys_min, void* stream) {
+    using namespace fls;
+    if (n == 0) return FLS_OK;
+    FLS_RETURN_IF(!d_keys || !d_ht, FLS_ERR_NULL);
+    FLS_RETURN_IF(ht_len == 0, FLS_ERR_ARG);
+    const uint32_t tile = kBuildThreads * kBuildItems;
+    hipLaunchKernelGGL(build_pht_kernel, dim3((n + tile - 1) / tile), dim3(kBuildThreads), 0, as_stream(stream),
+                       d_filter, filter_eq, d_keys, d_vals, n, d_ht, ht_len, keys_min);
+    return check_launch();
+}
+
+int fls_query_q21_pht(const fls_column* datekey, const fls_column* suppkey, const fls_column* partkey,
+                      const fls_column* revenue, uint64_t n_rows, const fls_q21_pht* ht, const fls_q21_params* params,
+                      int64_t* d_sums, uint64_t* d_counts, uint32_t flags, void* stream) {
+    using namespace fls;
+    FLS_RETURN_IF(!ht || !ht->ht_s || !ht->ht_p || !ht->ht_d, FLS_ERR_NULL);
+    FLS_RETURN_IF(!ht->s_len || !ht->p_len || !ht->d_len, FLS_ERR_ARG);
+    Q21Dev a{};
+    int st = fill_fact(a, datekey, suppkey, partkey, revenue, n_rows);
+    if (st) return st;
+    hipStream_t s = as_stream(stream);
+    if ((st = fill_groups(a, params, d_sums, d_counts, flags, s)) != FLS_OK) return st;
+    if (n_rows == 0) return FLS_OK;
+    a.ht_s = ht->ht_s;
+    a.ht_p = ht->ht_p;
+    a.ht_d = ht->ht_d;
+    a.s_len = ht->s_len;
+    a.p_len = ht->p_len;
+    a.d_len = ht->d_len;
+    a.s_min = ht->s_min;
+    a.p_min = ht->p_min;
+    a.d_min = ht->d_min;
+    a.n_groups = ceil_div(a.n_vec, 8);
+    if (a.date.bw == 12 && a.supp.bw == 16 && a.part.bw == 20 && a.rev.bw == 24)
+        return launch_resident_q21(q21_pht_kernel<12, 16, 20, 24>, 256, 0, a, s);
+    return launch_resident_q21(q21_pht_kernel<-1, -1, -1, -1>, 256, 0, a, s);
+}
+
+uint64_t fls_q21_workspace_bytes(uint32_t n_supp, uint32_t n_part, uint32_t n_date) {
+    return fls::q21_layout(n_supp, n_part, n_date).total;
+}
+
+int fls_q21_build(const fls_q21_dims* dims, const fls_q21_params* params, void* d_workspace, uint64_t ws_bytes,
+                  void* stream) {
+    using namespace fls;
+    FLS_RETURN_IF(!dims || !params || !d_workspace, FLS_ERR_NULL);
+    FLS_RETURN_IF(!dims->s_region || !dims->p_category || !dims->p_brand || !dims->d_year, FLS_ERR_NULL);
+    const Q21Layout L = q21_layout(dims->n_supp, dims->n_part, dims->n_date);
+    FLS_RETURN_IF(ws_bytes < L.total, FLS_ERR_CAPACITY);
+    char* ws = static_cast<char*>(d_workspace);
+    hipStream_t s = as_stream(stream);
+    const uint32_t words = (uint32_t)(L.brand / 4u);  // both bitmaps
+    hipLaunchKernelGGL(q21_zero_words, dim3(256), dim3(256), 0, s, reinterpret_cast<uint32_t*>(ws), words);
+    hipLaunchKernelGGL(q21_build_compact, dim3(1024), dim3(256), 0, s, dims->s_region, dims->n_supp,
+                       dims->p_category, dims->p_brand, dims->n_part, dims->d_year, dims->n_date, params->region,
+                       params->category, reinterpret_cast<uint32_t*>(ws + L.supp_bm),
+                       reinterpret_cast<uint32_t*>(ws + L.part_bm), reinterpret_cast<uint16_t*>(ws + L.brand),
+                       reinterpret_cast<uint16_t*>(ws + L.year));
+    return check_launch();
+}
+
+int fls_query_q21(const fls_column* datekey, const fls_column* suppkey, const fls_column* partkey,
+                  const fls_column* revenue, uint64_t n_rows, const fls_q21_dims* dims, const void* d_workspace,
+                  const fls_q21_params* params, int64_t* d_sums, uint64_t* d_counts, uint32_t flags, void* stream) {
+    using namespace fls;
+    FLS_RETURN_IF(!dims || !d_workspace, FLS_ERR_NULL);
+    Q21Dev a{};
+    int st = fill_fact(a, datekey, suppkey, partkey, revenue, n_rows);
+    if (st) return st;
+    hipStream_t s = as_stream(stream);
+    if ((st = fill_groups(a, params, d_sums, d_counts, flags, s)) != FLS_OK) return st;
+    if (n_rows == 0) return FLS_OK;
+    const Q21Layout L = q21_layout(dims->n_supp, dims->n_part, dims->n_date);
+    const char* ws = static_cast<const char*>(d_workspace);
+    a.supp_bm = reinterpret_cast<const uint32_t*>(ws + L.supp_bm);
+    a.part_bm = reinterpret_cast<const uint32_t*>(ws + L.part_bm);
+    a.brand = reinterpret_cast<const uint16_t*>(ws + L.brand);
+    a.year = reinterpret_cast<const uint16_t*>(ws + L.year);
+    a.n_supp = dims->n_supp;
+    a.n_part = dims->n_part;
+    a.n_date = dims->n_date;
+    a.keys_min = dims->keys_min;
+    a.supp_words = L.supp_words;
+    a.part_words = L.part_words;
+    a.n_groups = ceil_div(a.n_vec, 32);
+    const size_t lds = 4ull * (L.supp_words + L.part_words) + 2ull * dims->n_date + 16u;
+    const bool in_lds = lds <= 150u * 1024u;
+    const bool fast = a.date.bw == 12 && a.supp.bw == 16 && a.part.bw == 20 && a.rev.bw == 24;
+    if (fast) return launch_compact<12, 16, 20, 24>(a, in_lds, lds, s);
+    return launch_compact<-1, -1, -1, -1>(a, in_lds, lds, s);
+}
+
+}  // extern "C"

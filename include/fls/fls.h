@@ -160,6 +160,71 @@ int fls_query_q11(const fls_column* orderdate, const fls_column* quantity, const
                   const fls_column* price, uint64_t n_rows, const fls_q11_params* params, uint32_t flags,
                   int64_t* d_revenue, uint64_t* d_count, void* stream);
 
+/* ============================================ hash probe + group-by (Q2.1)
+ * Shape of fls_q21.cu:21-78: keep fact rows whose supplier passes one
+ * dimension filter and whose part passes another, look up the part's brand
+ * and the date's year, and sum revenue per group slot
+ *   slot = (brand * n_years + (year - year_base)) % n_slots    (fls_q21.cu:71)
+ * into d_sums[n_slots] (int64) with d_counts[n_slots] qualifying rows per
+ * slot; a group exists where its count is non-zero.  Both tables are
+ * overwritten (zeroed first) unless flags has FLS_Q_ACCUMULATE.            */
+typedef struct fls_q21_params {
+    uint32_t category;  /* part filter: p_category == category */
+    uint32_t region;    /* supplier filter: s_region == region */
+    uint32_t year_base; /* 1992 */
+    uint32_t n_years;   /* 7 */
+    uint32_t n_slots;   /* 7000 */
+} fls_q21_params;
+
+/* Reference perfect hash tables (crystal/join.cuh:6-275).  PHT_1 holds the key
+ * in slot HASH(key) = (key - min) % len, 0 meaning empty; PHT_2 holds
+ * [key, value] pairs.  ht_s: supplier PHT_1; ht_p: part PHT_2 -> brand; ht_d:
+ * date PHT_2 -> year.                                                         */
+typedef struct fls_q21_pht {
+    const uint32_t* ht_s;
+    uint32_t s_len, s_min;
+    const uint32_t* ht_p;
+    uint32_t p_len, p_min;
+    const uint32_t* ht_d;
+    uint32_t d_len, d_min;
+} fls_q21_pht;
+
+/* build_hashtable_s/_p/_d (fls_q21.cu:197-263): rows with
+ * d_filter[i] == filter_eq (every row when d_filter is NULL) enter the table
+ * as PHT_1 (d_vals NULL) or PHT_2 (value d_vals[i]).  d_ht must be zeroed by
+ * the caller (2*ht_len words for PHT_2), as runQuery does with cudaMemset
+ * (fls_q21.cu:294-297).  Keys must be non-zero.                              */
+int fls_build_pht_u32(const uint32_t* d_filter, uint32_t filter_eq, const uint32_t* d_keys, const uint32_t* d_vals,
+                      uint32_t n, uint32_t* d_ht, uint32_t ht_len, uint32_t keys_min, void* stream);
+/* probe_v1 (fls_q21.cu:21-78) over reference hash tables: drop-in path.      */
+int fls_query_q21_pht(const fls_column* datekey, const fls_column* suppkey, const fls_column* partkey,
+                      const fls_column* revenue, uint64_t n_rows, const fls_q21_pht* ht, const fls_q21_params* params,
+                      int64_t* d_sums, uint64_t* d_counts, uint32_t flags, void* stream);
+
+/* Dense-key dimension columns: row i holds key keys_min + i.                 */
+typedef struct fls_q21_dims {
+    const uint32_t* s_region;
+    uint32_t n_supp;
+    const uint32_t* p_category;
+    const uint32_t* p_brand;
+    uint32_t n_part;
+    const uint32_t* d_year;
+    uint32_t n_date;
+    uint32_t keys_min;
+} fls_q21_dims;
+
+/* MI355X-native path: fls_q21_build reduces the dimensions into a caller-owned
+ * workspace of fls_q21_workspace_bytes() bytes (bitmaps of qualifying
+ * suppliers and parts, 16-bit brand per part, 16-bit year per date);
+ * fls_query_q21 probes it with the bitmaps and the year table staged in LDS.
+ * Keys outside [keys_min, keys_min + n) do not match.                        */
+uint64_t fls_q21_workspace_bytes(uint32_t n_supp, uint32_t n_part, uint32_t n_date);
+int fls_q21_build(const fls_q21_dims* dims, const fls_q21_params* params, void* d_workspace, uint64_t ws_bytes,
+                  void* stream);
+int fls_query_q21(const fls_column* datekey, const fls_column* suppkey, const fls_column* partkey,
+                  const fls_column* revenue, uint64_t n_rows, const fls_q21_dims* dims, const void* d_workspace,
+                  const fls_q21_params* params, int64_t* d_sums, uint64_t* d_counts, uint32_t flags, void* stream);
+
 /* Encoded bytes a full scan of a column reads for n_rows (packed words, plus
  * per-vector metadata and bases for DELTA).                                   */
 uint64_t fls_column_bytes(const fls_column* col, uint64_t n_rows);

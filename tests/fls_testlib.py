@@ -215,3 +215,22 @@ def o_q11(cols, n, params=(19930000, 19940000, 25, 1, 3)):
     c = ctypes.c_uint64(0)
     s = oracle().orc_q11(*[np.ascontiguousarray(x, np.uint32) for x in cols], n, *params, ctypes.byref(c))
     return int(s), int(c.value)
+
+
+def o_gen_c3(seed, n, n_date=2556, n_supp=40000, n_part=800000, n_cust=600000):
+    o = oracle()
+    fact = [np.zeros(n, np.uint32) for _ in range(4)]
+    o.orc_gen_c3_fact(seed, n, fact[0], fact[1], fact[2], None, fact[3])
+    dims = [np.zeros(max(1, k), np.uint32) for k in (n_date, n_supp, n_supp, n_part, n_part, n_cust, n_cust)]
+    o.orc_gen_c3_dims(seed, n_date, n_supp, n_part, n_cust, *dims)
+    names = ["d_year", "s_region", "s_nation", "p_category", "p_brand", "c_region", "c_nation"]
+    return fact, dict(zip(names, dims))
+
+
+def o_q21(fact, dims, n, cat=12, reg=2):
+    sums = np.zeros(7000, np.int64)
+    counts = np.zeros(7000, np.uint64)
+    datefk, suppfk, partfk, rev = [np.ascontiguousarray(x, np.uint32) for x in fact]
+    oracle().orc_q21(datefk, suppfk, partfk, rev, n, dims["d_year"], dims["s_region"], dims["p_category"],
+                     dims["p_brand"], cat, reg, sums, counts)
+    return sums, counts

@@ -1,0 +1,99 @@
+"""GPU parity of the fused decode + hash-probe + group-by (Q2.1 shape) vs the oracle."""
+import numpy as np
+import pytest
+
+import fastlanesgpu_amd as fls
+import fls_testlib as T
+
+pytestmark = pytest.mark.gpu
+SMALL = dict(n_date=2556, n_supp=400, n_part=8000, n_cust=10)
+
+
+def dev_dims(dims_np, sizes):
+    torch = T.torch_cuda()
+    d = {k: torch.from_numpy(v.view(np.int32).copy()).cuda() for k, v in dims_np.items()}
+    d.update({k: v for k, v in sizes.items()})
+    return d
+
+
+def run_both(fact, dims_np, sizes, n, enc=None):
+    d = dev_dims(dims_np, sizes)
+    enc = enc or fls.encode_c3_table(*fact, n_date=sizes["n_date"], n_supp=sizes["n_supp"], n_part=sizes["n_part"])
+    cols = [c.to_device() for c in enc]
+    ws = fls.q21_build(d)
+    out_c = fls.query_q21(*cols, n, d, ws).cpu().numpy()
+    pht, keep = fls.q21_build_pht(d)
+    out_p = fls.query_q21_pht(*cols, n, pht).cpu().numpy()
+    del keep
+    return out_c, out_p
+
+
+def check(out, want):
+    sums, counts = want
+    np.testing.assert_array_equal(out[7000:].view(np.uint64), counts)
+    np.testing.assert_array_equal(out[:7000], sums)
+
+
+def test_generators_match_oracle():
+    n = 70_001
+    fact_d = fls.gen_c3_fact(5, n, with_customer=True)
+    dims_d = fls.gen_c3_dims(5, **{k: v // 10 for k, v in fls.C3_DIMS.items()})
+    fact_o, dims_o = T.o_gen_c3(5, n, **{k: v // 10 for k, v in fls.C3_DIMS.items()})
+    for k, want in zip(["datefk", "suppfk", "partfk", "revenue"], fact_o):
+        np.testing.assert_array_equal(fact_d[k].cpu().numpy().view(np.uint32), want)
+    for k, want in dims_o.items():
+        np.testing.assert_array_equal(dims_d[k].cpu().numpy().view(np.uint32)[: want.size], want)
+
+
+@pytest.mark.parametrize("n", [1, 700, 1024, 50_000 + 17])
+def test_q21_small(n):
+    fact, dims = T.o_gen_c3(11, n, **SMALL)
+    fact[1] %= SMALL["n_supp"]
+    fact[2] %= SMALL["n_part"]
+    want = T.o_q21(fact, dims, n)
+    out_c, out_p = run_both(fact, dims, SMALL, n)
+    check(out_c, want)
+    check(out_p, want)
+
+
+def test_q21_runtime_widths():
+    # keys packed wider than needed take the run-time-width kernels
+    n = 30_000
+    fact, dims = T.o_gen_c3(12, n, **SMALL)
+    fact[1] %= SMALL["n_supp"]
+    fact[2] %= SMALL["n_part"]
+    enc = [fls.HostColumn(fls.ENC_BITPACK, n, fls.pack_column(fact[0], 13), bw=13, base=1),
+           fls.HostColumn(fls.ENC_BITPACK, n, fls.pack_column(fact[1], 17), bw=17, base=1),
+           fls.HostColumn(fls.ENC_BITPACK, n, fls.pack_column(fact[2], 21), bw=21, base=1),
+           fls.bitpack_column(fact[3], 25)]
+    want = T.o_q21(fact, dims, n)
+    out_c, out_p = run_both(fact, dims, SMALL, n, enc)
+    check(out_c, want)
+    check(out_p, want)
+
+
+def test_q21_full_size_and_result_rows():
+    n = 120_000_000
+    fd = fls.gen_c3_fact(1, n)
+    fact = [fd[k].cpu().numpy().view(np.uint32) for k in ("datefk", "suppfk", "partfk", "revenue")]
+    del fd
+    dims_d = fls.gen_c3_dims(1)
+    dims_np = {k: v.cpu().numpy().view(np.uint32) for k, v in dims_d.items() if not k.startswith("n_")}
+    want = T.o_q21(fact, dims_np, n)
+    assert want[1].sum() > 800_000  # ~0.8 % of the rows survive
+    sizes = {k: fls.C3_DIMS[k] for k in ("n_date", "n_supp", "n_part")}
+    enc = fls.encode_c3_table(*fact)
+    assert [c.bw for c in enc] == [12, 16, 20, 24]
+    cols = [c.to_device() for c in enc]
+    ws = fls.q21_build(dims_d)
+    out = fls.query_q21(*cols, n, dims_d, ws)
+    check(out.cpu().numpy(), want)
+    pht, keep = fls.q21_build_pht(dims_d)
+    out_p = fls.query_q21_pht(*cols, n, pht)
+    check(out_p.cpu().numpy(), want)
+    # result rows in slot order, as runQuery compacts them
+    rows = fls.q21_groups(out)
+    assert len(rows) == int((want[1] > 0).sum())
+    slots = [(b * 7 + (y - 1992)) % 7000 for y, b, _ in rows]
+    assert slots == sorted(slots)
+    del sizes, keep
