@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--rows", type=int, default=120_000_000)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the secondary workloads")
     return ap.parse_args()
 
 
@@ -91,6 +92,59 @@ def copy_bandwidth(n_bytes=1 << 30, reps=20):
     torch.cuda.synchronize()
     ok = bool(torch.equal(src, dst))
     return 2 * n_bytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9, ok
+
+
+def bench_c1(steps, warmup, seed):
+    """Plain bit-unpack: 2^24 values at bw 11, decode + write + checksum."""
+    import torch
+
+    import fastlanesgpu_amd as fls
+
+    n, bw = 1 << 24, 11
+    vals = fls.gen_c1(seed, n, bw)
+    words = fls.pack_column(vals.cpu().numpy().view("uint32"), bw)
+    d_in = torch.from_numpy(words.view("int32")).cuda()
+    d_out = torch.empty(n, dtype=torch.int32, device="cuda")
+    d_sum = torch.zeros(1, dtype=torch.int64, device="cuda")
+    fn = lambda: fls.unpack_checksum(d_in, d_out, n, bw, d_sum)  # noqa: E731
+    _, per = time_kernel(fn, d_sum, steps, warmup)
+    ms = statistics.mean(per)
+    ok = torch.equal(d_out, vals)
+    bytes_moved = words.size * 4 + n * 4
+    return {"workload": "2^24 u32 uniform [0,2^11), bw 11: decode + write + checksum", "call_ms": ms,
+            "values_per_s": n / (ms * 1e-3), "GBs_read_plus_write": bytes_moved / (ms * 1e-3) / 1e9,
+            "decoded_equal": bool(ok), "note": "call = zero-fill kernel + decode kernel"}
+
+
+def bench_c3(steps, warmup, seed):
+    """Fused decode + semi-joins + group-by (Q2.1 shape), 120M rows, 4 dimension tables."""
+    import torch
+
+    import fastlanesgpu_amd as fls
+
+    n = 120_000_000
+    fd = fls.gen_c3_fact(seed, n)
+    fact = [fd[k].cpu().numpy().view("uint32") for k in ("datefk", "suppfk", "partfk", "revenue")]
+    del fd
+    enc = fls.encode_c3_table(*fact)
+    del fact
+    cols = [c.to_device() for c in enc]
+    nbytes = sum(c.nbytes() for c in enc)
+    dims = fls.gen_c3_dims(seed)
+    ws = fls.q21_build(dims)
+    out = torch.zeros(14000, dtype=torch.int64, device="cuda")
+    fn = lambda: fls.query_q21(*cols, n, dims, ws, accumulate=True, d_out=out)  # noqa: E731
+    _, per = time_kernel(fn, out, steps, warmup)
+    res_c = out.clone()
+    pht, keep = fls.q21_build_pht(dims)
+    fn2 = lambda: fls.query_q21_pht(*cols, n, pht, accumulate=True, d_out=out)  # noqa: E731
+    _, per2 = time_kernel(fn2, out, max(3, steps // 5), 2)
+    ms, ms2 = statistics.mean(per), statistics.mean(per2)
+    return {"workload": "fused decode+probe+group-by (Q2.1 shape), 120M rows; fks at 12/16/20 bits, revenue 24",
+            "encoded_bytes": nbytes, "kernel_ms_compact": ms, "rows_per_s_compact": n / (ms * 1e-3),
+            "GBs_compact": nbytes / (ms * 1e-3) / 1e9, "frac_hbm_compact": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "kernel_ms_pht": ms2, "rows_per_s_pht": n / (ms2 * 1e-3),
+            "groups": int((res_c[7000:] > 0).sum().item()), "paths_agree": bool(torch.equal(res_c, out))}
 
 
 def cpu_baseline(enc, n_rows, gpu_result):
@@ -220,6 +274,12 @@ def main():
     if not args.no_cpu:
         log("[bench] CPU baseline")
         line["cpu_baseline"] = cpu_baseline(enc, n, result)
+    del dev, enc
+    if not args.no_secondary:
+        log("[bench] secondary: plain unpack")
+        line["secondary"] = {"c1_unpack": bench_c1(args.steps, args.warmup, args.seed)}
+        log("[bench] secondary: probe + group-by")
+        line["secondary"]["c3_probe_groupby"] = bench_c3(max(5, args.steps // 2), 2, args.seed)
     print(json.dumps(line), flush=True)
 
 
