@@ -392,3 +392,61 @@ def q21_groups(d_out, params=None):
             brand = params.n_slots // params.n_years
         rows.append((params.year_base + int(slot) % params.n_years, brand, int(sums[slot])))
     return rows
+
+
+# ----------------------------------------------------------- GPU encoders
+def _dev_u32(n):
+    import torch
+
+    return torch.empty(max(1, n), dtype=torch.int32, device="cuda")
+
+
+def _dev_u64(n):
+    import torch
+
+    return torch.empty(max(1, n), dtype=torch.int64, device="cuda")
+
+
+def pack_column_device(d_in, n, bw, base=0, transposed=False, stream=None):
+    """GPU bit-packing of n values (minus base) -> int32 tensor of packed words."""
+    out = _dev_u32(packed_words(n, bw))
+    call("fls_pack_column_device", dptr(d_in), n, bw, base, ENC_FLAG_TRANSPOSE if transposed else 0, dptr(out),
+         _stream(stream))
+    return out
+
+
+def delta_encode_device(d_in, n, stream=None):
+    """GPU FastLanes delta: plan (widths, bases, offsets), then pack.
+    Returns (words, vmeta, bases) device tensors."""
+    import torch
+
+    n_vec = (n + VECTOR_SIZE - 1) // VECTOR_SIZE
+    vmeta, bases = _dev_u64(n_vec), _dev_u32(32 * n_vec)
+    total = torch.zeros(1, dtype=torch.int64, device="cuda")
+    scratch = torch.empty(max(16, int(load().fls_encode_scratch_bytes(n))), dtype=torch.uint8, device="cuda")
+    call("fls_delta_plan_device", dptr(d_in), n, dptr(vmeta), dptr(bases), dptr(total), dptr(scratch),
+         _stream(stream))
+    words = _dev_u32(int(total.item()))
+    call("fls_delta_pack_device", dptr(d_in), n, dptr(vmeta), dptr(words), _stream(stream))
+    return words[: max(1, int(total.item()))], vmeta[:max(1, n_vec)], bases
+
+
+def rle_encode_device(d_in, n, stream=None):
+    """GPU FastLanes RLE: plan then pack.  Returns (words, vmeta, rmeta, bases)
+    device tensors; rmeta holds one 8-byte fls_rle_meta per vector."""
+    import torch
+
+    n_vec = (n + VECTOR_SIZE - 1) // VECTOR_SIZE
+    vmeta, rmeta, bases = _dev_u64(n_vec), _dev_u64(n_vec), _dev_u32(32 * n_vec)
+    total = torch.zeros(1, dtype=torch.int64, device="cuda")
+    scratch = torch.empty(max(16, int(load().fls_encode_scratch_bytes(n))), dtype=torch.uint8, device="cuda")
+    call("fls_rle_plan_device", dptr(d_in), n, dptr(vmeta), dptr(rmeta), dptr(bases), dptr(total), dptr(scratch),
+         _stream(stream))
+    words = _dev_u32(int(total.item()))
+    call("fls_rle_pack_device", dptr(d_in), n, dptr(vmeta), dptr(rmeta), dptr(words), _stream(stream))
+    return words, vmeta, rmeta, bases
+
+
+def rle_decode(d_words, d_vmeta, d_rmeta, d_bases, n_vec, d_out, stream=None):
+    call("fls_rle_decode_u32", dptr(d_words), dptr(d_vmeta), dptr(d_rmeta), dptr(d_bases), n_vec, dptr(d_out),
+         _stream(stream))

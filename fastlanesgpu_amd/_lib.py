@@ -131,3 +131,13 @@ SIGNATURES.update({
     "fls_query_q21": (c_int, [COLP, COLP, COLP, COLP, c_u64, ctypes.POINTER(Q21Dims), c_vp, _Q21P, c_vp, c_vp,
                               c_u32, c_vp]),
 })
+
+SIGNATURES.update({
+    "fls_encode_scratch_bytes": (c_u64, [c_u64]),
+    "fls_pack_column_device": (c_int, [c_vp, c_u64, c_u32, c_u32, c_u32, c_vp, c_vp]),
+    "fls_delta_plan_device": (c_int, [c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "fls_delta_pack_device": (c_int, [c_vp, c_u64, c_vp, c_vp, c_vp]),
+    "fls_rle_plan_device": (c_int, [c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "fls_rle_pack_device": (c_int, [c_vp, c_u64, c_vp, c_vp, c_vp, c_vp]),
+    "fls_rle_decode_u32": (c_int, [c_vp, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp]),
+})

@@ -1,6 +1,10 @@
 // Column descriptor validation and launch-shape helpers.
 #include "fls_columns.hpp"
 
+#include <mutex>
+#include <utility>
+#include <vector>
+
 namespace fls {
 
 int device_cus(uint32_t* cus) {
@@ -15,6 +19,46 @@ int device_cus(uint32_t* cus) {
         cached[dev] = n;
     }
     *cus = (uint32_t)cached[dev];
+    return FLS_OK;
+}
+
+int blocks_per_cu(const void* kernel, int block, size_t lds, int* per_cu) {
+    struct Key {
+        const void* k;
+        int block;
+        size_t lds;
+        int dev;
+    };
+    static std::mutex mu;
+    static std::vector<std::pair<Key, int>> cache;
+    int dev = 0;
+    FLS_HIP_CHECK(hipGetDevice(&dev));
+    {
+        std::lock_guard<std::mutex> g(mu);
+        for (const auto& e : cache)
+            if (e.first.k == kernel && e.first.block == block && e.first.lds == lds && e.first.dev == dev) {
+                *per_cu = e.second;
+                return FLS_OK;
+            }
+    }
+    int n = 0;
+    FLS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, block, lds));
+    if (n < 1) n = 1;
+    std::lock_guard<std::mutex> g(mu);
+    cache.push_back({Key{kernel, block, lds, dev}, n});
+    *per_cu = n;
+    return FLS_OK;
+}
+
+int resident_grid_of(const void* kernel, int block, size_t lds, uint64_t work, uint32_t* grid) {
+    int per_cu = 0;
+    int st = blocks_per_cu(kernel, block, lds, &per_cu);
+    if (st) return st;
+    uint32_t cus = 0;
+    if ((st = device_cus(&cus)) != FLS_OK) return st;
+    uint64_t g = (uint64_t)cus * (uint64_t)per_cu;
+    if (work < g) g = work;
+    *grid = (uint32_t)(g ? g : 1);
     return FLS_OK;
 }
 

@@ -24,25 +24,18 @@ int persistent_grid(uint64_t work, uint32_t blocks_per_cu, uint32_t* grid);
 // Number of compute units of the current device (cached per device).
 int device_cus(uint32_t* cus);
 
+// Resident blocks per CU of `kernel` at `block` work-items and `lds` bytes of
+// dynamic LDS, queried once per (kernel, block, lds) and cached.
+int blocks_per_cu(const void* kernel, int block, size_t lds, int* per_cu);
+
 // Grid of a persistent kernel: every block resident at once (CUs x the
-// kernel's occupancy at `block` work-items), capped by the work.  A larger
-// grid would leave the excess blocks to run as a tail after the first ones
-// retire.  The occupancy is queried once per kernel.
+// kernel's occupancy), capped by the work.  A larger grid would leave the
+// excess blocks to run as a tail after the first ones retire.
+int resident_grid_of(const void* kernel, int block, size_t lds, uint64_t work, uint32_t* grid);
+
 template <typename Kernel>
 int resident_grid(Kernel kernel, int block, uint64_t work, uint32_t* grid) {
-    static int per_cu = 0;
-    if (!per_cu) {
-        int n = 0;
-        FLS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, block, 0));
-        per_cu = n > 0 ? n : 1;
-    }
-    uint32_t cus = 0;
-    int st = device_cus(&cus);
-    if (st) return st;
-    uint64_t g = (uint64_t)cus * (uint64_t)per_cu;
-    if (work < g) g = work;
-    *grid = (uint32_t)(g ? g : 1);
-    return FLS_OK;
+    return resident_grid_of(reinterpret_cast<const void*>(kernel), block, 0, work, grid);
 }
 
 // Fixed-width (BITPACK + FOR) column: BW >= 0 is a compile-time width that
@@ -107,6 +100,42 @@ __device__ __forceinline__ uint32_t live_row_mask(uint32_t live, bool transposed
         m |= (uint32_t)((transposed ? dev::untranspose_index(p) : p) < live) << r;
     }
     return m;
+}
+
+// Value k of a linear LSB-first bit-packing at width bw (bw <= 32).
+__device__ __forceinline__ uint32_t linear_extract(const uint32_t* __restrict w, uint32_t k, uint32_t bw) {
+    if (bw == 0) return 0u;
+    const uint32_t bit = k * bw, i = bit >> 5, s = bit & 31u;
+    const uint32_t lo = w[i];
+    const uint32_t hi = (s + bw > 32u) ? w[i + 1] : 0u;
+    const uint32_t mask = bw >= 32 ? 0xffffffffu : ((1u << bw) - 1u);
+    return __builtin_amdgcn_alignbit(hi, lo, s) & mask;
+}
+
+// FastLanes RLE decode of vector v, one lane, into 32 rows (transposed order).
+// `stage` is this vector's 1024-entry LDS slice; the 32 lanes of the vector
+// first unpack the run values into it (about 4 per lane at 128 runs), then
+// each row reads its value by run index.  All 64 work-items of the wave must
+// call it (it waits on the wave's own LDS writes only).
+__device__ __forceinline__ void decode_rle(const ColDev& c, const fls_rle_meta* __restrict rmeta, uint64_t v,
+                                           uint32_t* __restrict stage, bool live, uint32_t (&x)[32]) {
+    fls_vmeta m = 0;
+    fls_rle_meta rm{};
+    if (live) {
+        m = c.vmeta[v];
+        rm = rmeta[v];
+        const uint32_t* vals = c.words + FLS_VMETA_OFFSET(m) + 32u * FLS_VMETA_BW(m);
+        for (uint32_t k = dev::lane32(); k < rm.n_runs; k += 32u) stage[k] = rm.vbase + linear_extract(vals, k, rm.vbw);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (!live) return;
+    dev::unpack_device(c.words + FLS_VMETA_OFFSET(m), x, FLS_VMETA_BW(m));
+    dev::rsum_lane(x, c.bases[v * 32u + dev::lane32()]);
+    const uint32_t last = rm.n_runs ? rm.n_runs - 1u : 0u;
+#pragma unroll
+    for (int r = 0; r < 32; ++r) x[r] = stage[min(x[r], last)];
 }
 
 // Predicate-fused FastLanes delta walk of one lane at compile-time width BW:

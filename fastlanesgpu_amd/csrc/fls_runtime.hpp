@@ -21,7 +21,7 @@ inline int check_launch() {
     return e == hipSuccess ? FLS_OK : record_hip_error(e);
 }
 
-inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+__host__ __device__ inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 // Zero n 64-bit words on the stream with a kernel (not hipMemsetAsync, so the
 // call is safe inside captured graphs).

@@ -7,6 +7,7 @@
 // coalesced dword stores.
 #include "fls/device/unpack.hpp"
 #include "fls/device/wave.hpp"
+#include "fls_columns.hpp"
 #include "fls_runtime.hpp"
 
 namespace fls {
@@ -20,10 +21,14 @@ template <int BW, bool WRITE, bool CHECKSUM>
 __global__ __launch_bounds__(kBlock) void unpack_global_kernel(const uint32_t* __restrict in,
                                                                uint32_t* __restrict out, uint64_t n_vec,
                                                                uint64_t n_values, uint64_t* __restrict checksum) {
-    const uint64_t v = (uint64_t)blockIdx.x * kVecPerBlock + (threadIdx.x >> 5);
     const uint32_t l = dev::lane32();
     uint64_t acc = 0;
-    if (v < n_vec) {
+    // grid-stride over groups of 8 vectors: with the checksum the grid is the
+    // resident set, so each block ends with a single 64-bit atomic
+    const uint64_t n_groups = ceil_div(n_vec, kVecPerBlock);
+    for (uint64_t g = blockIdx.x; g < n_groups; g += gridDim.x) {
+        const uint64_t v = g * kVecPerBlock + (threadIdx.x >> 5);
+        if (v >= n_vec) continue;
         uint32_t vals[32];
         dev::unpack_lane<BW>(in + v * (32u * BW), vals);
         if constexpr (WRITE) {
@@ -52,12 +57,20 @@ __global__ __launch_bounds__(kBlock) void unpack_global_kernel(const uint32_t* _
                     s1 += (uint64_t)x * (uint32_t)r;
                 }
             }
-            acc = first * s0 + 32u * s1;
+            acc += first * s0 + 32u * s1;
         }
     }
     if constexpr (CHECKSUM) {
+        __shared__ uint64_t s_part[kBlock / 64];
         acc = dev::wave_sum(acc);
-        if (dev::is_wave_leader() && acc != 0) atomicAdd((unsigned long long*)checksum, (unsigned long long)acc);
+        if (dev::is_wave_leader()) s_part[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t t = 0;
+#pragma unroll
+            for (int w = 0; w < kBlock / 64; ++w) t += s_part[w];
+            if (t) atomicAdd((unsigned long long*)checksum, (unsigned long long)t);
+        }
     }
 }
 
@@ -78,6 +91,24 @@ __global__ __launch_bounds__(kBlock) void delta_decode_kernel(const uint32_t* __
     for (int r = 0; r < 32; ++r) __builtin_nontemporal_store(vals[r], o + 32 * r);
 }
 
+// FastLanes RLE decode (run values staged per vector in LDS), transposed order.
+__global__ __launch_bounds__(kBlock) void rle_decode_kernel(const uint32_t* __restrict words,
+                                                            const fls_vmeta* __restrict vmeta,
+                                                            const fls_rle_meta* __restrict rmeta,
+                                                            const uint32_t* __restrict bases, uint64_t n_vec,
+                                                            uint32_t* __restrict out) {
+    __shared__ uint32_t stage[kVecPerBlock][1024];
+    const uint64_t v = (uint64_t)blockIdx.x * kVecPerBlock + (threadIdx.x >> 5);
+    const bool live = v < n_vec;
+    const ColDev c{FLS_ENC_RLE, 0, 0, 1, words, vmeta, bases};
+    uint32_t x[32];
+    decode_rle(c, rmeta, v, stage[threadIdx.x >> 5], live, x);
+    if (!live) return;
+    uint32_t* o = out + v * 1024u + dev::lane32();
+#pragma unroll
+    for (int r = 0; r < 32; ++r) __builtin_nontemporal_store(x[r], o + 32 * r);
+}
+
 template <bool WRITE, bool CHECKSUM>
 struct UnpackTable {
     using Fn = void (*)(const uint32_t*, uint32_t*, uint64_t, uint64_t, uint64_t*);
@@ -92,9 +123,15 @@ template <bool WRITE, bool CHECKSUM>
 int launch_unpack(const uint32_t* in, uint32_t* out, uint64_t n_vec, uint64_t n_values, uint32_t bw,
                   uint64_t* checksum, hipStream_t stream) {
     if (n_vec == 0) return FLS_OK;
-    const uint64_t blocks = ceil_div(n_vec, kVecPerBlock);
+    uint64_t blocks = ceil_div(n_vec, kVecPerBlock);
     FLS_RETURN_IF(blocks > 0x7fffffffu, FLS_ERR_SIZE);
     auto fn = UnpackTable<WRITE, CHECKSUM>::table[bw];
+    if constexpr (CHECKSUM) {
+        uint32_t grid = 0;
+        int st = resident_grid_of(reinterpret_cast<const void*>(fn), kBlock, 0, blocks, &grid);
+        if (st) return st;
+        blocks = grid;
+    }
     hipLaunchKernelGGL(fn, dim3((uint32_t)blocks), dim3(kBlock), 0, stream, in, out, n_vec, n_values, checksum);
     return check_launch();
 }
@@ -131,6 +168,17 @@ int fls_delta_decode_u32(const uint32_t* d_words, const fls_vmeta* d_vmeta, cons
     FLS_RETURN_IF(blocks > 0x7fffffffu, FLS_ERR_SIZE);
     hipLaunchKernelGGL(fls::delta_decode_kernel, dim3((uint32_t)blocks), dim3(fls::kBlock), 0,
                        fls::as_stream(stream), d_words, d_vmeta, d_bases, n_vec, d_out);
+    return fls::check_launch();
+}
+
+int fls_rle_decode_u32(const uint32_t* d_words, const fls_vmeta* d_vmeta, const fls_rle_meta* d_rmeta,
+                       const uint32_t* d_bases, uint64_t n_vec, uint32_t* d_out, void* stream) {
+    if (n_vec == 0) return FLS_OK;
+    FLS_RETURN_IF(!d_words || !d_vmeta || !d_rmeta || !d_bases || !d_out, FLS_ERR_NULL);
+    const uint64_t blocks = fls::ceil_div(n_vec, fls::kVecPerBlock);
+    FLS_RETURN_IF(blocks > 0x7fffffffu, FLS_ERR_SIZE);
+    hipLaunchKernelGGL(fls::rle_decode_kernel, dim3((uint32_t)blocks), dim3(fls::kBlock), 0, fls::as_stream(stream),
+                       d_words, d_vmeta, d_rmeta, d_bases, n_vec, d_out);
     return fls::check_launch();
 }
 

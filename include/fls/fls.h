@@ -96,6 +96,41 @@ typedef uint64_t fls_vmeta;
 int fls_delta_encode_column_u32(const uint32_t* in, uint64_t n, uint32_t* out, uint64_t out_cap,
                                 fls_vmeta* vmeta, uint32_t* bases, uint64_t* out_words);
 
+/* FastLanes RLE of one vector (project format; the reference's RLE is a CPU
+ * sketch, fastlanes/src/include/fls_gen/rle/rle.hpp:8-43): the vector's runs
+ * become
+ *   - a run index per row (0-based run number), FastLanes-delta coded exactly
+ *     like a delta column: 32 bases (index at each lane's chain start) and
+ *     per-row deltas of width ibw (0 or 1) packed at the vector's offset;
+ *   - the run values minus vbase, linear LSB-first bit-packing at vbw bits,
+ *     right after the 32*ibw index words.
+ * vmeta holds (offset, ibw); fls_rle_meta the rest.  Decoding yields
+ * transposed row order, like delta.                                           */
+typedef struct fls_rle_meta {
+    uint32_t vbase;  /* smallest run value */
+    uint16_t n_runs; /* runs in the vector (<= 1024) */
+    uint8_t vbw;     /* width of run value - vbase */
+    uint8_t ibw;     /* width of the run-index deltas (0 or 1) */
+} fls_rle_meta;
+
+/* ============================================================ GPU encoders
+ * Device buffers in, device buffers out, on `stream`.  Variable-width
+ * encodings run as plan (per-vector widths, bases, sizes and the offsets'
+ * prefix sum into vmeta; the total word count into *d_total_words) then pack
+ * (into a buffer of that many words).  d_scratch needs
+ * fls_encode_scratch_bytes(n) bytes.                                          */
+uint64_t fls_encode_scratch_bytes(uint64_t n);
+/* pack (value - base) at bw for n values; FLS_ENC_FLAG_TRANSPOSE as above     */
+int fls_pack_column_device(const uint32_t* d_in, uint64_t n, uint32_t bw, uint32_t base, uint32_t flags,
+                           uint32_t* d_out, void* stream);
+int fls_delta_plan_device(const uint32_t* d_in, uint64_t n, fls_vmeta* d_vmeta, uint32_t* d_bases,
+                          uint64_t* d_total_words, void* d_scratch, void* stream);
+int fls_delta_pack_device(const uint32_t* d_in, uint64_t n, const fls_vmeta* d_vmeta, uint32_t* d_out, void* stream);
+int fls_rle_plan_device(const uint32_t* d_in, uint64_t n, fls_vmeta* d_vmeta, fls_rle_meta* d_rmeta,
+                        uint32_t* d_bases, uint64_t* d_total_words, void* d_scratch, void* stream);
+int fls_rle_pack_device(const uint32_t* d_in, uint64_t n, const fls_vmeta* d_vmeta, const fls_rle_meta* d_rmeta,
+                        uint32_t* d_out, void* stream);
+
 /* ========================================================== GPU decode (gfx950)
  * Grid-level decode of n_vec fixed-width vectors into normal order
  * out[v*1024 + 32*r + l].  Replaces __global__ unpack_global(in, out, bw)
@@ -111,6 +146,11 @@ int fls_unpack_u32_checksum(const uint32_t* d_in, uint32_t* d_out, uint64_t n_va
  * in transposed order (as fastlanes_bench_delta.cu:183-191 checks it).        */
 int fls_delta_decode_u32(const uint32_t* d_words, const fls_vmeta* d_vmeta, const uint32_t* d_bases,
                          uint64_t n_vec, uint32_t* d_out, void* stream);
+/* FastLanes RLE decode (format at fls_rle_meta), transposed order.  No
+ * reference kernel: the reference's RLE exists only as a CPU sketch
+ * (fls_gen/rle/rle.hpp:8-43) and an unused bit-unpack (hardcoded_16.cuh:1210). */
+int fls_rle_decode_u32(const uint32_t* d_words, const fls_vmeta* d_vmeta, const fls_rle_meta* d_rmeta,
+                       const uint32_t* d_bases, uint64_t n_vec, uint32_t* d_out, void* stream);
 
 /* ================================================================ columns
  * A compressed column on the device.  All columns read by one query must use
