@@ -450,3 +450,100 @@ def rle_encode_device(d_in, n, stream=None):
 def rle_decode(d_words, d_vmeta, d_rmeta, d_bases, n_vec, d_out, stream=None):
     call("fls_rle_decode_u32", dptr(d_words), dptr(d_vmeta), dptr(d_rmeta), dptr(d_bases), n_vec, dptr(d_out),
          _stream(stream))
+
+
+# ----------------------------------------------------------- batched segments
+from ._lib import BatchInfo, Segment, SegmentInfo, Tune  # noqa: E402
+
+SCHED_STATIC, SCHED_DYNAMIC = 0, 1
+
+
+class Batch:
+    """Owner of an fls_batch handle: many compressed segments decoded and
+    range-filtered in one persistent launch (plus graph, autotuner, baseline)."""
+
+    def __init__(self, handle, keep=None):
+        self._h = handle
+        self._keep = keep  # device buffers a caller-described batch borrows
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        i = BatchInfo()
+        call("fls_batch_get_info", self._h, ctypes.byref(i))
+        return {k: getattr(i, k) for k, _ in BatchInfo._fields_}
+
+    def segment(self, s):
+        i = SegmentInfo()
+        call("fls_batch_segment", self._h, s, ctypes.byref(i))
+        return {k: getattr(i, k) for k, _ in SegmentInfo._fields_}
+
+    def run(self, tune, d_counts, stream=None):
+        call("fls_batch_run", self._h, ctypes.byref(tune), dptr(d_counts), _stream(stream))
+
+    def run_per_segment(self, tune, d_counts, stream=None):
+        call("fls_batch_run_per_segment", self._h, ctypes.byref(tune), dptr(d_counts), _stream(stream))
+
+    def graph_launch(self, tune, d_counts, stream=None):
+        call("fls_batch_graph_launch", self._h, ctypes.byref(tune), dptr(d_counts), _stream(stream))
+
+    def autotune(self, d_counts, cache_path=None, force=False, stream=None):
+        t = Tune()
+        ms = ctypes.c_double(0)
+        call("fls_batch_autotune", self._h, cache_path.encode() if cache_path else None, int(force), dptr(d_counts),
+             ctypes.byref(t), ctypes.byref(ms), _stream(stream))
+        return t, ms.value
+
+    def close(self):
+        if self._h:
+            load().fls_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # pragma: no cover
+            pass
+
+
+def tune(tile_vectors=64, block_threads=256, schedule=SCHED_STATIC):
+    return Tune(tile_vectors, block_threads, schedule, 0)
+
+
+def batch_c4(seed, n_segments=4096, log2_min_len=10, log2_max_len=24, stream=None):
+    """Build the batched-segments workload on the GPU (see fls_batch_create_c4)."""
+    h = ctypes.c_void_p()
+    call("fls_batch_create_c4", seed, n_segments, log2_min_len, log2_max_len, ctypes.byref(h), _stream(stream))
+    return Batch(h.value)
+
+
+def c4_segment_params(seed, s, log2_min_len=10, log2_max_len=24):
+    enc, bw, n = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+    load().fls_c4_segment_params(seed, s, log2_min_len, log2_max_len, ctypes.byref(enc), ctypes.byref(bw),
+                                 ctypes.byref(n))
+    return enc.value, bw.value, n.value
+
+
+def gen_c4_segment(seed, s, log2_min_len=10, log2_max_len=24, stream=None):
+    import torch
+
+    n = c4_segment_params(seed, s, log2_min_len, log2_max_len)[2]
+    out = torch.empty(n, dtype=torch.int32, device="cuda")
+    call("fls_gen_c4_segment", seed, s, log2_min_len, log2_max_len, out.data_ptr(), _stream(stream))
+    return out
+
+
+def batch_from_segments(segments):
+    """segments: list of dicts {col: Column, rmeta: tensor|None, n_values, lo, hi}."""
+    arr = (Segment * len(segments))()
+    keep = []
+    for i, s in enumerate(segments):
+        arr[i].col = s["col"].struct()
+        arr[i].rmeta = dptr(s.get("rmeta"))
+        arr[i].n_values, arr[i].lo, arr[i].hi = s["n_values"], s["lo"], s["hi"]
+        keep.append(s)
+    h = ctypes.c_void_p()
+    call("fls_batch_create", arr, len(segments), ctypes.byref(h))
+    return Batch(h.value, keep)

@@ -80,6 +80,14 @@ def load(path=None):
     p = pathlib.Path(path) if path else pathlib.Path(os.environ.get("FLS_LIB", LIB_PATH))
     if not p.exists():
         raise FileNotFoundError(f"{p} is not built: run `make -C {_HERE}` (or __graft_entry__.build())")
+    # torch wheels ship their own libamdhip64.so.7.  Loading torch first makes
+    # libfls.so bind to that already-loaded runtime, so the process has one
+    # HIP runtime; loading libfls.so first would pull in the system copy and
+    # leave two runtimes competing for the device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:  # pragma: no cover - C/C++ hosts do not need torch
+        pass
     lib = ctypes.CDLL(str(p))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)  # AttributeError if the export is missing
@@ -140,4 +148,42 @@ SIGNATURES.update({
     "fls_rle_plan_device": (c_int, [c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "fls_rle_pack_device": (c_int, [c_vp, c_u64, c_vp, c_vp, c_vp, c_vp]),
     "fls_rle_decode_u32": (c_int, [c_vp, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp]),
+})
+
+
+class Tune(ctypes.Structure):
+    """struct fls_tune."""
+    _fields_ = [("tile_vectors", c_u32), ("block_threads", c_u32), ("schedule", c_u32), ("reserved", c_u32)]
+
+
+class BatchInfo(ctypes.Structure):
+    """struct fls_batch_info."""
+    _fields_ = [("n_segments", c_u32), ("n_values", c_u64), ("n_vectors", c_u64), ("encoded_bytes", c_u64),
+                ("n_bitpack", c_u32), ("n_delta", c_u32), ("n_rle", c_u32)]
+
+
+class SegmentInfo(ctypes.Structure):
+    """struct fls_segment_info."""
+    _fields_ = [("encoding", c_u32), ("bw", c_u32), ("n_values", c_u64), ("lo", c_u32), ("hi", c_u32),
+                ("n_words", c_u64)]
+
+
+class Segment(ctypes.Structure):
+    """struct fls_segment."""
+    _fields_ = [("col", FlsColumn), ("rmeta", c_vp), ("n_values", c_u64), ("lo", c_u32), ("hi", c_u32)]
+
+
+_TP = ctypes.POINTER(Tune)
+SIGNATURES.update({
+    "fls_batch_create": (c_int, [ctypes.POINTER(Segment), c_u32, ctypes.POINTER(c_vp)]),
+    "fls_batch_create_c4": (c_int, [c_u64, c_u32, c_u32, c_u32, ctypes.POINTER(c_vp), c_vp]),
+    "fls_c4_segment_params": (None, [c_u64, c_u32, c_u32, c_u32, c_u32p, c_u32p, c_u64p]),
+    "fls_gen_c4_segment": (c_int, [c_u64, c_u32, c_u32, c_u32, c_vp, c_vp]),
+    "fls_batch_get_info": (c_int, [c_vp, ctypes.POINTER(BatchInfo)]),
+    "fls_batch_segment": (c_int, [c_vp, c_u32, ctypes.POINTER(SegmentInfo)]),
+    "fls_batch_run": (c_int, [c_vp, _TP, c_vp, c_vp]),
+    "fls_batch_run_per_segment": (c_int, [c_vp, _TP, c_vp, c_vp]),
+    "fls_batch_graph_launch": (c_int, [c_vp, _TP, c_vp, c_vp]),
+    "fls_batch_autotune": (c_int, [c_vp, ctypes.c_char_p, c_int, c_vp, _TP, ctypes.POINTER(ctypes.c_double), c_vp]),
+    "fls_batch_destroy": (None, [c_vp]),
 })

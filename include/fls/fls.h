@@ -269,6 +269,74 @@ int fls_query_q21(const fls_column* datekey, const fls_column* suppkey, const fl
  * per-vector metadata and bases for DELTA).                                   */
 uint64_t fls_column_bytes(const fls_column* col, uint64_t n_rows);
 
+/* ============================================ batched segments (no reference)
+ * Many compressed segments of uneven length, width and encoding decoded and
+ * range-filtered (lo <= value <= hi) in ONE persistent launch, counting the
+ * qualifying values per segment into d_counts[n_segments] (overwritten).
+ * The reference launches one kernel per query column set; this has no
+ * counterpart there.                                                         */
+typedef struct fls_batch fls_batch; /* opaque */
+
+typedef struct fls_segment {
+    fls_column col;             /* BITPACK, DELTA or RLE (device buffers) */
+    const fls_rle_meta* rmeta;  /* RLE only */
+    uint64_t n_values;
+    uint32_t lo, hi;            /* range predicate */
+} fls_segment;
+
+typedef struct fls_segment_info {
+    uint32_t encoding, bw;
+    uint64_t n_values;
+    uint32_t lo, hi;
+    uint64_t n_words;
+} fls_segment_info;
+
+typedef struct fls_batch_info {
+    uint32_t n_segments;
+    uint64_t n_values, n_vectors, encoded_bytes;
+    uint32_t n_bitpack, n_delta, n_rle;
+} fls_batch_info;
+
+enum { FLS_SCHED_STATIC = 0, FLS_SCHED_DYNAMIC = 1 };
+/* launch shape: vectors per tile (a tile never spans segments), work-items
+ * per block (256 or 512), tiles handed out round-robin or from a work queue */
+typedef struct fls_tune {
+    uint32_t tile_vectors, block_threads, schedule, reserved;
+} fls_tune;
+
+/* Segments the caller already holds on the device (pointers are borrowed). */
+int fls_batch_create(const fls_segment* segs, uint32_t n_segments, fls_batch** out);
+/* The batched-segments workload, generated and encoded on the GPU: segment s
+ * has length 2^e + U[0,2^e) with octave e uniform in [log2_min, log2_max)
+ * (log-uniform by octave), bw U{1..32}, encoding 40% BITPACK (values
+ * U[0,2^bw)), 30% DELTA (sorted: value_i = i*m + U[0,m), m = 2^(b-1),
+ * b = min(bw, 32 - bitwidth(len))), 30% RLE (runs starting with probability
+ * 1/8 per row, i.e. geometric lengths of mean 8, run values U[0,2^bw)); the
+ * predicate is the middle half of the segment's [min, max]:
+ * lo = min + (max-min)/4, hi = min + 3(max-min)/4.                           */
+int fls_batch_create_c4(uint64_t seed, uint32_t n_segments, uint32_t log2_min_len, uint32_t log2_max_len,
+                        fls_batch** out, void* stream);
+void fls_c4_segment_params(uint64_t seed, uint32_t s, uint32_t log2_min_len, uint32_t log2_max_len,
+                           uint32_t* encoding, uint32_t* bw, uint64_t* n_values);
+int fls_gen_c4_segment(uint64_t seed, uint32_t s, uint32_t log2_min_len, uint32_t log2_max_len, uint32_t* d_out,
+                       void* stream);
+int fls_batch_get_info(const fls_batch* b, fls_batch_info* info);
+int fls_batch_segment(const fls_batch* b, uint32_t s, fls_segment_info* info);
+/* one persistent launch (after two tiny zero-fill kernels) */
+int fls_batch_run(fls_batch* b, const fls_tune* tune, uint64_t* d_counts, void* stream);
+/* baseline: one launch per segment */
+int fls_batch_run_per_segment(fls_batch* b, const fls_tune* tune, uint64_t* d_counts, void* stream);
+/* fls_batch_run captured once in a hipGraph (re-captured when tune or
+ * d_counts change) and replayed on `stream`                                  */
+int fls_batch_graph_launch(fls_batch* b, const fls_tune* tune, uint64_t* d_counts, void* stream);
+/* time every (tile, block, schedule) candidate on this batch, keep the best;
+ * results are cached in cache_path (NULL: $FLS_AUTOTUNE_CACHE, else
+ * ~/.cache/fls_mi355x/autotune.txt) under a key of device + batch shape and
+ * reused unless force != 0.  *best_ms = -1 when the cache answered.          */
+int fls_batch_autotune(fls_batch* b, const char* cache_path, int force, uint64_t* d_counts, fls_tune* best,
+                       double* best_ms, void* stream);
+void fls_batch_destroy(fls_batch* b);
+
 /* ====================================================== synthetic workloads
  * Deterministic generators computed on the GPU (no host data moves).  Every
  * value is a pure function of (seed, column, row).  They have no counterpart

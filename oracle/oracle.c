@@ -390,3 +390,62 @@ int64_t orc_q11_encoded(const orc_col* od, const orc_col* q, const orc_col* d, c
     if (count) *count = cnt;
     return sum;
 }
+
+/* =========================================================== batched segments */
+enum { ST_SEG_PARAMS = 1000, ST_SEG_VALUES = 100000, ST_SEG_RUNS = 200000 };
+
+void orc_c4_params(uint64_t seed, uint32_t s, uint32_t lmin, uint32_t lmax, uint32_t* enc, uint32_t* bw,
+                   uint64_t* len) {
+    uint32_t e = lmin + orc_uniform(seed, ST_SEG_PARAMS + s, 0, lmax - lmin);
+    *len = (1ULL << e) + orc_uniform(seed, ST_SEG_PARAMS + s, 1, 1ULL << e);
+    *bw = 1 + orc_uniform(seed, ST_SEG_PARAMS + s, 2, 32);
+    uint32_t r = orc_uniform(seed, ST_SEG_PARAMS + s, 3, 10);
+    *enc = r < 4 ? 0 : (r < 7 ? 1 : 2); /* bitpack, delta, RLE */
+}
+
+void orc_c4_values(uint64_t seed, uint32_t s, uint32_t enc, uint32_t bw, uint64_t len, uint32_t* out) {
+    if (enc == 0) {
+        for (uint64_t i = 0; i < len; ++i) out[i] = orc_uniform(seed, ST_SEG_VALUES + s, i, 1ULL << bw);
+    } else if (enc == 1) {
+        /* sorted: i*m + U[0,m) with m = 2^(b-1), b = min(bw, 32 - bitwidth(len)) */
+        unsigned lw = 0;
+        for (uint64_t x = len; x; x >>= 1) ++lw;
+        unsigned b = lw < 32 ? 32 - lw : 1;
+        if (b < 1) b = 1;
+        if (bw < b) b = bw;
+        uint64_t m = 1ULL << (b - 1);
+        for (uint64_t i = 0; i < len; ++i) out[i] = (uint32_t)(i * m) + orc_uniform(seed, ST_SEG_VALUES + s, i, m);
+    } else {
+        uint32_t cur = 0;
+        for (uint64_t i = 0; i < len; ++i) {
+            if (i == 0 || orc_uniform(seed, ST_SEG_RUNS + s, i, 8) == 0)
+                cur = orc_uniform(seed, ST_SEG_VALUES + s, i, 1ULL << bw);
+            out[i] = cur;
+        }
+    }
+}
+
+void orc_c4_counts(uint64_t seed, uint32_t n_seg, uint32_t lmin, uint32_t lmax, uint64_t* counts, uint32_t* lo,
+                   uint32_t* hi, int n_threads) {
+#pragma omp parallel for num_threads(n_threads) schedule(dynamic, 1)
+    for (int64_t ss = 0; ss < (int64_t)n_seg; ++ss) {
+        uint32_t s = (uint32_t)ss, enc, bw;
+        uint64_t len;
+        orc_c4_params(seed, s, lmin, lmax, &enc, &bw, &len);
+        uint32_t* v = (uint32_t*)malloc(sizeof(uint32_t) * len);
+        orc_c4_values(seed, s, enc, bw, len, v);
+        uint32_t mn = 0xffffffffu, mx = 0;
+        for (uint64_t i = 0; i < len; ++i) {
+            if (v[i] < mn) mn = v[i];
+            if (v[i] > mx) mx = v[i];
+        }
+        uint64_t range = (uint64_t)(mx - mn);
+        uint32_t l = mn + (uint32_t)(range / 4), h = mn + (uint32_t)(range * 3 / 4);
+        uint64_t c = 0;
+        for (uint64_t i = 0; i < len; ++i) c += (v[i] >= l && v[i] <= h);
+        counts[s] = c;
+        lo[s] = l;
+        hi[s] = h;
+        free(v);
+    }
+}

@@ -125,6 +125,17 @@ void orc_q21(const uint32_t* datefk, const uint32_t* suppfk, const uint32_t* par
              uint64_t n, const uint32_t* d_year, const uint32_t* s_region, const uint32_t* p_category,
              const uint32_t* p_brand, uint32_t cat, uint32_t reg, int64_t* sums, uint64_t* counts);
 
+/* ---------------------------------------------------------------- batched segments
+ * Definition of the batched-segments workload (the GPU builder generates the
+ * same values from the same seed): parameters of segment s, its values (RLE
+ * walked sequentially here), and per-segment counts of values inside the
+ * middle half of the segment's [min, max].                                   */
+void orc_c4_params(uint64_t seed, uint32_t s, uint32_t lmin, uint32_t lmax, uint32_t* enc, uint32_t* bw,
+                   uint64_t* len);
+void orc_c4_values(uint64_t seed, uint32_t s, uint32_t enc, uint32_t bw, uint64_t len, uint32_t* out);
+void orc_c4_counts(uint64_t seed, uint32_t n_seg, uint32_t lmin, uint32_t lmax, uint64_t* counts, uint32_t* lo,
+                   uint32_t* hi, int n_threads);
+
 /* ---------------------------------------------------------------- CPU baseline
  * Decode + query over ENCODED columns, the CPU counterpart of the fused GPU
  * kernels.  Per-lane scalar extraction in the style of the generated decoder

@@ -234,3 +234,30 @@ def o_q21(fact, dims, n, cat=12, reg=2):
     oracle().orc_q21(datefk, suppfk, partfk, rev, n, dims["d_year"], dims["s_region"], dims["p_category"],
                      dims["p_brand"], cat, reg, sums, counts)
     return sums, counts
+
+
+def o_c4_counts(seed, n_seg, lmin, lmax, threads=16):
+    o = oracle()
+    if not hasattr(o.orc_c4_counts, "_set"):
+        o.orc_c4_counts.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, U64P, U32P,
+                                    U32P, ctypes.c_int]
+        o.orc_c4_values.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                    ctypes.c_uint64, U32P]
+        o.orc_c4_params.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32] + \
+            [ctypes.POINTER(ctypes.c_uint32)] * 2 + [ctypes.POINTER(ctypes.c_uint64)]
+        o.orc_c4_counts._set = True
+    counts = np.zeros(n_seg, np.uint64)
+    lo = np.zeros(n_seg, np.uint32)
+    hi = np.zeros(n_seg, np.uint32)
+    o.orc_c4_counts(seed, n_seg, lmin, lmax, counts, lo, hi, threads)
+    return counts, lo, hi
+
+
+def o_c4_values(seed, s, lmin, lmax):
+    o_c4_counts(seed, 1, lmin, lmax, 1)  # sets argtypes
+    o = oracle()
+    enc, bw, n = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+    o.orc_c4_params(seed, s, lmin, lmax, ctypes.byref(enc), ctypes.byref(bw), ctypes.byref(n))
+    out = np.zeros(n.value, np.uint32)
+    o.orc_c4_values(seed, s, enc.value, bw.value, n.value, out)
+    return enc.value, bw.value, out

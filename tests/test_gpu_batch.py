@@ -1,0 +1,124 @@
+"""Batched decode + range filter over uneven segments (one persistent launch,
+per-segment baseline, hipGraph replay, autotuner) against the oracle."""
+import os
+
+import numpy as np
+import pytest
+
+import fastlanesgpu_amd as fls
+import fls_testlib as T
+
+pytestmark = pytest.mark.gpu
+
+
+def counts_of(t):
+    return T.from_dev(t, np.uint64)
+
+
+def test_segment_generator_matches_oracle():
+    for s in (0, 1, 2, 3, 5, 8, 13):
+        enc, bw, want = T.o_c4_values(31, s, 10, 16)
+        assert fls.c4_segment_params(31, s, 10, 16)[:2] == (enc, bw)
+        got = fls.gen_c4_segment(31, s, 10, 16)
+        np.testing.assert_array_equal(T.from_dev(got, np.uint32), want)
+
+
+@pytest.mark.parametrize("tile", [1, 8, 64, 512])
+def test_small_batch_all_paths(tile):
+    torch = T.torch_cuda()
+    n_seg, lmin, lmax = 96, 10, 16
+    want, lo, hi = T.o_c4_counts(7, n_seg, lmin, lmax)
+    b = fls.batch_c4(7, n_seg, lmin, lmax)
+    info = b.info()
+    assert info["n_segments"] == n_seg and info["n_bitpack"] + info["n_delta"] + info["n_rle"] == n_seg
+    assert min(info["n_bitpack"], info["n_delta"], info["n_rle"]) > 0
+    for s in range(n_seg):
+        si = b.segment(s)
+        assert (si["lo"], si["hi"]) == (lo[s], hi[s])
+    d = torch.zeros(n_seg, dtype=torch.int64, device="cuda")
+    for block in (256, 512):
+        for sched in (fls.SCHED_STATIC, fls.SCHED_DYNAMIC):
+            t = fls.tune(tile, block, sched)
+            d.fill_(-1)
+            b.run(t, d)
+            np.testing.assert_array_equal(counts_of(d), want)
+            d.fill_(-1)
+            b.run_per_segment(t, d)
+            np.testing.assert_array_equal(counts_of(d), want)
+
+
+def test_graph_replays():
+    torch = T.torch_cuda()
+    n_seg = 40
+    want, _, _ = T.o_c4_counts(9, n_seg, 10, 17)
+    b = fls.batch_c4(9, n_seg, 10, 17)
+    d = torch.zeros(n_seg, dtype=torch.int64, device="cuda")
+    t = fls.tune(32, 256, fls.SCHED_DYNAMIC)
+    for rep in range(4):
+        d.fill_(-7)  # eager work between replays; the graph must re-zero
+        torch.cuda.synchronize()
+        b.graph_launch(t, d)
+        np.testing.assert_array_equal(counts_of(d), want, err_msg=f"replay {rep}")
+    t2 = fls.tune(8, 512, fls.SCHED_STATIC)  # new shape: re-captured
+    b.graph_launch(t2, d)
+    np.testing.assert_array_equal(counts_of(d), want)
+
+
+def test_autotune_and_cache(tmp_path):
+    torch = T.torch_cuda()
+    n_seg = 48
+    want, _, _ = T.o_c4_counts(11, n_seg, 10, 16)
+    b = fls.batch_c4(11, n_seg, 10, 16)
+    d = torch.zeros(n_seg, dtype=torch.int64, device="cuda")
+    cache = str(tmp_path / "tune.txt")
+    best, ms = b.autotune(d, cache_path=cache)
+    assert ms > 0 and best.tile_vectors > 0 and os.path.exists(cache)
+    best2, ms2 = b.autotune(d, cache_path=cache)
+    assert ms2 == -1.0 and (best2.tile_vectors, best2.block_threads, best2.schedule) == \
+        (best.tile_vectors, best.block_threads, best.schedule)
+    b.run(best, d)
+    np.testing.assert_array_equal(counts_of(d), want)
+
+
+def test_caller_described_segments():
+    torch = T.torch_cuda()
+    rng = np.random.default_rng(3)
+    segs, want = [], []
+    specs = [("bitpack", 5000, 9), ("delta", 70_000, 0), ("rle", 33_333, 14), ("bitpack", 1, 3), ("rle", 1024, 1)]
+    for kind, n, bw in specs:
+        if kind == "bitpack":
+            x = rng.integers(0, 2**bw, n).astype(np.uint32)
+            col = fls.Column(fls.ENC_BITPACK, n, fls.pack_column_device(T.to_dev(x), n, bw), bw=bw)
+            rmeta = None
+        elif kind == "delta":
+            x = np.sort(rng.integers(0, 2**30, n)).astype(np.uint32)
+            w, vm, bs = fls.delta_encode_device(T.to_dev(x), n)
+            col = fls.Column(fls.ENC_DELTA, n, w, vmeta=vm, bases=bs)
+            rmeta = None
+        else:
+            starts = rng.random(n) < 0.125
+            starts[0] = True
+            x = rng.integers(0, 2**bw, n).astype(np.uint32)[np.cumsum(starts) - 1]
+            w, vm, rm, bs = fls.rle_encode_device(T.to_dev(x), n)
+            col = fls.Column(fls.ENC_RLE, n, w, vmeta=vm, bases=bs)
+            rmeta = rm
+        lo, hi = int(np.percentile(x, 20)), int(np.percentile(x, 70))
+        segs.append({"col": col, "rmeta": rmeta, "n_values": n, "lo": lo, "hi": hi})
+        want.append(int(((x >= lo) & (x <= hi)).sum()))
+    b = fls.batch_from_segments(segs)
+    d = torch.zeros(len(segs), dtype=torch.int64, device="cuda")
+    b.run(fls.tune(4, 256, fls.SCHED_DYNAMIC), d)
+    np.testing.assert_array_equal(counts_of(d), np.array(want, np.uint64))
+
+
+def test_full_batch_4096_segments():
+    torch = T.torch_cuda()
+    n_seg = 4096
+    b = fls.batch_c4(1, n_seg, 10, 24)
+    info = b.info()
+    assert info["n_values"] > 5_000_000_000
+    d = torch.zeros(n_seg, dtype=torch.int64, device="cuda")
+    b.run(fls.tune(64, 256, fls.SCHED_DYNAMIC), d)
+    got = counts_of(d).copy()
+    want, lo, hi = T.o_c4_counts(1, n_seg, 10, 24, threads=min(16, os.cpu_count() or 1))
+    np.testing.assert_array_equal(got, want)
