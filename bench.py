@@ -147,6 +147,57 @@ def bench_c3(steps, warmup, seed):
             "groups": int((res_c[7000:] > 0).sum().item()), "paths_agree": bool(torch.equal(res_c, out))}
 
 
+def bench_c4(steps, seed):
+    """4096 uneven segments (bitpack/delta/RLE) decoded + range-filtered:
+    one graph-replayed persistent launch vs one launch per segment."""
+    import torch
+
+    import fastlanesgpu_amd as fls
+
+    t = time.perf_counter()
+    b = fls.batch_c4(seed, 4096, 10, 24)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t
+    info = b.info()
+    d = torch.zeros(4096, dtype=torch.int64, device="cuda")
+    best, tune_ms = b.autotune(d)
+    ref = None
+
+    def timed(fn, reps):
+        nonlocal ref
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        got = d.clone()
+        same = True if ref is None else bool(torch.equal(ref, got))
+        ref = got if ref is None else ref
+        return e0.elapsed_time(e1) / reps, same
+
+    graph_ms, _ = timed(lambda: b.graph_launch(best, d), max(5, steps))
+    run_ms, same_run = timed(lambda: b.run(best, d), max(5, steps))
+    seg_ms, same_seg = timed(lambda: b.run_per_segment(best, d), 3)
+    nbytes = info["encoded_bytes"]
+    b.close()
+    return {"workload": "4096 segments, lengths log-uniform by octave in [2^10,2^24), bw U{1..32}, "
+                        "40/30/30 bitpack/delta/RLE, middle-half range filter, count per segment",
+            "n_values": info["n_values"], "encoded_bytes": nbytes,
+            "segments_by_encoding": [info["n_bitpack"], info["n_delta"], info["n_rle"]],
+            "build_s": build_s,
+            "tune": {"tile_vectors": best.tile_vectors, "block_threads": best.block_threads,
+                     "schedule": ["static", "dynamic"][best.schedule], "tuning_ms": tune_ms},
+            "graph_ms": graph_ms, "launch_ms": run_ms, "per_segment_ms": seg_ms,
+            "values_per_s": info["n_values"] / (graph_ms * 1e-3),
+            "encoded_GBs": nbytes / (graph_ms * 1e-3) / 1e9,
+            "frac_hbm": nbytes / (graph_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "speedup_vs_per_segment": seg_ms / graph_ms,
+            "counts_agree": bool(same_run and same_seg)}
+
+
 def cpu_baseline(enc, n_rows, gpu_result):
     """Oracle decode + query on the host cores (all threads, then one thread)."""
     sys.path.insert(0, str(ROOT / "tests"))
@@ -280,6 +331,8 @@ def main():
         line["secondary"] = {"c1_unpack": bench_c1(args.steps, args.warmup, args.seed)}
         log("[bench] secondary: probe + group-by")
         line["secondary"]["c3_probe_groupby"] = bench_c3(max(5, args.steps // 2), 2, args.seed)
+        log("[bench] secondary: batched segments")
+        line["secondary"]["c4_batched"] = bench_c4(max(5, args.steps // 5), args.seed)
     print(json.dumps(line), flush=True)
 
 
