@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-secondary", action="store_true", help="skip the secondary workloads")
+    ap.add_argument("--no-c5", action="store_true", help="skip the learned-selector workload")
     return ap.parse_args()
 
 
@@ -198,6 +199,137 @@ def bench_c4(steps, seed):
             "counts_agree": bool(same_run and same_seg)}
 
 
+def bench_c5(seed, n_rows, n_train=1 << 22, reps=3):
+    """Learned encoding selector: train on 2^22 synthetic vectors (four
+    families, exhaustive labels) with the library's own MFMA kernels, then
+    encode a held-out 120M-row Q1.1 column set three ways -- learned (with the
+    exhaustive fallback below confidence 0.9), exhaustive from statistics,
+    exhaustive by trying all four encoders -- and run Q1.1 over the learned
+    output."""
+    import torch
+
+    import fastlanesgpu_amd as fls
+    from fastlanesgpu_amd import selector as sel
+
+    def ev():
+        return torch.cuda.Event(enable_timing=True)
+
+    # ---- training set + training
+    e0, e1 = ev(), ev()
+    e0.record()
+    feat, label, _ = sel.build_training_set(seed + 100, n_train)
+    e1.record()
+    torch.cuda.synchronize()
+    gen_ms = e0.elapsed_time(e1)
+    mlp = sel.Mlp(seed)
+    epochs, batch = 3, 4096
+    t = time.perf_counter()
+    losses = mlp.train(feat, label, epochs=epochs, batch=batch, lr=3e-3, seed=seed)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t
+    steps = epochs * ((n_train + batch - 1) // batch)
+    # MLP kernel timings (forward over the whole training set; one gradient step)
+    cls = torch.empty(n_train, dtype=torch.uint8, device="cuda")
+    conf = torch.empty(n_train, dtype=torch.float32, device="cuda")
+    fwd = lambda: fls.call("fls_mlp_forward", mlp.wb.data_ptr(), feat.data_ptr(), n_train,  # noqa: E731
+                           cls.data_ptr(), conf.data_ptr(), None, fls.current_stream())
+    fwd()
+    e0, e1 = ev(), ev()
+    e0.record()
+    for _ in range(reps):
+        fwd()
+    e1.record()
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / reps
+    train_acc = float((cls // 10 == label // 10).float().mean().item())
+    fb, lb = feat[:batch].contiguous(), label[:batch].contiguous()
+    mlp.accumulate_grad(fb, lb, batch)
+    e0, e1 = ev(), ev()
+    e0.record()
+    for _ in range(20):
+        mlp.accumulate_grad(fb, lb, batch)
+    e1.record()
+    torch.cuda.synchronize()
+    grad_ms = e0.elapsed_time(e1) / 20
+    mlp.grad.zero_()
+    flop_fwd = 2 * (16 * 64 + 64 * 64 + 64 * 40)  # 15,360 per vector
+    del feat, label
+
+    # ---- held-out column set: the Q1.1 table, values resident in HBM
+    cols = fls.gen_c2(seed + 7, n_rows)
+    n = n_rows
+    in_bytes = 4 * n * len(cols)
+    learned = sel.LearnedSelector(mlp, conf_min=0.9)
+    ws_l = [sel.MixedWorkspace(n) for _ in cols]
+    ws_x = [sel.MixedWorkspace(n) for _ in cols]
+    ws_t = [sel.MixedWorkspace(n) for _ in cols]
+    tbuf = sel.trial_buffer(n)
+
+    def run(fn):
+        fn()
+        torch.cuda.synchronize()
+        a, b = ev(), ev()
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    def enc_learned():
+        for c, w in zip(cols, ws_l):
+            w.fallbacks.zero_()
+            learned.encode(c, n, w)
+
+    def enc_stats():
+        for c, w in zip(cols, ws_x):
+            sel.mixed_encode_exhaustive(c, n, w)
+
+    def enc_trial():
+        for c, w in zip(cols, ws_t):
+            sel.mixed_encode_trial(c, n, w, buf=tbuf)
+
+    ms_l, ms_x, ms_t = run(enc_learned), run(enc_stats), run(enc_trial)
+    size = lambda ws: sum(int(w.total.item()) * 4 + 16 * sel.n_vectors(n) for w in ws)  # noqa: E731
+    s_l, s_x, s_t = size(ws_l), size(ws_x), size(ws_t)
+    nv = sel.n_vectors(n)
+    fallbacks = sum(int(w.fallbacks.item()) for w in ws_l)
+    per_col = []
+    for c, wl, wx in zip(("orderdate", "quantity", "discount", "price"), ws_l, ws_x):
+        el = torch.bincount((wl.cls[:nv] // 10).long(), minlength=4).tolist()
+        per_col.append({"column": c, "learned_bytes": int(wl.total.item()) * 4 + 16 * nv,
+                        "exhaustive_bytes": int(wx.total.item()) * 4 + 16 * nv, "learned_encodings": el})
+    # ---- Q1.1 over the learned output vs the same query on the raw values
+    mixed = [w.column(n) for w in ws_l]
+    rev, cnt = fls.query_q11(*mixed, n)
+    od, qt, dc, pr = (c.long() for c in cols)
+    m = (od > 19930000) & (od < 19940000) & (qt < 25) & (dc >= 1) & (dc <= 3)
+    ref = (int((pr * dc * m).sum().item()), int(m.sum().item()))
+    del od, qt, dc, pr, m
+    q_out = torch.zeros(2, dtype=torch.int64, device="cuda")
+    q_ms = run(lambda: fls.query_q11(*mixed, n, d_out=q_out, sync=False))
+    return {
+        "workload": f"train on {n_train} vectors (uniform / Zipf 1.1 / sorted+N(0,16) / runs mean 32), "
+                    f"encode held-out Q1.1 column set {len(cols)} x {n} u32",
+        "train": {"gen_features_labels_ms": gen_ms, "epochs": epochs, "batch": batch, "adam_steps": steps,
+                  "train_s": train_s, "loss_per_epoch": losses, "train_encoding_accuracy": train_acc,
+                  "fwd_ms_all_vectors": fwd_ms, "fwd_TFLOPs": flop_fwd * n_train / (fwd_ms * 1e-3) / 1e12,
+                  "grad_step_ms_batch": grad_ms,
+                  "grad_TFLOPs": 3 * flop_fwd * batch / (grad_ms * 1e-3) / 1e12,
+                  "mfma_peak_TFLOPs_bf16": 2500.0},
+        "encode_GBs": {"learned": in_bytes / (ms_l * 1e-3) / 1e9,
+                       "exhaustive_stats": in_bytes / (ms_x * 1e-3) / 1e9,
+                       "exhaustive_trial": in_bytes / (ms_t * 1e-3) / 1e9},
+        "encode_ms": {"learned": ms_l, "exhaustive_stats": ms_x, "exhaustive_trial": ms_t},
+        "speedup_vs_trial": ms_t / ms_l, "speedup_vs_stats": ms_x / ms_l,
+        "bytes": {"learned": s_l, "exhaustive_stats": s_x, "exhaustive_trial": s_t, "raw": in_bytes},
+        "size_error": s_l / s_x - 1, "trial_equals_stats": s_t == s_x,
+        "fallback_rate": fallbacks / (nv * len(cols)), "per_column": per_col,
+        "q11_over_learned": {"revenue": rev, "count": cnt, "equal_to_raw_values": (rev, cnt) == ref,
+                             "ms": q_ms, "rows_per_s": n / (q_ms * 1e-3)},
+    }
+
+
 def cpu_baseline(enc, n_rows, gpu_result):
     """Oracle decode + query on the host cores (all threads, then one thread)."""
     sys.path.insert(0, str(ROOT / "tests"))
@@ -333,6 +465,9 @@ def main():
         line["secondary"]["c3_probe_groupby"] = bench_c3(max(5, args.steps // 2), 2, args.seed)
         log("[bench] secondary: batched segments")
         line["secondary"]["c4_batched"] = bench_c4(max(5, args.steps // 5), args.seed)
+        if not args.no_c5:
+            log("[bench] secondary: learned encoding selector")
+            line["secondary"]["c5_learned_selector"] = bench_c5(args.seed, n)
     print(json.dumps(line), flush=True)
 
 

@@ -130,7 +130,7 @@ def delta_decode(d_words, d_vmeta, d_bases, n_vec, d_out, stream=None):
 
 
 # ----------------------------------------------------------- columns
-ENC_BITPACK, ENC_DELTA, ENC_RLE = 0, 1, 2
+ENC_BITPACK, ENC_DELTA, ENC_RLE, ENC_MIXED = 0, 1, 2, 3
 Q_PRED_LOAD = 1
 Q_ACCUMULATE = 2
 
@@ -141,15 +141,15 @@ class Column:
     """A compressed column resident on the device (torch tensors own the memory)."""
 
     def __init__(self, encoding, n_rows, words, bw=0, base=0, transposed=False, vmeta=None, bases=None,
-                 n_words=None):
+                 n_words=None, aux=None):
         self.encoding, self.n_rows, self.bw, self.base = encoding, n_rows, bw, base
-        self.transposed = bool(transposed) or encoding in (ENC_DELTA, ENC_RLE)
-        self.words, self.vmeta, self.bases = words, vmeta, bases
+        self.transposed = bool(transposed) or encoding in (ENC_DELTA, ENC_RLE, ENC_MIXED)
+        self.words, self.vmeta, self.bases, self.aux = words, vmeta, bases, aux
         self.n_words = int(n_words if n_words is not None else (words.numel() if words is not None else 0))
 
     def struct(self):
         return FlsColumn(self.encoding, self.bw, self.base, int(self.transposed), self.n_words,
-                         dptr(self.words), dptr(self.vmeta), dptr(self.bases))
+                         dptr(self.words), dptr(self.vmeta), dptr(self.bases), dptr(self.aux))
 
     def nbytes(self):
         s = self.struct()
@@ -171,19 +171,19 @@ def _torch_dev(a):
 class HostColumn:
     """A compressed column in host memory (numpy arrays), as the encoders emit it."""
 
-    def __init__(self, encoding, n_rows, words, bw=0, base=0, transposed=False, vmeta=None, bases=None):
+    def __init__(self, encoding, n_rows, words, bw=0, base=0, transposed=False, vmeta=None, bases=None, aux=None):
         self.encoding, self.n_rows, self.bw, self.base = encoding, n_rows, bw, base
-        self.transposed = bool(transposed) or encoding in (ENC_DELTA, ENC_RLE)
-        self.words, self.vmeta, self.bases = words, vmeta, bases
+        self.transposed = bool(transposed) or encoding in (ENC_DELTA, ENC_RLE, ENC_MIXED)
+        self.words, self.vmeta, self.bases, self.aux = words, vmeta, bases, aux
 
     def to_device(self):
+        opt = lambda a: None if a is None else _torch_dev(a)  # noqa: E731
         return Column(self.encoding, self.n_rows, _torch_dev(self.words), bw=self.bw, base=self.base,
-                      transposed=self.transposed,
-                      vmeta=None if self.vmeta is None else _torch_dev(self.vmeta),
-                      bases=None if self.bases is None else _torch_dev(self.bases), n_words=self.words.size)
+                      transposed=self.transposed, vmeta=opt(self.vmeta), bases=opt(self.bases),
+                      n_words=self.words.size, aux=opt(self.aux))
 
     def nbytes(self):
-        s = FlsColumn(self.encoding, self.bw, self.base, int(self.transposed), self.words.size, None, None, None)
+        s = FlsColumn(self.encoding, self.bw, self.base, int(self.transposed), self.words.size)
         return int(load().fls_column_bytes(ctypes.byref(s), self.n_rows))
 
 

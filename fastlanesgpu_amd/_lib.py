@@ -41,7 +41,7 @@ SIGNATURES = {
 class FlsColumn(ctypes.Structure):
     """struct fls_column (include/fls/fls.h)."""
     _fields_ = [("encoding", c_u32), ("bw", c_u32), ("base", c_u32), ("transposed", c_u32),
-                ("n_words", c_u64), ("words", c_vp), ("vmeta", c_vp), ("bases", c_vp)]
+                ("n_words", c_u64), ("words", c_vp), ("vmeta", c_vp), ("bases", c_vp), ("aux", c_vp)]
 
 
 class Q11Params(ctypes.Structure):
@@ -148,6 +148,26 @@ SIGNATURES.update({
     "fls_rle_plan_device": (c_int, [c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "fls_rle_pack_device": (c_int, [c_vp, c_u64, c_vp, c_vp, c_vp, c_vp]),
     "fls_rle_decode_u32": (c_int, [c_vp, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp]),
+})
+
+SIGNATURES.update({
+    "fls_exhaustive_select": (c_int, [c_vp, c_u64, c_vp, c_vp, c_vp]),
+    "fls_vector_features": (c_int, [c_vp, c_u64, c_vp, c_vp]),
+    "fls_mixed_max_words": (c_u64, [c_u64]),
+    "fls_mixed_plan_device": (c_int, [c_vp, c_u64, c_vp, c_vp, ctypes.c_float, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "fls_trial_workspace_bytes": (c_u64, [c_u64]),
+    "fls_mixed_encode_trial": (c_int, [c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "fls_mixed_pack_device": (c_int, [c_vp, c_u64, c_vp, c_vp, c_vp, c_vp]),
+    "fls_mixed_decode_u32": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp, c_vp]),
+    "fls_gen_train_vectors": (c_int, [c_u64, c_u64, c_u64, c_vp, c_vp, c_vp]),
+    "fls_mlp_param_count": (c_u32, []),
+    "fls_mlp_to_bf16": (c_int, [c_vp, c_vp, c_vp]),
+    "fls_mlp_forward": (c_int, [c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_vp]),
+    "fls_mlp_grad_workspace_bytes": (c_u64, [c_u64]),
+    "fls_mlp_grad": (c_int, [c_vp, c_vp, c_vp, c_u64, ctypes.c_float, c_vp, c_vp, c_vp, c_vp]),
+    "fls_mlp_adam": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                             ctypes.c_float, c_vp]),
+    "fls_mlp_mfma_selftest": (c_int, [c_vp, c_vp, c_vp, c_vp]),
 })
 
 

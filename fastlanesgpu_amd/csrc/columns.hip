@@ -73,7 +73,7 @@ int persistent_grid(uint64_t work, uint32_t blocks_per_cu, uint32_t* grid) {
     return FLS_OK;
 }
 
-int to_dev_column(const fls_column& c, uint64_t n_rows, ColDev* out, bool allow_delta) {
+int to_dev_column(const fls_column& c, uint64_t n_rows, ColDev* out, bool allow_var) {
     const uint64_t n_vec = ceil_div(n_rows, 1024);
     out->enc = c.encoding;
     out->bw = c.bw;
@@ -82,6 +82,7 @@ int to_dev_column(const fls_column& c, uint64_t n_rows, ColDev* out, bool allow_
     out->words = c.words;
     out->vmeta = c.vmeta;
     out->bases = c.bases;
+    out->aux = c.aux;
     switch (c.encoding) {
         case FLS_ENC_BITPACK:
             if (c.bw > 32) return FLS_ERR_BW;
@@ -89,9 +90,19 @@ int to_dev_column(const fls_column& c, uint64_t n_rows, ColDev* out, bool allow_
             if (c.n_words < n_vec * 32u * c.bw) return FLS_ERR_SIZE;
             return FLS_OK;
         case FLS_ENC_DELTA:
-            if (!allow_delta) return FLS_ERR_ENCODING;
+            if (!allow_var) return FLS_ERR_ENCODING;
             if (n_rows && (!c.vmeta || !c.bases)) return FLS_ERR_NULL;
             if (n_rows && c.n_words && !c.words) return FLS_ERR_NULL;
+            out->transposed = 1;
+            return FLS_OK;
+        case FLS_ENC_RLE:
+            if (!allow_var) return FLS_ERR_ENCODING;
+            if (n_rows && (!c.vmeta || !c.bases || !c.aux || !c.words)) return FLS_ERR_NULL;
+            out->transposed = 1;
+            return FLS_OK;
+        case FLS_ENC_MIXED:
+            if (!allow_var) return FLS_ERR_ENCODING;
+            if (n_rows && (!c.vmeta || !c.aux || !c.words)) return FLS_ERR_NULL;
             out->transposed = 1;
             return FLS_OK;
         default:
@@ -107,6 +118,8 @@ extern "C" uint64_t fls_column_bytes(const fls_column* c, uint64_t n_rows) {
     switch (c->encoding) {
         case FLS_ENC_BITPACK: return n_vec * 32u * c->bw * 4u;
         case FLS_ENC_DELTA: return c->n_words * 4u + n_vec * (sizeof(fls_vmeta) + 32u * 4u);
+        case FLS_ENC_RLE: return c->n_words * 4u + n_vec * (sizeof(fls_vmeta) + sizeof(fls_rle_meta) + 32u * 4u);
+        case FLS_ENC_MIXED: return c->n_words * 4u + n_vec * 16u;
         default: return c->n_words * 4u;
     }
 }

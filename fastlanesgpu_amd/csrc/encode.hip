@@ -10,6 +10,7 @@
 #include "fls/device/unpack.hpp"
 #include "fls/device/wave.hpp"
 #include "fls_encode.hpp"
+#include "fls_pack.hpp"
 #include "fls_runtime.hpp"
 
 namespace fls {
@@ -37,42 +38,6 @@ __device__ __forceinline__ uint32_t half_max(uint32_t x) {
 #pragma unroll
     for (int off = 16; off > 0; off >>= 1) x = max(x, (uint32_t)__shfl_xor(x, off, 64));
     return x;
-}
-
-// Pack 32 rows (row order) of one lane at compile-time width BW into the
-// lane's BW words at vec_out + lane + 32*k.
-template <int BW>
-__device__ __forceinline__ void pack_lane(const uint32_t (&x)[32], uint32_t* __restrict vec_out) {
-    if constexpr (BW == 0) {
-        return;
-    } else {
-        uint32_t w[BW];
-#pragma unroll
-        for (int k = 0; k < BW; ++k) w[k] = 0u;
-#pragma unroll
-        for (int r = 0; r < 32; ++r) {
-            const uint32_t v = x[r] & dev::Mask<BW>::value;
-            const int bit = r * BW, k = bit >> 5, s = bit & 31;
-            w[k] |= v << s;
-            if (s + BW > 32) w[k + 1] |= v >> (32 - s);
-        }
-        uint32_t* o = vec_out + dev::lane32();
-#pragma unroll
-        for (int k = 0; k < BW; ++k) o[32 * k] = w[k];
-    }
-}
-
-__device__ __forceinline__ void pack_lane_rt(const uint32_t (&x)[32], uint32_t* __restrict vec_out, uint32_t bw) {
-    switch (bw) {
-#define FLS_PK(B) \
-    case B: pack_lane<B>(x, vec_out); break;
-        FLS_PK(0) FLS_PK(1) FLS_PK(2) FLS_PK(3) FLS_PK(4) FLS_PK(5) FLS_PK(6) FLS_PK(7) FLS_PK(8) FLS_PK(9)
-        FLS_PK(10) FLS_PK(11) FLS_PK(12) FLS_PK(13) FLS_PK(14) FLS_PK(15) FLS_PK(16) FLS_PK(17) FLS_PK(18)
-        FLS_PK(19) FLS_PK(20) FLS_PK(21) FLS_PK(22) FLS_PK(23) FLS_PK(24) FLS_PK(25) FLS_PK(26) FLS_PK(27)
-        FLS_PK(28) FLS_PK(29) FLS_PK(30) FLS_PK(31) FLS_PK(32)
-#undef FLS_PK
-        default: break;
-    }
 }
 
 // ------------------------------------------------------------------ bitpack
@@ -136,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void delta_pack_kernel(const uint32_t* __re
     uint32_t d[32], base;
     chain_deltas(in, n, v, d, base);
     const fls_vmeta m = vmeta[v];
-    pack_lane_rt(d, out + FLS_VMETA_OFFSET(m), FLS_VMETA_BW(m));
+    pack_rows(d, out + FLS_VMETA_OFFSET(m), FLS_VMETA_BW(m));
 }
 
 // --------------------------------------------------------------------- RLE
@@ -251,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void rle_pack_kernel(const uint32_t* __rest
     const fls_rle_meta rm = rmeta[v];
     uint32_t* vec_out = out + FLS_VMETA_OFFSET(m);
     const uint32_t ibw = FLS_VMETA_BW(m);
-    pack_lane_rt(d, vec_out, ibw);
+    pack_rows(d, vec_out, ibw);
     // linear LSB-first packing of (value - vbase) at vbw bits after the index
     // words; lane l builds words l, l+32, ...
     uint32_t* vals_out = vec_out + 32u * ibw;

@@ -12,11 +12,12 @@ struct ColDev {
     const uint32_t* words;
     const fls_vmeta* vmeta;
     const uint32_t* bases;
+    const uint64_t* aux;  // RLE: fls_rle_meta per vector; MIXED: aux per vector
 };
 
 // Validate a host-side column descriptor for n_rows rows and build its device
 // view.  Fixed-width columns are bounds-checked against n_words.
-int to_dev_column(const fls_column& c, uint64_t n_rows, ColDev* out, bool allow_delta);
+int to_dev_column(const fls_column& c, uint64_t n_rows, ColDev* out, bool allow_var);
 
 // Grid size for a persistent kernel: min(work, CUs * blocks_per_cu).
 int persistent_grid(uint64_t work, uint32_t blocks_per_cu, uint32_t* grid);
@@ -125,7 +126,8 @@ __device__ __forceinline__ void decode_rle(const ColDev& c, const fls_rle_meta* 
         m = c.vmeta[v];
         rm = rmeta[v];
         const uint32_t* vals = c.words + FLS_VMETA_OFFSET(m) + 32u * FLS_VMETA_BW(m);
-        for (uint32_t k = dev::lane32(); k < rm.n_runs; k += 32u) stage[k] = rm.vbase + linear_extract(vals, k, rm.vbw);
+        const uint32_t nr = rm.n_runs < 1024u ? rm.n_runs : 1024u;  // the stage holds one vector's runs
+        for (uint32_t k = dev::lane32(); k < nr; k += 32u) stage[k] = rm.vbase + linear_extract(vals, k, rm.vbw);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -136,6 +138,68 @@ __device__ __forceinline__ void decode_rle(const ColDev& c, const fls_rle_meta* 
     const uint32_t last = rm.n_runs ? rm.n_runs - 1u : 0u;
 #pragma unroll
     for (int r = 0; r < 32; ++r) x[r] = stage[min(x[r], last)];
+}
+
+// MIXED column: vector v decoded with its own encoding (fls.h FLS_ENC_MIXED),
+// transposed order.  Like decode_rle, every work-item of the wave calls it.
+__device__ __forceinline__ void decode_mixed(const ColDev& c, uint64_t v, uint32_t* __restrict stage, bool live,
+                                             uint32_t (&x)[32]) {
+    uint32_t enc = 0, w = 0;
+    uint64_t ax = 0;
+    const uint32_t* p = c.words;
+    if (live) {
+        const fls_vmeta m = c.vmeta[v];
+        enc = (uint32_t)(m >> 6) & 3u;
+        w = (uint32_t)m & 63u;
+        ax = c.aux[v];
+        p = c.words + FLS_VMETA_OFFSET(m);
+    }
+    const uint32_t ibw = (uint32_t)(ax >> 48) & 1u, nr_field = (uint32_t)(ax >> 32) & 0xffffu;
+    const uint32_t nr = nr_field < 1024u ? nr_field : 1024u;  // the stage holds one vector's runs
+    if (live && enc == FLS_MIX_RLE) {
+        const uint32_t* vals = p + 32u + 32u * ibw;
+        for (uint32_t k = dev::lane32(); k < nr; k += 32u) stage[k] = (uint32_t)ax + linear_extract(vals, k, w);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (!live) return;
+    const uint32_t l = dev::lane32();
+    switch (enc) {
+        case FLS_MIX_BITPACK: dev::unpack_device(p, x, w); break;
+        case FLS_MIX_FOR:
+            dev::unpack_device(p, x, w);
+#pragma unroll
+            for (int r = 0; r < 32; ++r) x[r] += (uint32_t)ax;
+            break;
+        case FLS_MIX_DELTA:
+            dev::unpack_device(p + 32, x, w);
+            dev::rsum_lane(x, p[l]);
+            break;
+        default: {
+            dev::unpack_device(p + 32, x, ibw);
+            dev::rsum_lane(x, p[l]);
+            const uint32_t last = nr ? nr - 1u : 0u;
+#pragma unroll
+            for (int r = 0; r < 32; ++r) x[r] = stage[min(x[r], last)];
+        }
+    }
+}
+
+// Any column encoding, width known at run time.  `stage` is this vector's
+// 1024-entry LDS slice (RLE / MIXED); every work-item of the wave calls it.
+__device__ __forceinline__ void decode_col(const ColDev& c, uint64_t v, uint32_t* __restrict stage, bool live,
+                                           uint32_t (&x)[32]) {
+    switch (c.enc) {
+        case FLS_ENC_BITPACK:
+            if (live) decode_fixed<-1>(c, v, x);
+            break;
+        case FLS_ENC_DELTA:
+            if (live) decode_delta(c, v, x);
+            break;
+        case FLS_ENC_RLE: decode_rle(c, reinterpret_cast<const fls_rle_meta*>(c.aux), v, stage, live, x); break;
+        default: decode_mixed(c, v, stage, live, x); break;
+    }
 }
 
 // Predicate-fused FastLanes delta walk of one lane at compile-time width BW:

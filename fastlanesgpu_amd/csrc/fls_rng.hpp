@@ -30,6 +30,11 @@ __host__ __device__ __forceinline__ uint32_t uniform(uint64_t key, uint64_t i, u
 constexpr uint64_t kSegParams = 1000;
 constexpr uint64_t kSegValues = 100000;
 constexpr uint64_t kSegRuns = 200000;
+// Streams of the selector's training vectors.
+constexpr uint64_t kTrainFamily = 300001;
+constexpr uint64_t kTrainValues = 300002;
+constexpr uint64_t kTrainNoise = 300003;
+constexpr uint64_t kTrainRuns = 300004;
 
 }  // namespace rng
 }  // namespace fls

@@ -1,0 +1,419 @@
+// The learned encoding selector: an MLP 16 -> 64 -> 64 -> 40 (ReLU hidden
+// layers, softmax over the 40 classes), BF16 operands on the CDNA matrix
+// cores (v_mfma_f32_16x16x16_bf16) with FP32 accumulation, trained on the GPU
+// by its own forward/backward kernels and an FP32 Adam step.
+//
+// Parameters (FP32 master copy, and a BF16 copy the kernels read), in this
+// order: W1[16][64], b1[64], W2[64][64], b2[64], W3[64][40], b3[40]
+// (W[in][out]); 7,848 values.
+//
+// A wave computes 16 samples at a time (one 16-row MFMA tile).  Weights are
+// staged once per block in LDS as BF16; activations go through per-wave LDS
+// tiles, so every product -- including the transposed ones of the backward
+// pass -- reads its operands with plain strides.
+//
+// MFMA 16x16x16 operand layout used here (lane L of the wave, item i < 4):
+//   A (16 x 16, M x K): A[L % 16][4*(L/16) + i]
+//   B (16 x 16, K x N): B[4*(L/16) + i][L % 16]
+//   D (16 x 16):        D[4*(L/16) + i][L % 16]
+// fls_mlp_mfma_selftest checks it against a host matmul.
+#include <math.h>
+
+#include "fls/device/wave.hpp"
+#include "fls_runtime.hpp"
+
+namespace fls {
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kIn = 16, kH = 64, kOut = 40, kOutPad = 48;
+constexpr int oW1 = 0, oB1 = oW1 + kIn * kH, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH,
+              oB3 = oW3 + kH * kOut, kParams = oB3 + kOut;
+static_assert(kParams == 7848, "parameter count");
+
+// round-to-nearest-even FP32 -> BF16 (bit pattern), NaN kept quiet
+__host__ __device__ inline uint16_t to_bf16(float f) {
+    uint32_t u;
+    __builtin_memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float from_bf16(uint16_t h) {
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+
+__device__ __forceinline__ int lane64() { return (int)(threadIdx.x & 63u); }
+
+// element (r, c) at p[r*rs + c*cs]
+struct View {
+    const uint16_t* p;
+    int rs, cs;
+    __device__ uint16_t at(int r, int c) const { return p[r * rs + c * cs]; }
+};
+
+__device__ __forceinline__ bf16x4 frag_a(const View& A, int m0, int k0) {
+    const int L = lane64(), m = m0 + (L & 15), k = k0 + 4 * (L >> 4);
+    return bf16x4{(short)A.at(m, k), (short)A.at(m, k + 1), (short)A.at(m, k + 2), (short)A.at(m, k + 3)};
+}
+__device__ __forceinline__ bf16x4 frag_b(const View& B, int k0, int n0) {
+    const int L = lane64(), n = n0 + (L & 15), k = k0 + 4 * (L >> 4);
+    return bf16x4{(short)B.at(k, n), (short)B.at(k + 1, n), (short)B.at(k + 2, n), (short)B.at(k + 3, n)};
+}
+__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+// C tile (m0, n0) of A[M x K] * B[K x N], K a multiple of 16
+__device__ __forceinline__ f32x4 tile_mm(const View& A, const View& B, int m0, int n0, int K) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += 16) acc = mfma(frag_a(A, m0, k0), frag_b(B, k0, n0), acc);
+    return acc;
+}
+// row of item i of the D tile held by this lane, and its column
+__device__ __forceinline__ int d_row(int i) { return 4 * (lane64() >> 4) + i; }
+__device__ __forceinline__ int d_col() { return lane64() & 15; }
+
+// Per-block LDS: BF16 weights (transposed copies for the backward products
+// are not needed: strides do it) + per-wave activation tiles.
+struct WaveTiles {
+    uint16_t x[16 * kIn];        // X tile, row-major [16][16]
+    uint16_t h1[16 * kH];        // relu(X W1 + b1)
+    uint16_t h2[16 * kH];        // relu(H1 W2 + b2)
+    uint16_t dz[16 * kOutPad];   // dL/dZ (bf16), padded columns zero
+    uint16_t dh2[16 * kH];       // dL/dH2 (masked)
+    uint16_t dh1[16 * kH];       // dL/dH1 (masked)
+    float z[16 * kOutPad];       // logits
+};
+struct MlpLds {
+    uint16_t w1[kIn * kH], w2[kH * kH], w3[kH * kOutPad];
+    float b1[kH], b2[kH], b3[kOutPad];
+    WaveTiles t[4];
+};
+
+__device__ void stage_weights(const uint16_t* __restrict wb, MlpLds& s) {
+    for (int i = threadIdx.x; i < kIn * kH; i += blockDim.x) s.w1[i] = wb[oW1 + i];
+    for (int i = threadIdx.x; i < kH * kH; i += blockDim.x) s.w2[i] = wb[oW2 + i];
+    for (int i = threadIdx.x; i < kH * kOutPad; i += blockDim.x) {
+        const int r = i / kOutPad, c = i % kOutPad;
+        s.w3[i] = c < kOut ? wb[oW3 + r * kOut + c] : (uint16_t)0;
+    }
+    for (int i = threadIdx.x; i < kH; i += blockDim.x) {
+        s.b1[i] = from_bf16(wb[oB1 + i]);
+        s.b2[i] = from_bf16(wb[oB2 + i]);
+    }
+    for (int i = threadIdx.x; i < kOutPad; i += blockDim.x) s.b3[i] = i < kOut ? from_bf16(wb[oB3 + i]) : 0.f;
+}
+
+// forward of one 16-sample tile (rows >= valid are zero inputs)
+__device__ void forward_tile(const float* __restrict feat, uint64_t s0, uint64_t n, MlpLds& s, WaveTiles& t) {
+    const int L = lane64();
+    for (int i = L; i < 16 * kIn; i += 64) {
+        const uint64_t row = s0 + i / kIn;
+        t.x[i] = to_bf16(row < n ? feat[row * kIn + (i % kIn)] : 0.f);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const View X{t.x, kIn, 1}, W1{s.w1, kH, 1}, W2{s.w2, kH, 1}, W3{s.w3, kOutPad, 1};
+    for (int n0 = 0; n0 < kH; n0 += 16) {
+        const f32x4 acc = tile_mm(X, W1, 0, n0, kIn);
+        for (int i = 0; i < 4; ++i) {
+            const float h = acc[i] + s.b1[n0 + d_col()];
+            t.h1[d_row(i) * kH + n0 + d_col()] = to_bf16(h > 0.f ? h : 0.f);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const View H1{t.h1, kH, 1};
+    for (int n0 = 0; n0 < kH; n0 += 16) {
+        const f32x4 acc = tile_mm(H1, W2, 0, n0, kH);
+        for (int i = 0; i < 4; ++i) {
+            const float h = acc[i] + s.b2[n0 + d_col()];
+            t.h2[d_row(i) * kH + n0 + d_col()] = to_bf16(h > 0.f ? h : 0.f);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const View H2{t.h2, kH, 1};
+    for (int n0 = 0; n0 < kOutPad; n0 += 16) {
+        const f32x4 acc = tile_mm(H2, W3, 0, n0, kH);
+        for (int i = 0; i < 4; ++i) t.z[d_row(i) * kOutPad + n0 + d_col()] = acc[i] + s.b3[n0 + d_col()];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ------------------------------------------------------------- inference
+__global__ __launch_bounds__(256) void mlp_forward_kernel(const uint16_t* __restrict wb, const float* __restrict feat,
+                                                          uint64_t n, uint8_t* __restrict cls, float* __restrict conf,
+                                                          float* __restrict logits) {
+    __shared__ MlpLds s;
+    stage_weights(wb, s);
+    __syncthreads();
+    WaveTiles& t = s.t[threadIdx.x >> 6];
+    const uint64_t n_tiles = ceil_div(n, 16);
+    const uint64_t waves = (uint64_t)gridDim.x * 4;
+    for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < n_tiles; tile += waves) {
+        const uint64_t s0 = tile * 16;
+        forward_tile(feat, s0, n, s, t);
+        const int L = lane64();
+        if (L < 16 && s0 + L < n) {
+            // the encoding with the largest probability mass (classes 10e..10e+9),
+            // its most likely width bucket, and that mass as the confidence
+            const float* z = t.z + L * kOutPad;
+            float mx = z[0];
+            for (int c = 1; c < kOut; ++c) mx = fmaxf(mx, z[c]);
+            float mass[4] = {0.f, 0.f, 0.f, 0.f};
+            for (int c = 0; c < kOut; ++c) mass[c / 10] += expf(z[c] - mx);
+            int be = 0;
+            for (int e = 1; e < 4; ++e)
+                if (mass[e] > mass[be]) be = e;
+            int best = 10 * be;
+            for (int c = 10 * be + 1; c < 10 * be + 10; ++c)
+                if (z[c] > z[best]) best = c;
+            const uint64_t row = s0 + L;
+            cls[row] = (uint8_t)best;
+            if (conf) conf[row] = mass[be] / (mass[0] + mass[1] + mass[2] + mass[3]);
+            if (logits)
+                for (int c = 0; c < kOut; ++c) logits[row * kOut + c] = z[c];
+        }
+    }
+}
+
+// -------------------------------------------------------------- gradients
+// Softmax cross-entropy, scaled by inv_batch.  Every wave accumulates the
+// gradients of its tiles in registers and writes them, with its summed loss,
+// to its own row of `part` (kPart floats); grad_reduce_kernel then adds the
+// rows in wave order, so a step's gradient is bit-reproducible.
+constexpr int kPart = kParams + 8;  // [kParams] = loss, rest padding
+constexpr int kGradMaxBlocks = 64;
+
+__global__ __launch_bounds__(256) void mlp_grad_kernel(const uint16_t* __restrict wb, const float* __restrict feat,
+                                                       const uint8_t* __restrict label, uint64_t n, float inv_batch,
+                                                       float* __restrict part) {
+    __shared__ MlpLds s;
+    stage_weights(wb, s);
+    __syncthreads();
+    WaveTiles& t = s.t[threadIdx.x >> 6];
+    const int L = lane64();
+    // register accumulators of this wave's weight gradients (D-tile layout)
+    f32x4 gw1[4], gw2[16], gw3[12];
+    for (int i = 0; i < 4; ++i) gw1[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < 16; ++i) gw2[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < 12; ++i) gw3[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float gb1 = 0.f, gb2 = 0.f, gb3 = 0.f, lsum = 0.f;  // lane L owns column L of each bias
+    const uint64_t n_tiles = ceil_div(n, 16);
+    const uint64_t waves = (uint64_t)gridDim.x * 4;
+    for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < n_tiles; tile += waves) {
+        const uint64_t s0 = tile * 16;
+        forward_tile(feat, s0, n, s, t);
+        // softmax + dZ = (p - onehot) * inv_batch, rows past n contribute nothing
+        if (L < 16) {
+            const uint64_t row = s0 + L;
+            float* z = t.z + L * kOutPad;
+            uint16_t* dz = t.dz + L * kOutPad;
+            if (row < n) {
+                float mx = z[0];
+                for (int c = 1; c < kOut; ++c) mx = fmaxf(mx, z[c]);
+                float sum = 0.f;
+                for (int c = 0; c < kOut; ++c) sum += expf(z[c] - mx);
+                const int y = label[row];
+                lsum += logf(sum) + mx - z[y];
+                for (int c = 0; c < kOut; ++c) {
+                    const float p = expf(z[c] - mx) / sum;
+                    dz[c] = to_bf16((p - (c == y ? 1.f : 0.f)) * inv_batch);
+                }
+            } else {
+                for (int c = 0; c < kOut; ++c) dz[c] = 0;
+            }
+            for (int c = kOut; c < kOutPad; ++c) dz[c] = 0;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const View DZ{t.dz, kOutPad, 1};
+        const View H2T{t.h2, 1, kH}, H1T{t.h1, 1, kH}, XT{t.x, 1, kIn};
+        // dW3 += H2^T dZ   (64 x 48, K = 16 samples)
+        for (int mi = 0; mi < 4; ++mi)
+            for (int ni = 0; ni < 3; ++ni)
+                gw3[mi * 3 + ni] = mfma(frag_a(H2T, 16 * mi, 0), frag_b(DZ, 0, 16 * ni), gw3[mi * 3 + ni]);
+        // dH2 = dZ W3^T, masked by H2 > 0
+        const View W3T{s.w3, 1, kOutPad};
+        for (int n0 = 0; n0 < kH; n0 += 16) {
+            const f32x4 acc = tile_mm(DZ, W3T, 0, n0, kOutPad);
+            for (int i = 0; i < 4; ++i) {
+                const int r = d_row(i), c = n0 + d_col();
+                t.dh2[r * kH + c] = to_bf16(from_bf16(t.h2[r * kH + c]) > 0.f ? acc[i] : 0.f);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const View DH2{t.dh2, kH, 1};
+        // dW2 += H1^T dH2
+        for (int mi = 0; mi < 4; ++mi)
+            for (int ni = 0; ni < 4; ++ni)
+                gw2[mi * 4 + ni] = mfma(frag_a(H1T, 16 * mi, 0), frag_b(DH2, 0, 16 * ni), gw2[mi * 4 + ni]);
+        // dH1 = dH2 W2^T, masked by H1 > 0
+        const View W2T{s.w2, 1, kH};
+        for (int n0 = 0; n0 < kH; n0 += 16) {
+            const f32x4 acc = tile_mm(DH2, W2T, 0, n0, kH);
+            for (int i = 0; i < 4; ++i) {
+                const int r = d_row(i), c = n0 + d_col();
+                t.dh1[r * kH + c] = to_bf16(from_bf16(t.h1[r * kH + c]) > 0.f ? acc[i] : 0.f);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const View DH1{t.dh1, kH, 1};
+        // dW1 += X^T dH1  (16 x 64)
+        for (int ni = 0; ni < 4; ++ni) gw1[ni] = mfma(frag_a(XT, 0, 0), frag_b(DH1, 0, 16 * ni), gw1[ni]);
+        // bias gradients: column sums over the tile's 16 rows
+        for (int r = 0; r < 16; ++r) {
+            gb1 += from_bf16(t.dh1[r * kH + L]);
+            gb2 += from_bf16(t.dh2[r * kH + L]);
+            if (L < kOutPad) gb3 += from_bf16(t.dz[r * kOutPad + L]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // this wave's row of partial sums: every parameter written exactly once
+    float* p = part + (uint64_t)(blockIdx.x * 4 + (threadIdx.x >> 6)) * kPart;
+    for (int mi = 0; mi < 4; ++mi)
+        for (int ni = 0; ni < 3; ++ni)
+            for (int i = 0; i < 4; ++i) {
+                const int r = 16 * mi + d_row(i), c = 16 * ni + d_col();
+                if (c < kOut) p[oW3 + r * kOut + c] = gw3[mi * 3 + ni][i];
+            }
+    for (int mi = 0; mi < 4; ++mi)
+        for (int ni = 0; ni < 4; ++ni)
+            for (int i = 0; i < 4; ++i) p[oW2 + (16 * mi + d_row(i)) * kH + 16 * ni + d_col()] = gw2[mi * 4 + ni][i];
+    for (int ni = 0; ni < 4; ++ni)
+        for (int i = 0; i < 4; ++i) p[oW1 + d_row(i) * kH + 16 * ni + d_col()] = gw1[ni][i];
+    p[oB1 + L] = gb1;
+    p[oB2 + L] = gb2;
+    if (L < kOut) p[oB3 + L] = gb3;
+    const float wl = dev::wave_sum(lsum);
+    if (L == 0) p[kParams] = wl;
+}
+
+// grad[i] += sum over waves of part[w][i] in wave order; *loss likewise
+__global__ void grad_reduce_kernel(const float* __restrict part, uint32_t n_waves, float* __restrict grad,
+                                   float* __restrict loss) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > kParams) return;
+    float acc = 0.f;
+    for (uint32_t w = 0; w < n_waves; ++w) acc += part[(uint64_t)w * kPart + i];
+    if (i < kParams) grad[i] += acc;
+    else if (loss) *loss += acc;
+}
+
+uint32_t grad_blocks(uint64_t n) {
+    const uint64_t tiles = ceil_div(n, 16);
+    const uint64_t g = ceil_div(tiles, 8);  // about two tiles per wave before the flush
+    return (uint32_t)(g < 1 ? 1 : g > kGradMaxBlocks ? kGradMaxBlocks : g);
+}
+
+// Adam on the FP32 master weights; refreshes the BF16 copy and zeroes grad
+__global__ void mlp_adam_kernel(float* __restrict w, float* __restrict m, float* __restrict v, float* __restrict grad,
+                                uint16_t* __restrict wb, float lr, float b1, float b2, float eps, float c1, float c2) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kParams; i += gridDim.x * blockDim.x) {
+        const float g = grad[i];
+        const float mi = b1 * m[i] + (1.f - b1) * g;
+        const float vi = b2 * v[i] + (1.f - b2) * g * g;
+        m[i] = mi;
+        v[i] = vi;
+        const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        w[i] = wi;
+        wb[i] = to_bf16(wi);
+        grad[i] = 0.f;
+    }
+}
+
+__global__ void to_bf16_kernel(const float* __restrict w, uint16_t* __restrict wb, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) wb[i] = to_bf16(w[i]);
+}
+
+// one 16x16x16 tile through the same fragment path, for the layout check
+__global__ void mfma_selftest_kernel(const uint16_t* __restrict a, const uint16_t* __restrict b, float* __restrict d) {
+    const View A{a, 16, 1}, B{b, 16, 1};
+    const f32x4 acc = tile_mm(A, B, 0, 0, 16);
+    for (int i = 0; i < 4; ++i) d[d_row(i) * 16 + d_col()] = acc[i];
+}
+
+uint32_t grid_of(uint64_t n) {
+    const uint64_t tiles = ceil_div(n, 16);
+    const uint64_t g = ceil_div(tiles, 4);
+    return (uint32_t)(g < 1024 ? (g ? g : 1) : 1024);
+}
+
+}  // namespace
+}  // namespace fls
+
+extern "C" {
+
+uint32_t fls_mlp_param_count(void) { return fls::kParams; }
+
+int fls_mlp_to_bf16(const float* d_w, uint16_t* d_wb, void* stream) {
+    if (!d_w || !d_wb) return FLS_ERR_NULL;
+    hipLaunchKernelGGL(fls::to_bf16_kernel, dim3(32), dim3(256), 0, fls::as_stream(stream), d_w, d_wb, fls::kParams);
+    return fls::check_launch();
+}
+
+int fls_mlp_forward(const uint16_t* d_wb, const float* d_feat, uint64_t n, uint8_t* d_class, float* d_conf,
+                    float* d_logits, void* stream) {
+    using namespace fls;
+    if (n == 0) return FLS_OK;
+    FLS_RETURN_IF(!d_wb || !d_feat || !d_class, FLS_ERR_NULL);
+    hipLaunchKernelGGL(mlp_forward_kernel, dim3(grid_of(n)), dim3(256), 0, as_stream(stream), d_wb, d_feat, n, d_class,
+                       d_conf, d_logits);
+    return check_launch();
+}
+
+uint64_t fls_mlp_grad_workspace_bytes(uint64_t n) {
+    return (uint64_t)fls::grad_blocks(n) * 4u * fls::kPart * sizeof(float);
+}
+
+int fls_mlp_grad(const uint16_t* d_wb, const float* d_feat, const uint8_t* d_label, uint64_t n, float inv_batch,
+                 void* d_work, float* d_grad, float* d_loss, void* stream) {
+    using namespace fls;
+    if (n == 0) return FLS_OK;
+    FLS_RETURN_IF(!d_wb || !d_feat || !d_label || !d_work || !d_grad, FLS_ERR_NULL);
+    const uint32_t blocks = grad_blocks(n);
+    float* part = static_cast<float*>(d_work);
+    hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks), dim3(256), 0, as_stream(stream), d_wb, d_feat, d_label, n,
+                       inv_batch, part);
+    if (const int rc = check_launch()) return rc;
+    hipLaunchKernelGGL(grad_reduce_kernel, dim3((uint32_t)ceil_div(kParams + 1, 256)), dim3(256), 0,
+                       as_stream(stream), part, blocks * 4u, d_grad, d_loss);
+    return check_launch();
+}
+
+int fls_mlp_adam(float* d_w, float* d_m, float* d_v, float* d_grad, uint16_t* d_wb, uint32_t step, float lr, float beta1,
+                 float beta2, float eps, void* stream) {
+    using namespace fls;
+    FLS_RETURN_IF(!d_w || !d_m || !d_v || !d_grad || !d_wb, FLS_ERR_NULL);
+    FLS_RETURN_IF(step == 0, FLS_ERR_ARG);
+    const float c1 = 1.f - powf(beta1, (float)step), c2 = 1.f - powf(beta2, (float)step);
+    hipLaunchKernelGGL(mlp_adam_kernel, dim3(32), dim3(256), 0, as_stream(stream), d_w, d_m, d_v, d_grad, d_wb, lr,
+                       beta1, beta2, eps, c1, c2);
+    return check_launch();
+}
+
+int fls_mlp_mfma_selftest(const uint16_t* d_a, const uint16_t* d_b, float* d_d, void* stream) {
+    if (!d_a || !d_b || !d_d) return FLS_ERR_NULL;
+    hipLaunchKernelGGL(fls::mfma_selftest_kernel, dim3(1), dim3(64), 0, fls::as_stream(stream), d_a, d_b, d_d);
+    return fls::check_launch();
+}
+
+}  // extern "C"

@@ -31,86 +31,54 @@ struct Q11Dev {
     uint64_t* count;
 };
 
-// Rows of this work-item that exist (only the last vector can be partial).
-__device__ __forceinline__ void valid_rows(uint64_t v, const Q11Dev& a, bool transposed, bool (&ok)[32]) {
-    const uint64_t first = v * 1024u;
-    const uint32_t l = dev::lane32();
-    if (first + 1024u <= a.n_rows) {
-#pragma unroll
-        for (int r = 0; r < 32; ++r) ok[r] = true;
-    } else {
-        const uint32_t live = (uint32_t)(a.n_rows - first);
-#pragma unroll
-        for (int r = 0; r < 32; ++r) {
-            const uint32_t p = l + 32u * r;
-            ok[r] = (transposed ? dev::untranspose_index(p) : p) < live;
-        }
-    }
-}
-
-template <int QBW, int DBW, int PBW, bool DATE_DELTA, bool PRED>
+// Generic path: every column through decode_col (any encoding, run-time
+// widths); RLE / MIXED columns stage their run values in LDS.
+template <bool PRED>
 __global__ __launch_bounds__(kBlock) void q11_kernel(const Q11Dev a) {
+    __shared__ uint32_t stage[kVecPerBlock][1024];
+    uint32_t* st = stage[threadIdx.x >> 5];
     uint64_t acc = 0;
-    uint64_t cnt = 0;  // wave-level count, identical in every lane of the wave
+    uint32_t cnt = 0;
     const bool transposed = a.price.transposed != 0;
+    const uint32_t dlo = a.date_gt + 1u, dspan = a.date_lt - a.date_gt - 1u;
+    const bool ok = a.date_lt > a.date_gt && a.disc_hi >= a.disc_lo;
+    const uint32_t span = a.disc_hi - a.disc_lo;
     for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
         const uint64_t v = g * kVecPerBlock + (threadIdx.x >> 5);
         if (v >= a.n_vec) continue;
-        bool f[32];
-        valid_rows(v, a, transposed, f);
-
+        const uint64_t first = v * 1024u;
+        uint32_t m = first + 1024u <= a.n_rows ? 0xffffffffu : live_row_mask((uint32_t)(a.n_rows - first), transposed);
+        if (!ok) m = 0;
         uint32_t x[32];
-        if constexpr (DATE_DELTA) decode_delta(a.date, v, x);
-        else decode_fixed<-1>(a.date, v, x);
-        // date_gt < x < date_lt  <=>  (x - date_gt - 1) < (date_lt - date_gt - 1)  (unsigned)
-        const uint32_t dlo = a.date_gt + 1u, dspan = a.date_lt - a.date_gt - 1u;
+        decode_col(a.date, v, st, true, x);
 #pragma unroll
-        for (int r = 0; r < 32; ++r) f[r] = f[r] && (x[r] - dlo) < dspan && a.date_lt > a.date_gt;
-        if constexpr (PRED) {
-            bool any = false;
+        for (int r = 0; r < 32; ++r) m &= ~((uint32_t)((x[r] - dlo) >= dspan) << r);
+        if (PRED && !__any(m != 0)) continue;
+        decode_col(a.qty, v, st, true, x);
 #pragma unroll
-            for (int r = 0; r < 32; ++r) any |= f[r];
-            if (!__any(any)) continue;
-        }
-
-        decode_fixed<QBW>(a.qty, v, x);
-#pragma unroll
-        for (int r = 0; r < 32; ++r) f[r] = f[r] && x[r] < a.qty_lt;
-        if constexpr (PRED) {
-            bool any = false;
-#pragma unroll
-            for (int r = 0; r < 32; ++r) any |= f[r];
-            if (!__any(any)) continue;
-        }
-
+        for (int r = 0; r < 32; ++r) m &= ~((uint32_t)(x[r] >= a.qty_lt) << r);
+        if (PRED && !__any(m != 0)) continue;
         uint32_t d[32];
-        decode_fixed<DBW>(a.disc, v, d);
-        const uint32_t span = a.disc_hi - a.disc_lo;
+        decode_col(a.disc, v, st, true, d);
 #pragma unroll
-        for (int r = 0; r < 32; ++r) f[r] = f[r] && (d[r] - a.disc_lo) <= span && a.disc_hi >= a.disc_lo;
-        if constexpr (PRED) {
-            bool any = false;
-#pragma unroll
-            for (int r = 0; r < 32; ++r) any |= f[r];
-            if (!__any(any)) continue;
-        }
-
-        decode_fixed<PBW>(a.price, v, x);
+        for (int r = 0; r < 32; ++r) m &= ~((uint32_t)((d[r] - a.disc_lo) > span) << r);
+        if (PRED && !__any(m != 0)) continue;
+        decode_col(a.price, v, st, true, x);
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
-            acc += (uint64_t)x[r] * (uint64_t)(f[r] ? d[r] : 0u);
-            cnt += __popcll(__ballot(f[r]));
+            const uint32_t f = (m >> r) & 1u;
+            acc += (uint64_t)x[r] * (uint64_t)(d[r] * f);
+            cnt += f;
         }
     }
-
-    // block reduction: wave sums -> LDS -> one atomic pair per block
     __shared__ uint64_t s_acc[kBlock / 64];
     __shared__ uint64_t s_cnt[kBlock / 64];
     acc = dev::wave_sum(acc);
+    const uint64_t c64 = dev::wave_sum((uint64_t)cnt);
     const int wave = threadIdx.x >> 6;
     if (dev::is_wave_leader()) {
         s_acc[wave] = acc;
-        s_cnt[wave] = cnt;
+        s_cnt[wave] = c64;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -222,10 +190,9 @@ int launch_resident(Kernel k, const Q11Dev& a, hipStream_t s) {
     return check_launch();
 }
 
-template <int QBW, int DBW, int PBW, bool DATE_DELTA>
-int launch(const Q11Dev& a, bool pred, hipStream_t s) {
-    if (pred) return launch_resident(q11_kernel<QBW, DBW, PBW, DATE_DELTA, true>, a, s);
-    return launch_resident(q11_kernel<QBW, DBW, PBW, DATE_DELTA, false>, a, s);
+int launch_generic(const Q11Dev& a, bool pred, hipStream_t s) {
+    if (pred) return launch_resident(q11_kernel<true>, a, s);
+    return launch_resident(q11_kernel<false>, a, s);
 }
 
 template <int QBW, int DBW, int PBW>
@@ -245,12 +212,12 @@ extern "C" int fls_query_q11(const fls_column* orderdate, const fls_column* quan
     Q11Dev a{};
     int st;
     if ((st = to_dev_column(*orderdate, n_rows, &a.date, true)) != FLS_OK) return st;
-    if ((st = to_dev_column(*quantity, n_rows, &a.qty, false)) != FLS_OK) return st;
-    if ((st = to_dev_column(*discount, n_rows, &a.disc, false)) != FLS_OK) return st;
-    if ((st = to_dev_column(*price, n_rows, &a.price, false)) != FLS_OK) return st;
-    const uint32_t tr = price->transposed != 0;
-    FLS_RETURN_IF((orderdate->encoding == FLS_ENC_DELTA ? 1u : (orderdate->transposed != 0)) != tr ||
-                      (quantity->transposed != 0) != tr || (discount->transposed != 0) != tr,
+    if ((st = to_dev_column(*quantity, n_rows, &a.qty, true)) != FLS_OK) return st;
+    if ((st = to_dev_column(*discount, n_rows, &a.disc, true)) != FLS_OK) return st;
+    if ((st = to_dev_column(*price, n_rows, &a.price, true)) != FLS_OK) return st;
+    // every column must use the same row order (delta / RLE / mixed imply transposed)
+    const uint32_t tr = a.price.transposed != 0;
+    FLS_RETURN_IF((a.date.transposed != 0) != tr || (a.qty.transposed != 0) != tr || (a.disc.transposed != 0) != tr,
                   FLS_ERR_ARG);
     a.n_rows = n_rows;
     a.n_vec = ceil_div(n_rows, 1024);
@@ -269,8 +236,8 @@ extern "C" int fls_query_q11(const fls_column* orderdate, const fls_column* quan
     }
     if (n_rows == 0) return FLS_OK;
     const bool pred = (flags & FLS_Q_PRED_LOAD) != 0;
-    const bool delta = a.date.enc == FLS_ENC_DELTA;
-    if (delta && a.qty.bw == 6 && a.disc.bw == 4 && a.price.bw == 24) return launch_fast<6, 4, 24>(a, pred, s);
-    if (delta) return launch<-1, -1, -1, true>(a, pred, s);
-    return launch<-1, -1, -1, false>(a, pred, s);
+    const bool fast = a.date.enc == FLS_ENC_DELTA && a.qty.enc == FLS_ENC_BITPACK && a.disc.enc == FLS_ENC_BITPACK &&
+                      a.price.enc == FLS_ENC_BITPACK && a.qty.bw == 6 && a.disc.bw == 4 && a.price.bw == 24;
+    if (fast) return launch_fast<6, 4, 24>(a, pred, s);
+    return launch_generic(a, pred, s);
 }

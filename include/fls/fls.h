@@ -159,8 +159,19 @@ int fls_rle_decode_u32(const uint32_t* d_words, const fls_vmeta* d_vmeta, const 
 enum {
     FLS_ENC_BITPACK = 0, /* fixed bw for every vector, + frame of reference `base` */
     FLS_ENC_DELTA = 1,   /* FastLanes delta, per-vector (offset,bw) + 32 bases     */
-    FLS_ENC_RLE = 2      /* FastLanes RLE (run values + delta-coded run index)     */
+    FLS_ENC_RLE = 2,     /* FastLanes RLE (run values + delta-coded run index)     */
+    FLS_ENC_MIXED = 3    /* each vector its own encoding (learned/exhaustive pick) */
 };
+
+/* Per-vector encodings of a MIXED column.  vmeta[v] = offset << 8 |
+ * enc << 6 | width; aux[v] = FOR base, or for RLE vbase | n_runs << 32 |
+ * ibw << 48.  Payload at `offset` (words), every vector in transposed order:
+ *   BITPACK: 32*width words;  FOR: 32*width words of value - base;
+ *   DELTA: 32 bases, then 32*width words of deltas;
+ *   RLE: 32 run-index bases, 32*ibw index-delta words, then the run values -
+ *        vbase linear-packed at `width` bits.
+ * A vector's encoded size is 4*payload words + 16 bytes (vmeta + aux).       */
+enum { FLS_MIX_BITPACK = 0, FLS_MIX_FOR = 1, FLS_MIX_DELTA = 2, FLS_MIX_RLE = 3 };
 
 typedef struct fls_column {
     uint32_t encoding;       /* FLS_ENC_* */
@@ -169,8 +180,9 @@ typedef struct fls_column {
     uint32_t transposed;     /* rows in FastLanes transposed order */
     uint64_t n_words;        /* packed words in `words` (for sizes/bounds) */
     const uint32_t* words;   /* packed words */
-    const fls_vmeta* vmeta;  /* DELTA / RLE: per-vector (offset, bw); else NULL */
-    const uint32_t* bases;   /* DELTA: 32 bases per vector; else NULL */
+    const fls_vmeta* vmeta;  /* DELTA / RLE / MIXED: per-vector metadata; else NULL */
+    const uint32_t* bases;   /* DELTA / RLE: 32 bases per vector; else NULL */
+    const uint64_t* aux;     /* RLE: fls_rle_meta per vector; MIXED: aux per vector */
 } fls_column;
 
 /* ==================================================== fused decode + query
@@ -268,6 +280,77 @@ int fls_query_q21(const fls_column* datekey, const fls_column* suppkey, const fl
 /* Encoded bytes a full scan of a column reads for n_rows (packed words, plus
  * per-vector metadata and bases for DELTA).                                   */
 uint64_t fls_column_bytes(const fls_column* col, uint64_t n_rows);
+
+/* ===================================================== encoding selection
+ * Classes: 4 encodings (FLS_MIX_*) x 10 width buckets
+ * (0 | 1-2 | 3-4 | 5-6 | 7-8 | 9-12 | 13-16 | 17-20 | 21-24 | 25-32);
+ * class = 10*encoding + bucket(width of that encoding).  No reference
+ * counterpart (the reference fixes each column's encoding by hand; its
+ * nearest helper is find_bw, ssb_utils.h:222-239).
+ * fls_exhaustive_select: for every vector the exact encoded size of all four
+ * encodings (d_sizes4[4*v+e], may be NULL) and the smallest class d_class[v].
+ * fls_vector_features: 16 float features per vector from a 1/8 sample.      */
+int fls_exhaustive_select(const uint32_t* d_in, uint64_t n, uint8_t* d_class, uint32_t* d_sizes4, void* stream);
+int fls_vector_features(const uint32_t* d_in, uint64_t n, float* d_feat, void* stream);
+/* Mixed column from per-vector classes (only each vector's chosen encoding is
+ * analysed): plan (vmeta, aux, total words; scratch as fls_encode_scratch_bytes)
+ * then pack into at most fls_mixed_max_words(n) words.  d_class is read and,
+ * for fallback vectors, rewritten: when d_conf is not NULL, a vector with
+ * d_conf[v] < conf_min is planned by exhaustive search instead, and
+ * *d_fallbacks (may be NULL) is incremented for it.                          */
+uint64_t fls_mixed_max_words(uint64_t n);
+int fls_mixed_plan_device(const uint32_t* d_in, uint64_t n, uint8_t* d_class, const float* d_conf, float conf_min,
+                          fls_vmeta* d_vmeta, uint64_t* d_aux, uint64_t* d_total_words, uint32_t* d_fallbacks,
+                          void* d_scratch, void* stream);
+int fls_mixed_pack_device(const uint32_t* d_in, uint64_t n, const fls_vmeta* d_vmeta, const uint64_t* d_aux,
+                          uint32_t* d_out, void* stream);
+/* Try-all-encodings search: every vector is encoded with each of the four
+ * encodings (four full plan + pack passes into candidate columns held in
+ * d_work, fls_trial_workspace_bytes(n) bytes), then the smallest candidate of
+ * each vector is copied into the MIXED column (d_out: fls_mixed_max_words(n)
+ * words).  The baseline the learned selector is measured against; its
+ * output equals the exhaustive-class column byte for byte.                  */
+uint64_t fls_trial_workspace_bytes(uint64_t n);
+int fls_mixed_encode_trial(const uint32_t* d_in, uint64_t n, uint8_t* d_class, fls_vmeta* d_vmeta, uint64_t* d_aux,
+                           uint32_t* d_out, uint64_t* d_total_words, void* d_work, void* stream);
+/* decode a MIXED column into transposed order (out[v*1024 + 32r + l])        */
+int fls_mixed_decode_u32(const uint32_t* d_words, const fls_vmeta* d_vmeta, const uint64_t* d_aux, uint64_t n_vec,
+                         uint32_t* d_out, void* stream);
+
+/* ======================================================= learned selector
+ * MLP 16 -> 64 -> 64 -> 40 (ReLU, softmax) on BF16 MFMA with FP32
+ * accumulation.  Parameters (fls_mlp_param_count() = 7848 floats):
+ * W1[16][64], b1[64], W2[64][64], b2[64], W3[64][40], b3[40], W[in][out].
+ * d_wb is the BF16 copy the kernels read (fls_mlp_to_bf16 / fls_mlp_adam
+ * refresh it).  No reference counterpart.
+ *   forward: per feature row, the encoding e with the largest softmax mass
+ *            (classes 10e..10e+9), the class = its most likely width bucket,
+ *            and conf = that mass; d_conf / d_logits may be NULL.
+ *   grad: adds the gradient of the mean cross-entropy (inv_batch = 1/batch)
+ *         over n rows into d_grad, the summed loss into *d_loss (may be NULL);
+ *         d_work holds fls_mlp_grad_workspace_bytes(n) bytes.  Per-wave
+ *         partial sums are added in a fixed order: the result is
+ *         bit-reproducible.
+ *   adam: one Adam step (step >= 1) on the FP32 weights; refreshes d_wb and
+ *         zeroes d_grad.
+ *   mfma_selftest: D = A * B for one 16x16x16 BF16 tile (row-major) through
+ *         the kernels' own operand path, to pin the matrix-core lane layout. */
+uint32_t fls_mlp_param_count(void);
+int fls_mlp_to_bf16(const float* d_w, uint16_t* d_wb, void* stream);
+int fls_mlp_forward(const uint16_t* d_wb, const float* d_feat, uint64_t n, uint8_t* d_class, float* d_conf,
+                    float* d_logits, void* stream);
+uint64_t fls_mlp_grad_workspace_bytes(uint64_t n);
+int fls_mlp_grad(const uint16_t* d_wb, const float* d_feat, const uint8_t* d_label, uint64_t n, float inv_batch,
+                 void* d_work, float* d_grad, float* d_loss, void* stream);
+int fls_mlp_adam(float* d_w, float* d_m, float* d_v, float* d_grad, uint16_t* d_wb, uint32_t step, float lr, float beta1,
+                 float beta2, float eps, void* stream);
+int fls_mlp_mfma_selftest(const uint16_t* d_a, const uint16_t* d_b, float* d_d, void* stream);
+/* Training vectors: n_vec x 1024 values from the four families of the
+ * learned selector's workload (uniform, Zipf s=1.1 via d_zipf_cdf[4096], the
+ * CDF of ranks 1..4096 scaled to 2^32, sorted + Gaussian noise sigma 16, runs
+ * of geometric mean 32); vector v's family and parameters come from the seed. */
+int fls_gen_train_vectors(uint64_t seed, uint64_t first_vector, uint64_t n_vec, const uint32_t* d_zipf_cdf,
+                          uint32_t* d_out, void* stream);
 
 /* ============================================ batched segments (no reference)
  * Many compressed segments of uneven length, width and encoding decoded and

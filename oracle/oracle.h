@@ -28,6 +28,10 @@
  *   orc_q11 / orc_q21                  pinned: textbook semantics checked against
  *                                      numpy brute force on small tables.
  *   orc_gen_* / orc_rand*              definitions of the synthetic inputs.
+ *   orc_mix_* / orc_exhaustive_class   the encoders above run literally; sizes
+ *                                      are their word counts (brute force).
+ *   orc_vector_features                definition of the selector's input;
+ *                                      closed-form cases in the tests.
  *   orc_mlp_*                          pinned: finite differences in double.
  */
 #ifndef FLS_ORACLE_H
@@ -157,6 +161,30 @@ void orc_decode_vector(const orc_col* c, uint64_t v, uint32_t* out);
 int64_t orc_q11_encoded(const orc_col* od, const orc_col* q, const orc_col* d, const orc_col* p, uint64_t n_rows,
                         uint64_t v_begin, uint64_t v_end, uint32_t date_gt, uint32_t date_lt, uint32_t qty_lt,
                         uint32_t disc_lo, uint32_t disc_hi, int n_threads, uint64_t* count);
+
+/* ---------------------------------------------------------------- selector
+ * (oracle/selector.c; no reference counterpart.)  Mixed-vector encodings:
+ * 0 bit-pack, 1 FOR, 2 FastLanes delta, 3 FastLanes RLE, each over one
+ * 1024-value vector in original order, payload in transposed row order.
+ * orc_mix_encode_u32 returns the payload words (words: up to 1088).
+ * Class = 10*encoding + width bucket (0 | 1-2 | 3-4 | 5-6 | 7-8 | 9-12 |
+ * 13-16 | 17-20 | 21-24 | 25-32).                                             */
+unsigned orc_width_bucket(unsigned w);
+unsigned orc_mix_encode_u32(const uint32_t* in, unsigned enc, uint32_t* words, unsigned* width, uint64_t* aux);
+void orc_mix_sizes(const uint32_t* in, uint32_t* sizes4, unsigned* widths4);
+unsigned orc_exhaustive_class(const uint32_t* in);
+void orc_vector_features(const uint32_t* in, float* f);
+void orc_gen_train_vector(uint64_t seed, uint64_t v, const uint32_t* zipf_cdf, uint32_t* out);
+/* MLP 16->64->64->40 in double: parameters as fls_mlp_* (7848 values).
+ * round_bf16 != 0 rounds inputs, weights, hidden activations and the
+ * back-propagated dZ/dH2/dH1 to BF16 where the GPU kernels do.
+ * orc_mlp_grad returns the summed loss and writes d(mean loss)/dw when
+ * inv_batch = 1/n.                                                           */
+unsigned orc_mlp_param_count(void);
+double orc_bf16(double x);
+void orc_mlp_forward(const double* w, const float* feat, uint64_t n, int round_bf16, double* logits);
+double orc_mlp_grad(const double* w, const float* feat, const uint8_t* label, uint64_t n, double inv_batch,
+                    int round_bf16, double* grad);
 
 #ifdef __cplusplus
 }

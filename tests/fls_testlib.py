@@ -261,3 +261,95 @@ def o_c4_values(seed, s, lmin, lmax):
     out = np.zeros(n.value, np.uint32)
     o.orc_c4_values(seed, s, enc.value, bw.value, n.value, out)
     return enc.value, bw.value, out
+
+
+# ----------------------------------------------------------------- selector oracle
+F32P = np.ctypeslib.ndpointer(dtype=np.float32, flags="C")
+F64P = np.ctypeslib.ndpointer(dtype=np.float64, flags="C")
+U8P = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C")
+
+
+def _sel():
+    o = oracle()
+    if not hasattr(o, "_sel_set"):
+        u, u64 = ctypes.c_uint, ctypes.c_uint64
+        o.orc_width_bucket.argtypes = [u]
+        o.orc_width_bucket.restype = u
+        o.orc_mix_encode_u32.argtypes = [U32P, u, U32P, ctypes.POINTER(u), ctypes.POINTER(u64)]
+        o.orc_mix_encode_u32.restype = u
+        o.orc_mix_sizes.argtypes = [U32P, U32P, U32P]
+        o.orc_exhaustive_class.argtypes = [U32P]
+        o.orc_exhaustive_class.restype = u
+        o.orc_vector_features.argtypes = [U32P, F32P]
+        o.orc_gen_train_vector.argtypes = [u64, u64, U32P, U32P]
+        o.orc_mlp_param_count.restype = u
+        o.orc_bf16.argtypes = [ctypes.c_double]
+        o.orc_bf16.restype = ctypes.c_double
+        o.orc_mlp_forward.argtypes = [F64P, F32P, u64, ctypes.c_int, F64P]
+        o.orc_mlp_grad.argtypes = [F64P, F32P, U8P, u64, ctypes.c_double, ctypes.c_int, F64P]
+        o.orc_mlp_grad.restype = ctypes.c_double
+        o._sel_set = True
+    return o
+
+
+def o_mix_encode(vec, enc):
+    """(payload words, width, aux) of one 1024-value vector."""
+    words = np.zeros(1088, np.uint32)
+    w, aux = ctypes.c_uint(), ctypes.c_uint64()
+    n = _sel().orc_mix_encode_u32(np.ascontiguousarray(vec, np.uint32), enc, words, ctypes.byref(w),
+                                  ctypes.byref(aux))
+    return words[:n].copy(), w.value, aux.value
+
+
+def o_mix_sizes(vec):
+    s, w = np.zeros(4, np.uint32), np.zeros(4, np.uint32)
+    _sel().orc_mix_sizes(np.ascontiguousarray(vec, np.uint32), s, w)
+    return s, w
+
+
+def o_exhaustive_class(vec):
+    return int(_sel().orc_exhaustive_class(np.ascontiguousarray(vec, np.uint32)))
+
+
+def o_features(vec):
+    f = np.zeros(16, np.float32)
+    _sel().orc_vector_features(np.ascontiguousarray(vec, np.uint32), f)
+    return f
+
+
+def o_train_vector(seed, v, cdf):
+    out = np.zeros(1024, np.uint32)
+    _sel().orc_gen_train_vector(seed, v, np.ascontiguousarray(cdf, np.uint32), out)
+    return out
+
+
+def o_mlp_forward(w, feat, round_bf16):
+    feat = np.ascontiguousarray(feat, np.float32)
+    n = feat.shape[0]
+    out = np.zeros((n, 40), np.float64)
+    _sel().orc_mlp_forward(np.ascontiguousarray(w, np.float64), feat, n, int(round_bf16), out)
+    return out
+
+
+def o_mlp_grad(w, feat, label, round_bf16, inv_batch=None):
+    feat = np.ascontiguousarray(feat, np.float32)
+    n = feat.shape[0]
+    g = np.zeros(int(_sel().orc_mlp_param_count()), np.float64)
+    loss = _sel().orc_mlp_grad(np.ascontiguousarray(w, np.float64), feat, np.ascontiguousarray(label, np.uint8), n,
+                               1.0 / n if inv_batch is None else inv_batch, int(round_bf16), g)
+    return loss, g
+
+
+def pad_vectors(a):
+    """Pad a column to whole vectors by repeating its last value (the GPU
+    encoders' convention for a partial last vector)."""
+    a = np.ascontiguousarray(a, np.uint32)
+    nv = (a.size + 1023) // 1024
+    out = np.empty(nv * 1024, np.uint32)
+    out[:a.size] = a
+    out[a.size:] = a[-1] if a.size else 0
+    return out
+
+
+def o_uniform_py(seed, stream, i, rng_range):
+    return int(oracle().orc_uniform(seed, stream, i, rng_range))

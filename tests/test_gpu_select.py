@@ -1,0 +1,385 @@
+"""GPU parity of the learned-selector path against the oracle:
+MFMA lane layout, training-vector generator, features, exhaustive sizes and
+classes, the mixed-column encoder word for word, the try-all-encodings search,
+the confidence fallback, Q1.1 over mixed columns, the MLP forward/gradient
+kernels against the FP64 restatement, Adam, and what training buys."""
+import numpy as np
+import pytest
+
+import fastlanesgpu_amd as fl
+from fastlanesgpu_amd import selector as sel
+from fls_testlib import (from_dev, o_exhaustive_class, o_features, o_gen_c2, o_mix_encode, o_mix_sizes, o_mlp_forward,
+                         o_mlp_grad, o_perm, o_q11, o_train_vector, pad_vectors, to_dev, torch_cuda)
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16_bits(a):
+    """numpy round-to-nearest-even float32 -> bf16 bit patterns (uint16)."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def bf16_val(bits):
+    return (bits.astype(np.uint32) << 16).view(np.float32)
+
+
+# ------------------------------------------------------------------- MFMA
+def test_mfma_layout_integers():
+    torch = torch_cuda()
+    rng = np.random.default_rng(0)
+    a = rng.integers(-8, 9, (16, 16)).astype(np.float32)
+    b = rng.integers(-8, 9, (16, 16)).astype(np.float32)
+    da = torch.from_numpy(bf16_bits(a).view(np.int16)).cuda()
+    db = torch.from_numpy(bf16_bits(b).view(np.int16)).cuda()
+    dd = torch.zeros(256, dtype=torch.float32, device="cuda")
+    fl.call("fls_mlp_mfma_selftest", da.data_ptr(), db.data_ptr(), dd.data_ptr(), None)
+    np.testing.assert_array_equal(dd.cpu().numpy().reshape(16, 16), a @ b)
+
+
+def test_mfma_layout_random():
+    torch = torch_cuda()
+    rng = np.random.default_rng(1)
+    a = bf16_val(bf16_bits(rng.normal(size=(16, 16))))
+    b = bf16_val(bf16_bits(rng.normal(size=(16, 16))))
+    da = torch.from_numpy(bf16_bits(a).view(np.int16)).cuda()
+    db = torch.from_numpy(bf16_bits(b).view(np.int16)).cuda()
+    dd = torch.zeros(256, dtype=torch.float32, device="cuda")
+    fl.call("fls_mlp_mfma_selftest", da.data_ptr(), db.data_ptr(), dd.data_ptr(), None)
+    ref = a.astype(np.float64) @ b.astype(np.float64)
+    np.testing.assert_allclose(dd.cpu().numpy().reshape(16, 16), ref, rtol=1e-5, atol=1e-5)
+
+
+# ------------------------------------------------------------- inputs
+def train_column(seed, first, n_vec):
+    torch_cuda()
+    d = sel.gen_train_vectors(seed, first, n_vec)
+    return d, from_dev(d).copy()
+
+
+def test_train_vectors_match_oracle():
+    cdf = sel.zipf_cdf()
+    d, h = train_column(3, 37, 48)
+    for v in range(48):
+        np.testing.assert_array_equal(h[1024 * v:1024 * (v + 1)], o_train_vector(3, 37 + v, cdf), err_msg=f"v={v}")
+
+
+def mixed_inputs():
+    """(name, host values) cases: training families, Q1.1 columns, ragged tails."""
+    _, tv = train_column(5, 0, 64)
+    c2 = o_gen_c2(9, 5 * 1024 + 300)
+    rng = np.random.default_rng(2)
+    return [
+        ("train64", tv),
+        ("orderdate", c2[0]), ("quantity", c2[1]), ("discount", c2[2]), ("price", c2[3]),
+        ("tiny", rng.integers(0, 1000, 77).astype(np.uint32)),
+        ("one_vector", rng.integers(0, 1 << 31, 1024).astype(np.uint32)),
+        ("runs_tail", np.repeat(rng.integers(0, 50, 400), 7)[:2700].astype(np.uint32)),
+    ]
+
+
+CASES = None
+
+
+def cases():
+    global CASES
+    if CASES is None:
+        CASES = mixed_inputs()
+    return CASES
+
+
+@pytest.mark.parametrize("k", range(8))
+def test_features_match_oracle(k):
+    name, x = cases()[k]
+    f = from_dev(sel.vector_features(to_dev(x), x.size), np.float32).reshape(-1, 16)
+    p = pad_vectors(x)
+    for v in range(p.size // 1024):
+        np.testing.assert_array_equal(f[v], o_features(p[1024 * v:1024 * (v + 1)]), err_msg=f"{name} v={v}")
+
+
+@pytest.mark.parametrize("k", range(8))
+def test_exhaustive_matches_oracle(k):
+    name, x = cases()[k]
+    cls, s4 = sel.exhaustive_select(to_dev(x), x.size)
+    cls, s4 = from_dev(cls, np.uint8), from_dev(s4).reshape(-1, 4)
+    p = pad_vectors(x)
+    for v in range(p.size // 1024):
+        vec = p[1024 * v:1024 * (v + 1)]
+        np.testing.assert_array_equal(s4[v], o_mix_sizes(vec)[0], err_msg=f"{name} v={v}")
+        assert cls[v] == o_exhaustive_class(vec), (name, v)
+
+
+# ------------------------------------------------------------- mixed columns
+def column_parts(ws, n):
+    torch = torch_cuda()
+    torch.cuda.synchronize()
+    nv = sel.n_vectors(n)
+    total = int(ws.total.item())
+    vmeta = ws.vmeta[:nv].cpu().numpy().view(np.uint64)
+    aux = ws.aux[:nv].cpu().numpy().view(np.uint64)
+    words = ws.words[:max(1, total)].cpu().numpy().view(np.uint32)
+    cls = ws.cls[:nv].cpu().numpy()
+    return total, vmeta, aux, words, cls
+
+
+def check_against_oracle(ws, x, classes):
+    """Every vector's payload, width and aux equal the oracle encoder's."""
+    n = x.size
+    total, vmeta, aux, words, _ = column_parts(ws, n)
+    p = pad_vectors(x)
+    nv = p.size // 1024
+    off = (vmeta >> np.uint64(8)).astype(np.int64)
+    ends = np.r_[off[1:], total]
+    for v in range(nv):
+        enc = int(classes[v]) // 10
+        ow, owidth, oaux = o_mix_encode(p[1024 * v:1024 * (v + 1)], enc)
+        assert int(vmeta[v] & np.uint64(0xFF)) == (enc << 6) | owidth, v
+        assert int(aux[v]) == oaux, v
+        np.testing.assert_array_equal(words[off[v]:ends[v]], ow, err_msg=f"v={v} enc={enc}")
+
+
+@pytest.mark.parametrize("k", range(8))
+def test_mixed_encode_random_classes(k):
+    torch = torch_cuda()
+    name, x = cases()[k]
+    n = x.size
+    nv = sel.n_vectors(n)
+    classes = np.random.default_rng(k).integers(0, 40, nv).astype(np.uint8)
+    ws = sel.MixedWorkspace(n)
+    ws.cls[:nv] = torch.from_numpy(classes).cuda()
+    d = to_dev(x)
+    sel.mixed_encode_classes(d, n, ws)
+    check_against_oracle(ws, x, classes)
+    out = from_dev(sel.mixed_decode(ws.column(n)))
+    p = pad_vectors(x)
+    for v in range(nv):
+        np.testing.assert_array_equal(out[1024 * v:1024 * (v + 1)],
+                                      o_perm("orc_transpose_i_u32", p[1024 * v:1024 * (v + 1)]), err_msg=f"v={v}")
+
+
+@pytest.mark.parametrize("k", range(8))
+def test_trial_equals_exhaustive(k):
+    name, x = cases()[k]
+    n = x.size
+    d = to_dev(x)
+    a, b = sel.MixedWorkspace(n), sel.MixedWorkspace(n, trial=True)
+    sel.mixed_encode_exhaustive(d, n, a)
+    sel.mixed_encode_trial(d, n, b)
+    pa, pb = column_parts(a, n), column_parts(b, n)
+    assert pa[0] == pb[0]
+    for u, w in zip(pa[1:], pb[1:]):
+        np.testing.assert_array_equal(u, w)
+    check_against_oracle(b, x, pb[4])
+
+
+def test_confidence_fallback():
+    torch = torch_cuda()
+    _, x = cases()[0]
+    n = x.size
+    nv = sel.n_vectors(n)
+    d = to_dev(x)
+    ex = sel.MixedWorkspace(n)
+    sel.mixed_encode_exhaustive(d, n, ex)
+    classes = torch.from_numpy(np.random.default_rng(4).integers(0, 40, nv).astype(np.uint8)).cuda()
+    # every vector below the threshold -> exhaustive everywhere
+    ws = sel.MixedWorkspace(n)
+    ws.cls[:nv] = classes
+    ws.fallbacks.zero_()
+    conf = torch.full((nv,), 0.25, dtype=torch.float32, device="cuda")
+    sel.mixed_plan(d, n, ws, conf=conf, conf_min=0.5)
+    sel.mixed_pack(d, n, ws)
+    assert int(ws.fallbacks.item()) == nv
+    for u, w in zip(column_parts(ws, n), column_parts(ex, n)):
+        np.testing.assert_array_equal(u, w)
+    # half below the threshold
+    ws2 = sel.MixedWorkspace(n)
+    ws2.cls[:nv] = classes
+    ws2.fallbacks.zero_()
+    conf2 = torch.from_numpy(np.where(np.arange(nv) % 2 == 0, 0.1, 0.95).astype(np.float32)).cuda()
+    sel.mixed_plan(d, n, ws2, conf=conf2, conf_min=0.5)
+    sel.mixed_pack(d, n, ws2)
+    assert int(ws2.fallbacks.item()) == (nv + 1) // 2
+    c_ex, c_rand = column_parts(ex, n)[4], classes.cpu().numpy()
+    expect = np.where(np.arange(nv) % 2 == 0, c_ex, c_rand)
+    check_against_oracle(ws2, x, expect)
+    np.testing.assert_array_equal(column_parts(ws2, n)[4], expect)
+
+
+@pytest.mark.parametrize("mode", ["exhaustive", "random"])
+def test_q11_over_mixed_columns(mode):
+    torch = torch_cuda()
+    n = 3 * (1 << 20) + 777
+    cols = o_gen_c2(17, n)
+    rng = np.random.default_rng(8)
+    enc_cols, keep = [], []
+    for c in cols:
+        d = to_dev(c)
+        ws = sel.MixedWorkspace(n)
+        if mode == "exhaustive":
+            sel.mixed_encode_exhaustive(d, n, ws)
+        else:
+            nv = sel.n_vectors(n)
+            ws.cls[:nv] = torch.from_numpy(rng.integers(0, 40, nv).astype(np.uint8)).cuda()
+            sel.mixed_encode_classes(d, n, ws)
+        enc_cols.append(ws.column(n))
+        keep.append(ws)
+    ref_rev, ref_cnt = o_q11(cols, n)
+    for pred in (False, True):
+        rev, cnt = fl.query_q11(*enc_cols, n, pred_load=pred)
+        assert (rev, cnt) == (ref_rev, ref_cnt), (mode, pred)
+
+
+# ------------------------------------------------------------------- MLP
+def rand_weights(seed):
+    rng = np.random.default_rng(seed)
+    w = sel.Mlp.init_weights(seed)
+    return (w + rng.normal(0, 0.05, w.size)).astype(np.float32)
+
+
+def test_mlp_forward_matches_oracle():
+    torch = torch_cuda()
+    w = rand_weights(3)
+    mlp = sel.Mlp(weights=w)
+    rng = np.random.default_rng(4)
+    n = 200  # 12.5 tiles of 16
+    feat = rng.random((n, 16)).astype(np.float32)
+    cls, conf, lg = mlp.forward(torch.from_numpy(feat).cuda(), n, logits=True)
+    lg = lg.cpu().numpy()
+    ref = o_mlp_forward(w.astype(np.float64), feat, True)
+    assert np.max(np.abs(lg - ref)) <= 1e-2 * max(1.0, np.max(np.abs(ref)))
+    # class/confidence follow from the kernel's own logits
+    z = lg.astype(np.float64)
+    e = np.exp(z - z.max(1, keepdims=True))
+    mass = e.reshape(n, 4, 10).sum(2)
+    be = mass.argmax(1)
+    np.testing.assert_array_equal(cls.cpu().numpy(), 10 * be + z.reshape(n, 4, 10)[np.arange(n), be].argmax(1))
+    np.testing.assert_allclose(conf.cpu().numpy(), mass.max(1) / mass.sum(1), rtol=1e-5)
+
+
+def test_mlp_grad_matches_oracle():
+    torch = torch_cuda()
+    w = rand_weights(5)
+    mlp = sel.Mlp(weights=w)
+    rng = np.random.default_rng(6)
+    n = 200
+    feat = rng.random((n, 16)).astype(np.float32)
+    label = rng.integers(0, 40, n).astype(np.uint8)
+    df, dl = torch.from_numpy(feat).cuda(), torch.from_numpy(label).cuda()
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    mlp.grad.zero_()
+    mlp.accumulate_grad(df, dl, n, 1.0 / n, loss)
+    g = mlp.grad.cpu().numpy().astype(np.float64)
+    ref_loss, ref = o_mlp_grad(w.astype(np.float64), feat, label, True)
+    assert float(loss.item()) == pytest.approx(ref_loss, rel=2e-3)
+    groups = [(0, 1024), (1024, 1088), (1088, 5184), (5184, 5248), (5248, 7808), (7808, 7848)]
+    for a, b in groups:
+        scale = np.max(np.abs(ref[a:b]))
+        assert np.max(np.abs(g[a:b] - ref[a:b])) <= 2e-2 * scale + 1e-7, (a, b)
+        assert np.corrcoef(g[a:b], ref[a:b])[0, 1] > 0.999, (a, b)
+    # bit-reproducible: a second accumulation from zero gives the same bits
+    g1 = mlp.grad.clone()
+    mlp.grad.zero_()
+    mlp.accumulate_grad(df, dl, n, 1.0 / n, None)
+    assert torch.equal(g1, mlp.grad)
+    # accumulation: two halves add up to the whole (up to FP32 rounding)
+    mlp.grad.zero_()
+    mlp.accumulate_grad(df[:96], dl[:96], 96, 1.0 / n, None)
+    mlp.accumulate_grad(df[96:], dl[96:], n - 96, 1.0 / n, None)
+    np.testing.assert_allclose(mlp.grad.cpu().numpy(), g1.cpu().numpy(), rtol=1e-4, atol=1e-6)
+
+
+def test_adam_step():
+    torch = torch_cuda()
+    w = rand_weights(7)
+    mlp = sel.Mlp(weights=w)
+    g = np.random.default_rng(8).normal(0, 1e-2, w.size).astype(np.float32)
+    mlp.grad.copy_(torch.from_numpy(g).cuda())
+    lr, b1, b2, eps = 1e-2, 0.9, 0.999, 1e-8
+    mlp.adam(lr, b1, b2, eps)
+    m = (1 - b1) * g.astype(np.float64)
+    v = (1 - b2) * g.astype(np.float64) ** 2
+    ref = w - lr * (m / (1 - b1)) / (np.sqrt(v / (1 - b2)) + eps)
+    np.testing.assert_allclose(mlp.w.cpu().numpy(), ref, rtol=1e-5, atol=1e-6)
+    assert not mlp.grad.any()
+    np.testing.assert_array_equal(mlp.wb.cpu().numpy().view(np.uint16), bf16_bits(mlp.w.cpu().numpy()))
+
+
+# ------------------------------------------------------------- training
+@pytest.fixture(scope="module")
+def trained():
+    torch_cuda()
+    feat, label, _ = sel.build_training_set(21, 1 << 15)
+    mlp = sel.Mlp(21)
+    losses = mlp.train(feat, label, epochs=3, batch=1024, lr=3e-3, seed=21)
+    return mlp, losses
+
+
+def size_of(ws):
+    return int(ws.total.item()) * 4 + 16 * ws.vmeta.numel()
+
+
+def test_training_reduces_loss(trained):
+    _, losses = trained
+    assert losses[-1] < 0.6 * losses[0], losses
+
+
+def test_learned_vs_without_network(trained):
+    """Held-out vectors: the trained network's encode is within 1% of the
+    exhaustive size with few fallbacks; without it (untrained weights, or one
+    fixed encoding for every vector) the size is clearly worse."""
+    torch = torch_cuda()
+    mlp, _ = trained
+    n_vec = 1 << 12
+    d = sel.gen_train_vectors(22, 0, n_vec)
+    n = n_vec * 1024
+    ex = sel.MixedWorkspace(n)
+    sel.mixed_encode_exhaustive(d, n, ex)
+    best = size_of(ex)
+
+    ws = sel.MixedWorkspace(n)
+    ws.fallbacks.zero_()
+    learned = sel.LearnedSelector(mlp, conf_min=0.9)
+    learned.encode(d, n, ws)
+    torch.cuda.synchronize()
+    err = size_of(ws) / best - 1
+    fallback_rate = int(ws.fallbacks.item()) / n_vec
+    assert err <= 0.01, err
+    assert fallback_rate < 0.5, fallback_rate
+    # network alone (no fallback) already picks the exhaustive encoding for most vectors
+    ws0 = sel.MixedWorkspace(n)
+    sel.LearnedSelector(mlp, conf_min=0.0).encode(d, n, ws0)
+    ex_enc = ex.cls[:n_vec].cpu().numpy() // 10
+    net_enc = ws0.cls[:n_vec].cpu().numpy() // 10
+    s4 = sel.exhaustive_select(d, n)[1].cpu().numpy().reshape(-1, 4)
+    same_size = s4[np.arange(n_vec), net_enc] == s4[np.arange(n_vec), ex_enc]
+    assert same_size.mean() > 0.9, same_size.mean()
+    # without the trained network
+    wu = sel.MixedWorkspace(n)
+    sel.LearnedSelector(sel.Mlp(99), conf_min=0.0).encode(d, n, wu)
+    assert size_of(wu) / best - 1 > 0.05
+    for e in range(4):
+        wf = sel.MixedWorkspace(n)
+        wf.cls.fill_(10 * e)
+        sel.mixed_encode_classes(d, n, wf)
+        assert size_of(wf) / best - 1 > 0.05, e
+
+
+def test_learned_q11_exact(trained):
+    """The learned selector encodes held-out Q1.1 columns; the query over its
+    output is exact and the size within 1% of exhaustive search."""
+    mlp, _ = trained
+    n = 2 * (1 << 20) + 1234
+    cols = o_gen_c2(23, n)
+    enc, keep, tot_l, tot_x = [], [], 0, 0
+    for c in cols:
+        d = to_dev(c)
+        ws, ex = sel.MixedWorkspace(n), sel.MixedWorkspace(n)
+        sel.LearnedSelector(mlp, conf_min=0.9).encode(d, n, ws)
+        sel.mixed_encode_exhaustive(d, n, ex)
+        tot_l += size_of(ws)
+        tot_x += size_of(ex)
+        enc.append(ws.column(n))
+        keep.append(ws)
+    assert tot_l <= 1.01 * tot_x, (tot_l, tot_x)
+    assert fl.query_q11(*enc, n) == o_q11(cols, n)
