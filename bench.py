@@ -95,6 +95,29 @@ def copy_bandwidth(n_bytes=1 << 30, reps=20):
     return 2 * n_bytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9, ok
 
 
+def read_bandwidth(n_bytes=1 << 30, reps=20):
+    """Read-only GB/s of the library's streaming checksum kernel (the
+    achievable read bandwidth that the decode kernels are held against)."""
+    import torch
+
+    import fastlanesgpu_amd as fls
+
+    src = torch.randint(0, 1 << 30, (n_bytes // 4,), dtype=torch.int32, device="cuda")
+    out = torch.zeros(1, dtype=torch.int64, device="cuda")
+    lib, s = fls.load(), fls.current_stream()
+    for _ in range(3):
+        fls.call("fls_read_bytes", src.data_ptr(), n_bytes, out.data_ptr(), s)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        lib.fls_read_bytes(src.data_ptr(), n_bytes, out.data_ptr(), s)
+    e1.record()
+    torch.cuda.synchronize()
+    ok = int(out.item()) == int(src.long().sum().item())
+    return n_bytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9, ok
+
+
 def bench_c1(steps, warmup, seed):
     """Plain bit-unpack: 2^24 values at bw 11, decode + write + checksum."""
     import torch
@@ -408,6 +431,7 @@ def main():
     achieved = total_bytes / (k_ms * 1e-3) / 1e9
     log("[bench] copy-kernel bandwidth")
     copy_gbs, copy_ok = copy_bandwidth()
+    read_gbs, read_ok = read_bandwidth()
     line = {
         "metric": METRIC,
         "value": n * args.steps / (total_ms * 1e-3),
@@ -443,6 +467,9 @@ def main():
             "copy_kernel_GBs": copy_gbs,
             "copy_kernel_ok": copy_ok,
             "frac_of_copy": achieved / copy_gbs,
+            "read_only_GBs": read_gbs,
+            "read_only_ok": read_ok,
+            "frac_of_read_only": achieved / read_gbs,
         },
         "result": {"revenue": result[0], "rows": result[1]},
         "pred_load": {

@@ -56,6 +56,7 @@ SIGNATURES.update({
     "fls_column_bytes": (c_u64, [COLP, c_u64]),
     "fls_gen_c1": (c_int, [c_u64, c_u64, c_u32, c_vp, c_vp]),
     "fls_copy_bytes": (c_int, [c_vp, c_vp, c_u64, c_vp]),
+    "fls_read_bytes": (c_int, [c_vp, c_u64, c_vp, c_vp]),
     "fls_gen_c2": (c_int, [c_u64, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "fls_gen_c3_fact": (c_int, [c_u64, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "fls_gen_c3_dims": (c_int, [c_u64, c_u32, c_u32, c_u32, c_u32] + [c_vp] * 8),

@@ -40,6 +40,7 @@ struct SegDev {
     const fls_vmeta* vmeta;
     const uint32_t* bases;
     const fls_rle_meta* rmeta;
+    const uint64_t* aux;  // MIXED: per-vector aux words
     uint32_t lo, hi;
 };
 
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(const SegDev* __restrict s
         if (t >= n_tiles) break;  // uniform over the block
         const TileDev tile = tiles[t];
         const SegDev sg = segs[tile.seg];
-        const ColDev col{sg.enc, sg.bw, 0u, sg.enc != FLS_ENC_BITPACK, sg.words, sg.vmeta, sg.bases};
+        const ColDev col{sg.enc, sg.bw, 0u, sg.enc != FLS_ENC_BITPACK, sg.words, sg.vmeta, sg.bases, sg.aux};
         const uint32_t span = sg.hi - sg.lo;
         uint64_t cnt = 0;
         for (uint32_t vb = 0; vb < tile.nv; vb += VPB) {
@@ -80,6 +81,8 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(const SegDev* __restrict s
             uint32_t x[32];
             if (sg.enc == FLS_ENC_RLE) {
                 decode_rle(col, sg.rmeta, v, stage[threadIdx.x >> 5], live, x);
+            } else if (sg.enc == FLS_ENC_MIXED) {
+                decode_mixed(col, v, stage[threadIdx.x >> 5], live, x);
             } else if (live) {
                 if (sg.enc == FLS_ENC_DELTA) decode_delta(col, v, x);
                 else dev::unpack_device(sg.words + v * (32u * sg.bw), x, sg.bw);
@@ -383,13 +386,17 @@ int fls_batch_create(const fls_segment* segs, uint32_t n_segments, fls_batch** o
         const fls_segment& g = segs[s];
         const fls_column& c = g.col;
         ColDev tmp;
-        int st = to_dev_column(c, g.n_values, &tmp, true);
-        if (c.encoding == FLS_ENC_RLE) st = (g.n_values && (!c.vmeta || !c.bases || !g.rmeta)) ? FLS_ERR_NULL : FLS_OK;
+        // RLE metadata comes from the segment's rmeta or, when that is NULL, the column's aux
+        const fls_rle_meta* rmeta = g.rmeta ? g.rmeta : reinterpret_cast<const fls_rle_meta*>(c.aux);
+        fls_column cc = c;
+        if (c.encoding == FLS_ENC_RLE) cc.aux = reinterpret_cast<const uint64_t*>(rmeta);
+        const int st = to_dev_column(cc, g.n_values, &tmp, true);
         if (st) {
             delete b;
             return st;
         }
-        b->segs.push_back(SegDev{c.encoding, c.bw, g.n_values, c.words, c.vmeta, c.bases, g.rmeta, g.lo, g.hi});
+        b->segs.push_back(
+            SegDev{c.encoding, c.bw, g.n_values, c.words, c.vmeta, c.bases, rmeta, c.aux, g.lo, g.hi});
         b->info.push_back(fls_segment_info{c.encoding, c.bw, g.n_values, g.lo, g.hi, c.n_words});
         if (c.encoding <= FLS_ENC_RLE) b->count_by_enc[c.encoding]++;
         b->encoded_bytes += 4ull * c.n_words;
@@ -505,7 +512,7 @@ int fls_batch_create_c4(uint64_t seed, uint32_t n_segments, uint32_t log2_min_le
         const bool meta = P[i].enc != FLS_ENC_BITPACK;
         b->segs.push_back(SegDev{P[i].enc, P[i].bw, P[i].len, w, meta ? b->d_vmeta + meta_off[i] : nullptr,
                                  meta ? b->d_bases + 32 * meta_off[i] : nullptr,
-                                 P[i].enc == FLS_ENC_RLE ? b->d_rmeta + meta_off[i] : nullptr, lo, hi});
+                                 P[i].enc == FLS_ENC_RLE ? b->d_rmeta + meta_off[i] : nullptr, nullptr, lo, hi});
         const uint64_t seg_words = P[i].enc == FLS_ENC_BITPACK ? ceil_div(P[i].len, 1024) * 32u * P[i].bw : totals[i];
         b->info.push_back(fls_segment_info{P[i].enc, P[i].bw, P[i].len, lo, hi, seg_words});
         const uint64_t nv = ceil_div(P[i].len, 1024);

@@ -1,4 +1,6 @@
 // Status strings, HIP error capture and tiny utility kernels of the C ABI.
+#include "fls/device/wave.hpp"
+#include "fls_columns.hpp"
 #include "fls_runtime.hpp"
 
 namespace fls {
@@ -24,10 +26,47 @@ int zero_u64(uint64_t* d, uint64_t n, hipStream_t stream) {
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// 16-byte-per-lane streaming copy, grid-stride.
+// 16-byte-per-lane streaming copy, grid-stride, four independent loads in
+// flight per work-item.
 __global__ __launch_bounds__(256) void copy_kernel(const u32x4* __restrict src, u32x4* __restrict dst, uint64_t n16) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
-        __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n16; i += 4 * stride) {
+        const u32x4 a = __builtin_nontemporal_load(src + i), b = __builtin_nontemporal_load(src + i + stride);
+        const u32x4 c = __builtin_nontemporal_load(src + i + 2 * stride);
+        const u32x4 d = __builtin_nontemporal_load(src + i + 3 * stride);
+        __builtin_nontemporal_store(a, dst + i);
+        __builtin_nontemporal_store(b, dst + i + stride);
+        __builtin_nontemporal_store(c, dst + i + 2 * stride);
+        __builtin_nontemporal_store(d, dst + i + 3 * stride);
+    }
+    for (; i < n16; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
+// Read-only stream: sum of all 32-bit words mod 2^64 (the achievable read
+// bandwidth that decode kernels are compared with).  Four independent 16-byte
+// loads in flight per work-item; block reduction, one atomic per block.
+__global__ __launch_bounds__(256) void read_kernel(const u32x4* __restrict src, uint64_t n16,
+                                                   unsigned long long* __restrict sum) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t acc = 0;
+    for (; i + 3 * stride < n16; i += 4 * stride) {
+        const u32x4 a = __builtin_nontemporal_load(src + i), b = __builtin_nontemporal_load(src + i + stride);
+        const u32x4 c = __builtin_nontemporal_load(src + i + 2 * stride);
+        const u32x4 d = __builtin_nontemporal_load(src + i + 3 * stride);
+        acc += (uint64_t)a.x + a.y + a.z + a.w + b.x + b.y + b.z + b.w;
+        acc += (uint64_t)c.x + c.y + c.z + c.w + d.x + d.y + d.z + d.w;
+    }
+    for (; i < n16; i += stride) {
+        const u32x4 a = __builtin_nontemporal_load(src + i);
+        acc += (uint64_t)a.x + a.y + a.z + a.w;
+    }
+    __shared__ uint64_t part[4];
+    acc = dev::wave_sum(acc);
+    if (dev::is_wave_leader()) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(sum, (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
 }
 
 }  // namespace fls
@@ -43,6 +82,20 @@ int fls_copy_bytes(const void* d_src, void* d_dst, uint64_t n_bytes, void* strea
     if (blocks > 256u * 16u) blocks = 256u * 16u;
     hipLaunchKernelGGL(fls::copy_kernel, dim3((uint32_t)blocks), dim3(256), 0, fls::as_stream(stream),
                        static_cast<const fls::u32x4*>(d_src), static_cast<fls::u32x4*>(d_dst), n16);
+    return fls::check_launch();
+}
+
+int fls_read_bytes(const void* d_src, uint64_t n_bytes, uint64_t* d_sum, void* stream) {
+    if (!d_sum) return FLS_ERR_NULL;
+    if (const int rc = fls::zero_u64(d_sum, 1, fls::as_stream(stream))) return rc;
+    if (n_bytes == 0) return FLS_OK;
+    if (!d_src) return FLS_ERR_NULL;
+    if ((n_bytes & 15u) || ((uintptr_t)d_src & 15u)) return FLS_ERR_ARG;
+    const uint64_t n16 = n_bytes / 16u;
+    uint32_t blocks = 0;
+    if (const int rc = fls::resident_grid(fls::read_kernel, 256, fls::ceil_div(n16, 256), &blocks)) return rc;
+    hipLaunchKernelGGL(fls::read_kernel, dim3(blocks), dim3(256), 0, fls::as_stream(stream),
+                       static_cast<const fls::u32x4*>(d_src), n16, reinterpret_cast<unsigned long long*>(d_sum));
     return fls::check_launch();
 }
 

@@ -6,8 +6,8 @@
 //    (encoding x width bucket).  The size is computed from the statistics the
 //    encoder itself would derive (widths, run count), so this IS the size the
 //    pack step produces.
-//  * features: 16 cheap statistics of a 1/8 sample of the vector (the first 4
-//    rows of every lane's chain) -- the learned selector's input.
+//  * features: 16 cheap statistics of a 1/8 sample of the vector (eight
+//    64-byte segments) -- the learned selector's input.
 //  * mixed column: each vector stored with its own encoding, all in FastLanes
 //    transposed row order so mixed columns of one table stay row-aligned.
 //    plan (chosen encoding's statistics -> size) -> scan -> pack.
@@ -176,13 +176,15 @@ __global__ __launch_bounds__(kBlock) void exhaustive_kernel(const uint32_t* __re
     }
 }
 
-// 16 features of a 1/8 sample: rows c0..c0+3 of every lane's chain
+// 16 features of a 1/8 sample: eight 16-row (64-byte) segments, rows
+// 128k..128k+15, so the sample reads 1/8 of the vector's memory lines; lane l
+// takes rows 128*(l/4) + 4*(l%4) + 0..3
 __global__ __launch_bounds__(kBlock) void features_kernel(const uint32_t* __restrict in, uint64_t n, uint64_t n_vec,
                                                           float* __restrict feat) {
     const uint64_t v = (uint64_t)blockIdx.x * kVecPerBlock + (threadIdx.x >> 5);
     if (v >= n_vec) return;
     const uint32_t l = dev::lane32();
-    const uint64_t c0 = v * 1024u + 32u * chain_slot(l);
+    const uint64_t c0 = v * 1024u + 128u * (l >> 2) + 4u * (l & 3u);
     uint32_t s[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) s[j] = padded(in, n, c0 + j);

@@ -54,6 +54,10 @@ const char* fls_version(void);
  * calibration kernel for bandwidth and HBM-counter measurements.  No
  * counterpart in the reference.                                              */
 int fls_copy_bytes(const void* d_src, void* d_dst, uint64_t n_bytes, void* stream);
+/* Read-only streaming pass: *d_sum = sum of the 32-bit words mod 2^64.  The
+ * achievable read bandwidth the decode kernels are compared with (16-byte
+ * aligned source, n_bytes a multiple of 16).  No reference counterpart.     */
+int fls_read_bytes(const void* d_src, uint64_t n_bytes, uint64_t* d_sum, void* stream);
 
 /* ========================================================= host encoders (CPU)
  * One 1024-value vector.  Replaces
@@ -289,7 +293,8 @@ uint64_t fls_column_bytes(const fls_column* col, uint64_t n_rows);
  * nearest helper is find_bw, ssb_utils.h:222-239).
  * fls_exhaustive_select: for every vector the exact encoded size of all four
  * encodings (d_sizes4[4*v+e], may be NULL) and the smallest class d_class[v].
- * fls_vector_features: 16 float features per vector from a 1/8 sample.      */
+ * fls_vector_features: 16 float features per vector from a 1/8 sample (rows
+ * 128k..128k+15, k < 8: one 64-byte segment in every 512 bytes).           */
 int fls_exhaustive_select(const uint32_t* d_in, uint64_t n, uint8_t* d_class, uint32_t* d_sizes4, void* stream);
 int fls_vector_features(const uint32_t* d_in, uint64_t n, float* d_feat, void* stream);
 /* Mixed column from per-vector classes (only each vector's chosen encoding is
@@ -361,8 +366,8 @@ int fls_gen_train_vectors(uint64_t seed, uint64_t first_vector, uint64_t n_vec, 
 typedef struct fls_batch fls_batch; /* opaque */
 
 typedef struct fls_segment {
-    fls_column col;             /* BITPACK, DELTA or RLE (device buffers) */
-    const fls_rle_meta* rmeta;  /* RLE only */
+    fls_column col;             /* BITPACK, DELTA, RLE or MIXED (device buffers) */
+    const fls_rle_meta* rmeta;  /* RLE only; NULL: taken from col.aux */
     uint64_t n_values;
     uint32_t lo, hi;            /* range predicate */
 } fls_segment;

@@ -93,13 +93,13 @@ unsigned orc_exhaustive_class(const uint32_t* in) {
 }
 
 /* ---------------------------------------------------------------- features
- * Sample: rows c0..c0+3 of each lane's chain, c0 = 32*(2*(l%16) + l/16), i.e.
- * the first 4 rows of every 32-row block of the vector.                      */
+ * Sample: 32 groups of 4 consecutive rows, group g = rows 128*(g/4) + 4*(g%4)
+ * + 0..3, i.e. rows 128k..128k+15 for k = 0..7.                              */
 void orc_vector_features(const uint32_t* in, float* f) {
     uint32_t orv = 0, mn = 0xffffffffu, mx = 0, dor = 0, zero = 0, nondec = 0, absmax = 0, dsum = 0, big = 0;
     uint32_t eq = 0, fmin = 0xffffffffu, fmax = 0, wsum = 0, at_min = 0;
     for (unsigned blk = 0; blk < 32; ++blk) {
-        const uint32_t* s = in + 32 * blk;
+        const uint32_t* s = in + 128 * (blk / 4) + 4 * (blk % 4);
         for (unsigned j = 0; j < 4; ++j) {
             orv |= s[j];
             if (s[j] < mn) mn = s[j];
@@ -121,7 +121,7 @@ void orc_vector_features(const uint32_t* in, float* f) {
         if (s[0] > fmax) fmax = s[0];
     }
     for (unsigned blk = 0; blk < 32; ++blk)
-        for (unsigned j = 0; j < 4; ++j) at_min += in[32 * blk + j] == mn;
+        for (unsigned j = 0; j < 4; ++j) at_min += in[128 * (blk / 4) + 4 * (blk % 4) + j] == mn;
     f[0] = orc_bit_width(orv) / 32.f;
     f[1] = orc_bit_width(mx - mn) / 32.f;
     f[2] = orc_bit_width(dor) / 32.f;

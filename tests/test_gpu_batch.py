@@ -84,9 +84,31 @@ def test_caller_described_segments():
     torch = T.torch_cuda()
     rng = np.random.default_rng(3)
     segs, want = [], []
-    specs = [("bitpack", 5000, 9), ("delta", 70_000, 0), ("rle", 33_333, 14), ("bitpack", 1, 3), ("rle", 1024, 1)]
+    specs = [("bitpack", 5000, 9), ("delta", 70_000, 0), ("rle", 33_333, 14), ("bitpack", 1, 3), ("rle", 1024, 1),
+             ("mixed", 50_000, 13), ("rle_aux", 9000, 20), ("mixed", 700, 31)]
+    from fastlanesgpu_amd import selector as sel
+
+    keep = []
     for kind, n, bw in specs:
-        if kind == "bitpack":
+        if kind == "mixed":
+            x = np.repeat(rng.integers(0, 2**bw, n // 5 + 1), 5)[:n].astype(np.uint32)
+            x[: n // 3] = np.sort(x[: n // 3])
+            ws = sel.MixedWorkspace(n)
+            nv = sel.n_vectors(n)
+            ws.cls[:nv] = torch.from_numpy(rng.integers(0, 40, nv).astype(np.uint8)).cuda()
+            d_x = T.to_dev(x)
+            sel.mixed_encode_classes(d_x, n, ws)
+            col = ws.column(n)
+            keep += [ws, d_x]
+            rmeta = None
+        elif kind == "rle_aux":
+            starts = rng.random(n) < 0.125
+            starts[0] = True
+            x = rng.integers(0, 2**bw, n).astype(np.uint32)[np.cumsum(starts) - 1]
+            w, vm, rm, bs = fls.rle_encode_device(T.to_dev(x), n)
+            col = fls.Column(fls.ENC_RLE, n, w, vmeta=vm, bases=bs, aux=rm)
+            rmeta = None
+        elif kind == "bitpack":
             x = rng.integers(0, 2**bw, n).astype(np.uint32)
             col = fls.Column(fls.ENC_BITPACK, n, fls.pack_column_device(T.to_dev(x), n, bw), bw=bw)
             rmeta = None

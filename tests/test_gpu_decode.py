@@ -97,3 +97,20 @@ def test_copy_kernel():
     torch.cuda.synchronize()
     assert torch.equal(src, dst)
     assert fls.load().fls_copy_bytes(src.data_ptr(), dst.data_ptr(), 10, None) == -6
+
+
+@pytest.mark.parametrize("n_words", [4, 1 << 20, (1 << 22) + 12])
+def test_copy_and_read_kernels_sizes(n_words):
+    """The unrolled copy and read-only kernels at sizes that exercise the
+    four-in-flight main loop and the remainder loop."""
+    torch = T.torch_cuda()
+    g = torch.Generator(device="cpu").manual_seed(n_words)
+    src = torch.randint(-(1 << 31), (1 << 31) - 1, (n_words,), dtype=torch.int32, generator=g).cuda()
+    dst = torch.zeros_like(src)
+    fls.call("fls_copy_bytes", src.data_ptr(), dst.data_ptr(), n_words * 4, fls.current_stream())
+    out = torch.full((1,), 12345, dtype=torch.int64, device="cuda")
+    fls.call("fls_read_bytes", src.data_ptr(), n_words * 4, out.data_ptr(), fls.current_stream())
+    torch.cuda.synchronize()
+    assert torch.equal(src, dst)
+    expect = int(src.cpu().numpy().view(np.uint32).astype(np.uint64).sum()) % (1 << 64)
+    assert int(out.cpu().numpy().view(np.uint64)[0]) == expect

@@ -240,3 +240,46 @@ def test_rle_roundtrip_and_run_count():
         out = np.zeros(1024, np.uint32)
         o.orc_rle_decode_u32(hdr, values, idx, out)
         np.testing.assert_array_equal(out, T.o_perm("orc_transpose_i_u32", full))
+
+
+def test_c4_workload_distribution():
+    """The batched-segments workload: 40/30/30 bitpack/delta/RLE, widths 1..32,
+    lengths log-uniform by octave in [2^10, 2^24), sorted delta segments, RLE
+    runs of geometric mean 8, range = middle half of [min, max]."""
+    import ctypes
+
+    T.o_c4_counts(1, 1, 10, 24, 1)  # sets argtypes
+    o = T.oracle()
+    enc, bw, ln = [], [], []
+    for s in range(4096):
+        e, b, n = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+        o.orc_c4_params(1, s, 10, 24, ctypes.byref(e), ctypes.byref(b), ctypes.byref(n))
+        enc.append(e.value)
+        bw.append(b.value)
+        ln.append(n.value)
+    enc, bw, ln = np.array(enc), np.array(bw), np.array(ln)
+    frac = np.bincount(enc, minlength=3) / 4096
+    assert abs(frac[0] - 0.4) < 0.03 and abs(frac[1] - 0.3) < 0.03 and abs(frac[2] - 0.3) < 0.03
+    assert bw.min() == 1 and bw.max() == 32
+    assert ln.min() >= 1 << 10 and ln.max() < 1 << 24
+    octave = np.floor(np.log2(ln)).astype(int)
+    counts = np.bincount(octave, minlength=24)[10:24]
+    assert counts.min() > 4096 / 14 * 0.7  # every octave about equally likely
+    small = [s for s in range(4096) if ln[s] <= 1 << 13][:40]
+    runs = []
+    c_all, lo_all, hi_all = T.o_c4_counts(1, max(small) + 1, 10, 24, 8)
+    for s in small:
+        e, b, v = T.o_c4_values(1, s, 10, 24)
+        assert v.size == ln[s]
+        if e != 1:
+            assert int(v.max()) < (1 << b)
+        else:  # sorted; consecutive steps below 2^bw
+            d = np.diff(v.astype(np.int64))
+            assert np.all(d >= 0) and int(d.max()) < (1 << b)
+        if e == 2:
+            runs.append(v.size / (1 + np.count_nonzero(np.diff(v.astype(np.int64)))))
+        lo = int(v.min()) + (int(v.max()) - int(v.min())) // 4
+        hi = int(v.min()) + (int(v.max()) - int(v.min())) * 3 // 4
+        assert (lo_all[s], hi_all[s]) == (lo, hi)
+        assert c_all[s] == np.count_nonzero((v >= lo) & (v <= hi))
+    assert runs and 5 < np.mean(runs) < 11  # geometric mean 8 (equal neighbours merge)
