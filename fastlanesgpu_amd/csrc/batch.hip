@@ -78,24 +78,15 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(const SegDev* __restrict s
             const uint32_t local = vb + (threadIdx.x >> 5);
             const bool live = local < tile.nv;
             const uint64_t v = (uint64_t)tile.v0 + local;
-            uint32_t x[32];
-            if (sg.enc == FLS_ENC_RLE) {
-                decode_rle(col, sg.rmeta, v, stage[threadIdx.x >> 5], live, x);
-            } else if (sg.enc == FLS_ENC_MIXED) {
-                decode_mixed(col, v, stage[threadIdx.x >> 5], live, x);
-            } else if (live) {
-                if (sg.enc == FLS_ENC_DELTA) decode_delta(col, v, x);
-                else dev::unpack_device(sg.words + v * (32u * sg.bw), x, sg.bw);
-            }
+            const uint32_t lo = sg.lo;
+            const uint32_t m = column_mask(col, sg.rmeta, v, stage[threadIdx.x >> 5], live,
+                                           [lo, span](uint32_t x) { return (x - lo) <= span; });
             if (live) {
                 const uint64_t first = v * 1024u;
                 const uint32_t valid = first + 1024u <= sg.n_values
                                            ? 0xffffffffu
                                            : live_row_mask((uint32_t)(sg.n_values - first), col.transposed != 0);
-                uint32_t c = 0;
-#pragma unroll
-                for (int r = 0; r < 32; ++r) c += (uint32_t)((x[r] - sg.lo) <= span) & ((valid >> r) & 1u);
-                cnt += c;
+                cnt += (uint32_t)__builtin_popcount(m & valid);
             }
         }
         cnt = dev::wave_sum(cnt);

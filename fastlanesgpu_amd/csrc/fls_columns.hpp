@@ -91,16 +91,27 @@ struct LaneCol {
 };
 
 // Row mask (bit r) of the rows of this lane that exist in a vector holding
-// `live` < 1024 rows, in normal or transposed row order.
+// `live` <= 1024 rows, in normal or transposed row order.  Closed forms (no
+// per-row index table, which the compiler would hoist into 32 registers):
+//  * normal order: row r is original row l + 32r, so the mask is a prefix;
+//  * transposed order: register row rsum_row(j) holds original row c0 + j of
+//    the lane's chain (c0 = 32*(2*(l%16) + l/16)), so the live rows are the
+//    first J = clamp(live - c0, 0, 32) chain elements.  Chain element j sits
+//    in row 4*(j%8) + {0,2,1,3}[j/8]: four groups of eight rows spaced 4 apart.
 __device__ __forceinline__ uint32_t live_row_mask(uint32_t live, bool transposed) {
     const uint32_t l = dev::lane32();
-    uint32_t m = 0;
-#pragma unroll
-    for (int r = 0; r < 32; ++r) {
-        const uint32_t p = l + 32u * r;
-        m |= (uint32_t)((transposed ? dev::untranspose_index(p) : p) < live) << r;
+    if (!transposed) {
+        const uint32_t k = live > l ? min((live - l + 31u) >> 5, 32u) : 0u;
+        return k >= 32u ? 0xffffffffu : ((1u << k) - 1u);
     }
-    return m;
+    const uint32_t c0 = 32u * (2u * (l & 15u) + (l >> 4));
+    const uint32_t J = live > c0 ? min(live - c0, 32u) : 0u;
+    const uint32_t q = J >> 3, s = J & 7u;
+    uint32_t m = (q > 0 ? 0x11111111u : 0u) | (q > 1 ? 0x44444444u : 0u) | (q > 2 ? 0x22222222u : 0u) |
+                 (q > 3 ? 0x88888888u : 0u);
+    const uint32_t part = 0x11111111u & ((1u << (4u * s)) - 1u);  // s <= 7
+    const uint32_t sh = q < 4 ? (0x3120u >> (4u * q)) & 3u : 0u;
+    return m | (part << sh);
 }
 
 // Value k of a linear LSB-first bit-packing at width bw (bw <= 32).
@@ -264,6 +275,179 @@ __device__ __forceinline__ uint32_t delta_mask(const ColDev& c, uint64_t v, Pred
 #undef FLS_DM_CASE
         default: return delta_mask_walk(in, bw, base, pred);
     }
+}
+
+// ------------------------------------------------------------------ row walks
+// One lane of one vector visited row by row: f(r, value) is called once per
+// register row r (a compile-time constant after unrolling, so a caller's
+// per-row arrays stay in registers), with only the packed words of the
+// vector's width held -- no 32-row decoded array is needed by the walk
+// itself.  Row masks are the walk with f = "set bit r if pred(value)"; a
+// kernel that handles every width and encoding this way stays near 40-80
+// VGPRs.
+
+// fixed width (bit-pack, + FOR base); lane_in = first word of this lane
+template <int BW, typename F>
+__device__ __forceinline__ void walk_fixed_lane(const uint32_t* __restrict lane_in, uint32_t base, F& f) {
+    if constexpr (BW == 0) {
+#pragma unroll
+        for (int r = 0; r < 32; ++r) f(r, base);
+    } else {
+        uint32_t w[BW];
+        dev::load_lane<BW>(lane_in, w);
+#pragma unroll
+        for (int r = 0; r < 32; ++r) f(r, dev::extract_at<BW>(w, r) + base);
+    }
+}
+
+// FastLanes delta: the running sum visits the rows in chain order
+template <int BW, typename F>
+__device__ __forceinline__ void walk_delta_lane(const uint32_t* __restrict lane_in, uint32_t first, F& f) {
+    if constexpr (BW == 0) {
+#pragma unroll
+        for (int r = 0; r < 32; ++r) f(r, first);
+    } else {
+        uint32_t w[BW];
+        dev::load_lane<BW>(lane_in, w);
+        uint32_t acc = first;
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+            const int r = dev::rsum_row(j);
+            acc += dev::extract_at<BW>(w, r);
+            f(r, acc);
+        }
+    }
+}
+
+#define FLS_WIDTH_CASES(X)                                                                                           \
+    X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19)  \
+        X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31) X(32)
+
+template <typename F>
+__device__ __forceinline__ void walk_fixed(const uint32_t* __restrict lane_in, uint32_t bw, uint32_t base, F& f) {
+    switch (bw) {
+#define FLS_WF_CASE(B)                               \
+    case B: walk_fixed_lane<B>(lane_in, base, f); \
+        break;
+        FLS_WIDTH_CASES(FLS_WF_CASE)
+#undef FLS_WF_CASE
+        default: break;
+    }
+}
+
+template <typename F>
+__device__ __forceinline__ void walk_delta(const uint32_t* __restrict lane_in, uint32_t bw, uint32_t first, F& f) {
+    switch (bw) {
+#define FLS_WD_CASE(B)                                \
+    case B: walk_delta_lane<B>(lane_in, first, f); \
+        break;
+        FLS_WIDTH_CASES(FLS_WD_CASE)
+#undef FLS_WD_CASE
+        default: break;
+    }
+}
+
+// FastLanes RLE: the run index of the lane's rows is a delta chain of width
+// ibw (0: the whole lane is one run; 1: one word, bit r = 1 where row r starts
+// a run); `stage` holds the vector's run values, `last` = n_runs - 1.
+template <typename F>
+__device__ __forceinline__ void walk_rle(const uint32_t* __restrict idx_lane, uint32_t ibw, uint32_t first_idx,
+                                         const uint32_t* __restrict stage, uint32_t last, F& f) {
+    if (ibw == 0) {
+        const uint32_t x = stage[min(first_idx, last)];
+#pragma unroll
+        for (int r = 0; r < 32; ++r) f(r, x);
+        return;
+    }
+    const uint32_t starts = __builtin_nontemporal_load(idx_lane);
+    uint32_t acc = first_idx;
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
+        const int r = dev::rsum_row(j);
+        acc += (starts >> r) & 1u;
+        f(r, stage[min(acc, last)]);
+    }
+}
+
+// Stage the n_runs (<= 1024) run values of a vector into LDS; every
+// work-item of the wave calls it (`active` = this half-wave has a vector).
+__device__ __forceinline__ void stage_runs(const uint32_t* __restrict vals, uint32_t n_runs, uint32_t vbw,
+                                           uint32_t vbase, uint32_t* __restrict stage, bool active) {
+    if (active) {
+        const uint32_t nr = n_runs < 1024u ? n_runs : 1024u;
+        for (uint32_t k = dev::lane32(); k < nr; k += 32u) stage[k] = vbase + linear_extract(vals, k, vbw);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Walk vector v of any column encoding (run-time width and encoding).  Every
+// work-item of the wave calls it (RLE run values are staged in `stage`, this
+// vector's 1024-entry LDS slice); f is called only when `live`.  `rmeta`:
+// the RLE metadata (c.aux of an RLE column).
+template <typename F>
+__device__ __forceinline__ void column_walk(const ColDev& c, const fls_rle_meta* __restrict rmeta, uint64_t v,
+                                            uint32_t* __restrict stage, bool live, F& f) {
+    const uint32_t l = dev::lane32();
+    if (c.enc == FLS_ENC_BITPACK) {
+        if (live) walk_fixed(c.words + v * (32u * c.bw) + l, c.bw, c.base, f);
+        return;
+    }
+    if (c.enc == FLS_ENC_DELTA) {
+        if (!live) return;
+        const fls_vmeta m = c.vmeta[v];
+        walk_delta(c.words + FLS_VMETA_OFFSET(m) + l, FLS_VMETA_BW(m), c.bases[v * 32u + l], f);
+        return;
+    }
+    if (c.enc == FLS_ENC_RLE) {
+        fls_vmeta m = 0;
+        fls_rle_meta rm{};
+        if (live) {
+            m = c.vmeta[v];
+            rm = rmeta[v];
+        }
+        const uint32_t* p = c.words + FLS_VMETA_OFFSET(m);
+        const uint32_t ibw = FLS_VMETA_BW(m) ? 1u : 0u;
+        stage_runs(p + 32u * ibw, rm.n_runs, rm.vbw, rm.vbase, stage, live);
+        if (!live) return;
+        const uint32_t last = rm.n_runs ? min((uint32_t)rm.n_runs, 1024u) - 1u : 0u;
+        walk_rle(p + l, ibw, c.bases[v * 32u + l], stage, last, f);
+        return;
+    }
+    // MIXED
+    uint32_t enc = 0, w = 0;
+    uint64_t ax = 0;
+    const uint32_t* p = c.words;
+    if (live) {
+        const fls_vmeta m = c.vmeta[v];
+        enc = (uint32_t)(m >> 6) & 3u;
+        w = (uint32_t)m & 63u;
+        ax = c.aux[v];
+        p = c.words + FLS_VMETA_OFFSET(m);
+    }
+    const uint32_t ibw = (uint32_t)(ax >> 48) & 1u, nr = (uint32_t)(ax >> 32) & 0xffffu;
+    stage_runs(p + 32u + 32u * ibw, nr, w, (uint32_t)ax, stage, live && enc == FLS_MIX_RLE);
+    if (!live) return;
+    switch (enc) {
+        case FLS_MIX_BITPACK: walk_fixed(p + l, w, 0u, f); break;
+        case FLS_MIX_FOR: walk_fixed(p + l, w, (uint32_t)ax, f); break;
+        case FLS_MIX_DELTA: walk_delta(p + 32u + l, w, p[l], f); break;
+        default: {
+            const uint32_t last = nr ? min(nr, 1024u) - 1u : 0u;
+            walk_rle(p + 32u + l, ibw, p[l], stage, last, f);
+        }
+    }
+}
+
+// Row mask {r : pred(value of row r)} of vector v, any encoding.
+template <typename Pred>
+__device__ __forceinline__ uint32_t column_mask(const ColDev& c, const fls_rle_meta* __restrict rmeta, uint64_t v,
+                                                uint32_t* __restrict stage, bool live, Pred pred) {
+    uint32_t m = 0;
+    auto f = [&](int r, uint32_t x) { m |= (uint32_t)pred(x) << r; };
+    column_walk(c, rmeta, v, stage, live, f);
+    return m;
 }
 
 }  // namespace fls

@@ -31,8 +31,12 @@ struct Q11Dev {
     uint64_t* count;
 };
 
-// Generic path: every column through decode_col (any encoding, run-time
-// widths); RLE / MIXED columns stage their run values in LDS.
+// Generic path: any encoding per column (bit-pack + FOR, delta, RLE, mixed),
+// run-time widths.  Every column is walked row by row at its own width
+// (column_walk): orderdate and quantity reduce straight to row masks,
+// discount is kept as 32 values (it is both filtered and multiplied), and
+// price is multiplied in place during its walk -- one 32-value array live,
+// not one per column.  RLE / MIXED columns stage run values in LDS.
 template <bool PRED>
 __global__ __launch_bounds__(kBlock) void q11_kernel(const Q11Dev a) {
     __shared__ uint32_t stage[kVecPerBlock][1024];
@@ -42,34 +46,37 @@ __global__ __launch_bounds__(kBlock) void q11_kernel(const Q11Dev a) {
     const bool transposed = a.price.transposed != 0;
     const uint32_t dlo = a.date_gt + 1u, dspan = a.date_lt - a.date_gt - 1u;
     const bool ok = a.date_lt > a.date_gt && a.disc_hi >= a.disc_lo;
-    const uint32_t span = a.disc_hi - a.disc_lo;
+    const uint32_t span = a.disc_hi - a.disc_lo, qty_lt = a.qty_lt, disc_lo = a.disc_lo;
+    const auto* date_rm = reinterpret_cast<const fls_rle_meta*>(a.date.aux);
+    const auto* qty_rm = reinterpret_cast<const fls_rle_meta*>(a.qty.aux);
+    const auto* disc_rm = reinterpret_cast<const fls_rle_meta*>(a.disc.aux);
+    const auto* price_rm = reinterpret_cast<const fls_rle_meta*>(a.price.aux);
     for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
         const uint64_t v = g * kVecPerBlock + (threadIdx.x >> 5);
-        if (v >= a.n_vec) continue;
-        const uint64_t first = v * 1024u;
-        uint32_t m = first + 1024u <= a.n_rows ? 0xffffffffu : live_row_mask((uint32_t)(a.n_rows - first), transposed);
-        if (!ok) m = 0;
-        uint32_t x[32];
-        decode_col(a.date, v, st, true, x);
-#pragma unroll
-        for (int r = 0; r < 32; ++r) m &= ~((uint32_t)((x[r] - dlo) >= dspan) << r);
+        const bool live = v < a.n_vec;  // RLE / MIXED staging needs the whole wave, so no early exit
+        uint32_t m = 0;
+        if (live && ok) {
+            const uint64_t first = v * 1024u;
+            m = first + 1024u <= a.n_rows ? 0xffffffffu : live_row_mask((uint32_t)(a.n_rows - first), transposed);
+        }
+        m &= column_mask(a.date, date_rm, v, st, live, [dlo, dspan](uint32_t x) { return (x - dlo) < dspan; });
         if (PRED && !__any(m != 0)) continue;
-        decode_col(a.qty, v, st, true, x);
-#pragma unroll
-        for (int r = 0; r < 32; ++r) m &= ~((uint32_t)(x[r] >= a.qty_lt) << r);
+        m &= column_mask(a.qty, qty_rm, v, st, live, [qty_lt](uint32_t x) { return x < qty_lt; });
         if (PRED && !__any(m != 0)) continue;
         uint32_t d[32];
-        decode_col(a.disc, v, st, true, d);
+        auto keep_disc = [&](int r, uint32_t x) { d[r] = x; };
+        column_walk(a.disc, disc_rm, v, st, live, keep_disc);
+        if (live) {
 #pragma unroll
-        for (int r = 0; r < 32; ++r) m &= ~((uint32_t)((d[r] - a.disc_lo) > span) << r);
-        if (PRED && !__any(m != 0)) continue;
-        decode_col(a.price, v, st, true, x);
-#pragma unroll
-        for (int r = 0; r < 32; ++r) {
-            const uint32_t f = (m >> r) & 1u;
-            acc += (uint64_t)x[r] * (uint64_t)(d[r] * f);
-            cnt += f;
+            for (int r = 0; r < 32; ++r) {
+                m &= ~((uint32_t)((d[r] - disc_lo) > span) << r);
+                d[r] &= 0u - ((m >> r) & 1u);  // rows that fail contribute 0
+            }
         }
+        if (PRED && !__any(m != 0)) continue;
+        auto mac = [&](int r, uint32_t x) { acc += (uint64_t)x * (uint64_t)d[r]; };
+        column_walk(a.price, price_rm, v, st, live, mac);
+        cnt += (uint32_t)__builtin_popcount(m);
     }
     __shared__ uint64_t s_acc[kBlock / 64];
     __shared__ uint64_t s_cnt[kBlock / 64];
