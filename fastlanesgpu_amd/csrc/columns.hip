@@ -50,15 +50,20 @@ int blocks_per_cu(const void* kernel, int block, size_t lds, int* per_cu) {
     return FLS_OK;
 }
 
+uint32_t balanced_grid(uint64_t max_blocks, uint64_t work) {
+    if (max_blocks == 0) max_blocks = 1;
+    if (work <= max_blocks) return (uint32_t)(work ? work : 1);
+    const uint64_t rounds = ceil_div(work, max_blocks);
+    return (uint32_t)ceil_div(work, rounds);
+}
+
 int resident_grid_of(const void* kernel, int block, size_t lds, uint64_t work, uint32_t* grid) {
     int per_cu = 0;
     int st = blocks_per_cu(kernel, block, lds, &per_cu);
     if (st) return st;
     uint32_t cus = 0;
     if ((st = device_cus(&cus)) != FLS_OK) return st;
-    uint64_t g = (uint64_t)cus * (uint64_t)per_cu;
-    if (work < g) g = work;
-    *grid = (uint32_t)(g ? g : 1);
+    *grid = balanced_grid((uint64_t)cus * (uint64_t)per_cu, work);
     return FLS_OK;
 }
 
@@ -66,10 +71,7 @@ int persistent_grid(uint64_t work, uint32_t blocks_per_cu, uint32_t* grid) {
     uint32_t cus = 0;
     int st = device_cus(&cus);
     if (st) return st;
-    uint64_t g = (uint64_t)cus * blocks_per_cu;
-    if (work < g) g = work;
-    if (g == 0) g = 1;
-    *grid = (uint32_t)g;
+    *grid = balanced_grid((uint64_t)cus * blocks_per_cu, work);
     return FLS_OK;
 }
 

@@ -19,7 +19,14 @@ struct ColDev {
 // view.  Fixed-width columns are bounds-checked against n_words.
 int to_dev_column(const fls_column& c, uint64_t n_rows, ColDev* out, bool allow_var);
 
-// Grid size for a persistent kernel: min(work, CUs * blocks_per_cu).
+// Grid of a grid-stride kernel over `work` equal units with at most
+// max_blocks resident: the fewest blocks that keep the same number of rounds
+// (ceil(work / max_blocks)), so every block gets the same number of units and
+// the last round has no idle blocks (14,649 units on 1,536 slots: 1,465
+// blocks x 10 units instead of a 10th round with 54 % of the blocks busy).
+uint32_t balanced_grid(uint64_t max_blocks, uint64_t work);
+
+// Grid size for a persistent kernel: balanced_grid(CUs * blocks_per_cu, work).
 int persistent_grid(uint64_t work, uint32_t blocks_per_cu, uint32_t* grid);
 
 // Number of compute units of the current device (cached per device).

@@ -227,10 +227,8 @@ int launch_resident_q21(Kernel k, int block, size_t lds, const Q21Dev& a, hipStr
     uint32_t cus = 0;
     int st = device_cus(&cus);
     if (st) return st;
-    uint64_t grid = (uint64_t)cus * per_cu;
-    if (grid > a.n_groups) grid = a.n_groups;
-    if (!grid) grid = 1;
-    hipLaunchKernelGGL(k, dim3((uint32_t)grid), dim3(block), lds, s, a);
+    const uint32_t grid = balanced_grid((uint64_t)cus * per_cu, a.n_groups);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(block), lds, s, a);
     return check_launch();
 }
 

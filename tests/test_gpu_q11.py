@@ -86,3 +86,29 @@ def test_q11_full_size_120m():
     dev = [c.to_device() for c in enc]
     assert fls.query_q11(*dev, n) == want
     assert fls.query_q11(*dev, n, pred_load=True) == want
+
+
+@pytest.mark.parametrize("layout", ["rle_date", "all_delta", "rle_small_cols"])
+def test_q11_rle_and_delta_columns(layout):
+    """The generic path over GPU-encoded RLE and delta columns in every
+    position (all share the transposed row order)."""
+    n = 70 * 1024 + 333
+    cols = T.o_gen_c2(31, n)
+
+    def rle(x):
+        w, vm, rm, bs = fls.rle_encode_device(T.to_dev(x), n)
+        return fls.Column(fls.ENC_RLE, n, w, vmeta=vm, bases=bs, aux=rm)
+
+    def delta(x):
+        w, vm, bs = fls.delta_encode_device(T.to_dev(x), n)
+        return fls.Column(fls.ENC_DELTA, n, w, vmeta=vm, bases=bs)
+
+    def packed(x):
+        return fls.bitpack_column(x, transposed=True).to_device()
+
+    kinds = {"rle_date": (rle, packed, packed, delta), "all_delta": (delta, delta, delta, delta),
+             "rle_small_cols": (delta, rle, rle, packed)}[layout]
+    enc = [f(c) for f, c in zip(kinds, cols)]
+    want = T.o_q11(cols, n)
+    for pred in (False, True):
+        assert fls.query_q11(*enc, n, pred_load=pred) == want, (layout, pred)
