@@ -309,9 +309,9 @@ def test_adam_step():
 @pytest.fixture(scope="module")
 def trained():
     torch_cuda()
-    feat, label, _ = sel.build_training_set(21, 1 << 15)
+    feat, label, _ = sel.build_training_set(21, 1 << 17)
     mlp = sel.Mlp(21)
-    losses = mlp.train(feat, label, epochs=3, batch=1024, lr=3e-3, seed=21)
+    losses = mlp.train(feat, label, epochs=4, batch=512, lr=3e-3, seed=21)  # 1024 Adam steps
     return mlp, losses
 
 
