@@ -307,15 +307,26 @@ __global__ __launch_bounds__(256) void mlp_grad_kernel(const uint16_t* __restric
     if (L == 0) p[kParams] = wl;
 }
 
-// grad[i] += sum over waves of part[w][i] in wave order; *loss likewise
-__global__ void grad_reduce_kernel(const float* __restrict part, uint32_t n_waves, float* __restrict grad,
-                                   float* __restrict loss) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > kParams) return;
+// grad[i] += sum over waves of part[w][i]; *loss likewise (i = kParams).  A
+// block serves 16 parameters with 16 slices each: slice s adds waves
+// s, s+16, s+32, ... in order, then slice sums are added in slice order --
+// a fixed order, so the result is bit-reproducible.
+__global__ __launch_bounds__(256) void grad_reduce_kernel(const float* __restrict part, uint32_t n_waves,
+                                                          float* __restrict grad, float* __restrict loss) {
+    __shared__ float red[16][17];
+    const int col = threadIdx.x & 15, sl = threadIdx.x >> 4;
+    const int i = blockIdx.x * 16 + col;
     float acc = 0.f;
-    for (uint32_t w = 0; w < n_waves; ++w) acc += part[(uint64_t)w * kPart + i];
-    if (i < kParams) grad[i] += acc;
-    else if (loss) *loss += acc;
+    if (i <= kParams)
+        for (uint32_t w = sl; w < n_waves; w += 16) acc += part[(uint64_t)w * kPart + i];
+    red[sl][col] = acc;
+    __syncthreads();
+    if (sl == 0 && i <= kParams) {
+        float t = 0.f;
+        for (int k = 0; k < 16; ++k) t += red[k][col];
+        if (i < kParams) grad[i] += t;
+        else if (loss) *loss += t;
+    }
 }
 
 uint32_t grad_blocks(uint64_t n) {
@@ -394,7 +405,7 @@ int fls_mlp_grad(const uint16_t* d_wb, const float* d_feat, const uint8_t* d_lab
     hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks), dim3(256), 0, as_stream(stream), d_wb, d_feat, d_label, n,
                        inv_batch, part);
     if (const int rc = check_launch()) return rc;
-    hipLaunchKernelGGL(grad_reduce_kernel, dim3((uint32_t)ceil_div(kParams + 1, 256)), dim3(256), 0,
+    hipLaunchKernelGGL(grad_reduce_kernel, dim3((uint32_t)ceil_div(kParams + 1, 16)), dim3(256), 0,
                        as_stream(stream), part, blocks * 4u, d_grad, d_loss);
     return check_launch();
 }

@@ -13,9 +13,13 @@
 //    plan (chosen encoding's statistics -> size) -> scan -> pack.
 //
 // Work-item t of a 256-work-item block serves lane t%32 of vector
-// blockIdx.x*8 + t/32.  Lane l handles the 32 consecutive rows
-// c0 = 32*(2*(l%16) + l/16) .. c0+31 of its vector: FastLanes delta's chain,
-// whose j-th row is stored in register row rsum_row(j) (rsum.cuh:8-106).
+// blockIdx.x*8 + t/32.  Loads are coalesced: lane l reads rows 32j + l
+// (j = 0..31), i.e. element l of every 32-row chain j -- FastLanes delta's
+// chains are 32 consecutive rows, chain c owned in the packed format by the
+// lane l with 2*(l%16) + l/16 = c, its element j in register row rsum_row(j)
+// (rsum.cuh:8-106).  Chain deltas are therefore neighbour-lane differences
+// (one shuffle), and the pack step moves chains to their owning lanes
+// through a padded LDS tile.  Every exchange stays inside one wave.
 #include "fls/device/unpack.hpp"
 #include "fls/device/wave.hpp"
 #include "fls_columns.hpp"
@@ -58,72 +62,56 @@ __device__ __forceinline__ uint32_t padded(const uint32_t* __restrict in, uint64
     return in[k < n ? k : n - 1];
 }
 
-// The lane's chain: x[j] = row c0+j of vector v (j = 0..31), and the row just
-// before the chain (only meaningful when the chain is not the vector's first).
-__device__ __forceinline__ void load_chain(const uint32_t* __restrict in, uint64_t n, uint64_t v, uint32_t (&x)[32],
-                                           uint32_t& before, bool& first_chain) {
-    const uint32_t c = chain_slot(dev::lane32());
-    const uint64_t c0 = v * 1024u + 32u * c;
+// Coalesced load of vector v: x[j] = row 32j + l (rows past n repeat row n-1).
+__device__ __forceinline__ void load_rows(const uint32_t* __restrict in, uint64_t n, uint64_t v, uint32_t (&x)[32]) {
+    const uint64_t base = v * 1024u + dev::lane32();
+    if (v * 1024u + 1024u <= n) {
 #pragma unroll
-    for (int j = 0; j < 32; ++j) x[j] = padded(in, n, c0 + j);
-    first_chain = c == 0;
-    before = first_chain ? 0u : padded(in, n, c0 - 1);
+        for (int j = 0; j < 32; ++j) x[j] = __builtin_nontemporal_load(in + base + 32u * j);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 32; ++j) x[j] = padded(in, n, base + 32u * j);
+    }
 }
 
-// All four encodings' statistics of the vector (half-wave reductions).
-__device__ __forceinline__ VecStats analyze_all(const uint32_t (&x)[32], uint32_t before, bool first_chain) {
-    uint32_t orv = 0, mn = 0xffffffffu, mx = 0, dor = 0, starts = 0, inner = 0;
-    uint32_t prev = x[0];
+// Statistics of the vector from the coalesced layout (half-wave reductions).
+// ALL: every encoding's; otherwise only those of encoding `enc`.
+template <bool ALL>
+__device__ __forceinline__ VecStats analyze_rows(const uint32_t (&x)[32], uint32_t enc) {
+    const uint32_t l = dev::lane32();
+    const bool need_or = ALL || enc == MIX_BITPACK;
+    const bool need_range = ALL || enc == MIX_FOR || enc == MIX_RLE;
+    const bool need_delta = ALL || enc == MIX_DELTA;
+    const bool need_runs = ALL || enc == MIX_RLE;
+    uint32_t orv = 0, mn = 0xffffffffu, mx = 0, dor = 0, starts = 0, inner = 0, last = 0;
 #pragma unroll
     for (int j = 0; j < 32; ++j) {
         orv |= x[j];
         mn = min(mn, x[j]);
         mx = max(mx, x[j]);
-        if (j) {
-            dor |= x[j] - prev;
-            const uint32_t s = x[j] != prev;
-            starts += s;
-            inner |= s;
+        if (need_delta || need_runs) {
+            const uint32_t up = __shfl_up(x[j], 1, 32);  // element l-1 of chain j
+            if (l) {
+                dor |= x[j] - up;
+                const uint32_t st = x[j] != up;
+                starts += st;
+                inner |= st;
+            } else {
+                starts += j ? (uint32_t)(x[j] != last) : 1u;  // across the chain boundary
+            }
+            if (need_runs) last = __shfl(x[j], 31, 32);     // element 31 of chain j
         }
-        prev = x[j];
     }
-    starts += first_chain ? 1u : (uint32_t)(x[0] != before);
-    VecStats st;
-    st.w_bp = wid(h_or(orv));
-    st.vmin = h_min(mn);
-    st.w_for = wid(h_max(mx) - st.vmin);
-    st.w_delta = wid(h_or(dor));
-    st.n_runs = h_sum(starts);
-    st.ibw = h_or(inner);
-    return st;
-}
-
-// Statistics of one chosen encoding only (what the encoder of a learned
-// choice needs): widths / base / run count for that encoding, nothing else.
-__device__ __forceinline__ VecStats analyze_one(uint32_t enc, const uint32_t (&x)[32], uint32_t before,
-                                                bool first_chain) {
     VecStats st{};
-    if (enc == MIX_BITPACK) {
-        uint32_t orv = 0;
-#pragma unroll
-        for (int j = 0; j < 32; ++j) orv |= x[j];
-        st.w_bp = wid(h_or(orv));
-    } else if (enc == MIX_FOR) {
-        uint32_t mn = 0xffffffffu, mx = 0;
-#pragma unroll
-        for (int j = 0; j < 32; ++j) {
-            mn = min(mn, x[j]);
-            mx = max(mx, x[j]);
-        }
+    if (need_or) st.w_bp = wid(h_or(orv));
+    if (need_range) {
         st.vmin = h_min(mn);
         st.w_for = wid(h_max(mx) - st.vmin);
-    } else if (enc == MIX_DELTA) {
-        uint32_t dor = 0;
-#pragma unroll
-        for (int j = 1; j < 32; ++j) dor |= x[j] - x[j - 1];
-        st.w_delta = wid(h_or(dor));
-    } else {
-        st = analyze_all(x, before, first_chain);
+    }
+    if (need_delta) st.w_delta = wid(h_or(dor));
+    if (need_runs) {
+        st.n_runs = h_sum(starts);
+        st.ibw = h_or(inner);
     }
     return st;
 }
@@ -158,10 +146,9 @@ __global__ __launch_bounds__(kBlock) void exhaustive_kernel(const uint32_t* __re
                                                             uint32_t* __restrict sizes4) {
     const uint64_t v = (uint64_t)blockIdx.x * kVecPerBlock + (threadIdx.x >> 5);
     if (v >= n_vec) return;
-    uint32_t x[32], before;
-    bool first;
-    load_chain(in, n, v, x, before, first);
-    const VecStats st = analyze_all(x, before, first);
+    uint32_t x[32];
+    load_rows(in, n, v, x);
+    const VecStats st = analyze_rows<true>(x, 0);
     if (dev::lane32() == 0) {
         uint32_t best = 0, best_size = 0xffffffffu;
         for (uint32_t e = 0; e < 4; ++e) {
@@ -245,18 +232,17 @@ __global__ __launch_bounds__(kBlock) void mixed_plan_kernel(const uint32_t* __re
                                                             uint32_t* __restrict fallbacks) {
     const uint64_t v = (uint64_t)blockIdx.x * kVecPerBlock + (threadIdx.x >> 5);
     if (v >= n_vec) return;
-    uint32_t x[32], before;
-    bool first;
-    load_chain(in, n, v, x, before, first);
+    uint32_t x[32];
+    load_rows(in, n, v, x);
     const bool fallback = conf && conf[v] < conf_min;
     uint32_t enc;
     VecStats st;
     if (fallback) {
-        st = analyze_all(x, before, first);
+        st = analyze_rows<true>(x, 0);
         enc = best_encoding(st);
     } else {
         enc = mix_class_enc(cls[v]);
-        st = analyze_one(enc, x, before, first);
+        st = analyze_rows<false>(x, enc);
     }
     if (dev::lane32() == 0) {
         const uint32_t w = enc_width(enc, st);
@@ -270,12 +256,24 @@ __global__ __launch_bounds__(kBlock) void mixed_plan_kernel(const uint32_t* __re
     }
 }
 
-// pack every vector with its planned encoding; RLE needs the cross-chain
-// prefix of run starts and the run values in LDS, so the block passes two
-// unconditional barriers
+// pack every vector with its planned encoding.  The vector is loaded
+// coalesced, moved to chain-per-lane order through a padded LDS tile
+// (32 x 33 words: row j of chain c at 33c + j, conflict-free both ways), and
+// packed from registers.  RLE additionally needs the prefix of run starts
+// over the chains before the lane's and the run values, both through LDS.
+// A vector lives in one half-wave, so every exchange is wave-local: the
+// kernel has no block barrier.
+constexpr int kTile = 32 * 33;
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 struct PackLds {
+    uint32_t tile[kVecPerBlock][kTile];  // transpose tile, then the RLE run values
     uint32_t cnt[kVecPerBlock][32];
-    uint32_t vals[kVecPerBlock][1024];
 };
 
 __global__ __launch_bounds__(kBlock) void mixed_pack_kernel(const uint32_t* __restrict in, uint64_t n, uint64_t n_vec,
@@ -287,45 +285,56 @@ __global__ __launch_bounds__(kBlock) void mixed_pack_kernel(const uint32_t* __re
     const int slot = threadIdx.x >> 5;
     const uint32_t l = dev::lane32();
     const bool live = v < n_vec;
-    uint32_t x[32], before = 0, enc = 0, w = 0;
-    bool first = false;
-    uint64_t off = 0, ax = 0;
-    uint32_t starts = 0;
+    uint32_t* tile = lds.tile[slot];
+    const uint32_t c = chain_slot(l);  // the chain this lane packs
+    uint32_t x[32];
     if (live) {
-        load_chain(in, n, v, x, before, first);
+        uint32_t r[32];
+        load_rows(in, n, v, r);
+#pragma unroll
+        for (int j = 0; j < 32; ++j) tile[33 * j + l] = r[j];  // element l of chain j
+    }
+    wave_sync();
+    uint32_t enc = 0, w = 0, before = 0;
+    uint64_t off = 0, ax = 0;
+    if (live) {
+#pragma unroll
+        for (int j = 0; j < 32; ++j) x[j] = tile[33 * c + j];
+        before = c ? tile[33 * (c - 1) + 31] : 0u;
         const fls_vmeta m = vmeta[v];
         enc = (uint32_t)(m >> 6) & 3u;
         w = (uint32_t)m & 63u;
         off = FLS_VMETA_OFFSET(m);
         ax = aux[v];
-        if (enc == MIX_RLE) {
-            starts = first ? 1u : (uint32_t)(x[0] != before);
-#pragma unroll
-            for (int j = 1; j < 32; ++j) starts += x[j] != x[j - 1];
-            lds.cnt[slot][chain_slot(l)] = starts;
-        }
     }
-    __syncthreads();  // barrier 1 of 2
+    // RLE: run starts of this chain, then the index of its first run
     uint32_t rows[32];
     uint32_t first_idx = 0;
-    if (live && enc == MIX_RLE) {
-        const uint32_t c = chain_slot(l);
+    const bool rle = live && enc == MIX_RLE;
+    if (rle) {
+        uint32_t starts = c == 0 ? 1u : (uint32_t)(x[0] != before);
+#pragma unroll
+        for (int j = 1; j < 32; ++j) starts += x[j] != x[j - 1];
+        lds.cnt[slot][c] = starts;
+    }
+    wave_sync();  // cnt written; every lane has read its chain from the tile
+    if (rle) {
         uint32_t before_cnt = 0;
         for (uint32_t k = 0; k < c; ++k) before_cnt += lds.cnt[slot][k];
-        const uint32_t s0 = first ? 1u : (uint32_t)(x[0] != before);
+        const uint32_t s0 = c == 0 ? 1u : (uint32_t)(x[0] != before);
         first_idx = before_cnt + s0 - 1u;
         uint32_t idx = first_idx;
-        lds.vals[slot][idx] = x[0];
+        tile[idx] = x[0];  // run values reuse the tile (idx < 1024)
 #pragma unroll
         for (int j = 1; j < 32; ++j) {
-            const uint32_t s = x[j] != x[j - 1];
-            idx += s;
-            rows[dev::rsum_row(j)] = s;
-            if (s) lds.vals[slot][idx] = x[j];
+            const uint32_t st = x[j] != x[j - 1];
+            idx += st;
+            rows[dev::rsum_row(j)] = st;
+            if (st) tile[idx] = x[j];
         }
         rows[0] = 0u;
     }
-    __syncthreads();  // barrier 2 of 2
+    wave_sync();  // run values in place
     if (!live) return;
     uint32_t* o = out + off;
     if (enc == MIX_BITPACK || enc == MIX_FOR) {
@@ -352,7 +361,7 @@ __global__ __launch_bounds__(kBlock) void mixed_pack_kernel(const uint32_t* __re
             const uint32_t k0 = bit0 / w, k1 = min(nr - 1u, (bit0 + 31u) / w);
             for (uint32_t k = k0; k <= k1; ++k) {
                 const uint32_t kb = k * w;
-                const uint32_t val = lds.vals[slot][k] - vbase;
+                const uint32_t val = tile[k] - vbase;
                 if (kb >= bit0) {
                     if (kb - bit0 < 32u) word |= val << (kb - bit0);
                 } else {
@@ -363,7 +372,6 @@ __global__ __launch_bounds__(kBlock) void mixed_pack_kernel(const uint32_t* __re
         }
     }
 }
-
 
 __global__ __launch_bounds__(kBlock) void mixed_decode_kernel(const uint32_t* __restrict words,
                                                               const fls_vmeta* __restrict vmeta,
