@@ -79,15 +79,22 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(const SegDev* __restrict s
             const bool live = local < tile.nv;
             const uint64_t v = (uint64_t)tile.v0 + local;
             const uint32_t lo = sg.lo;
-            const uint32_t m = column_mask(col, sg.rmeta, v, stage[threadIdx.x >> 5], live,
-                                           [lo, span](uint32_t x) { return (x - lo) <= span; });
-            if (live) {
-                const uint64_t first = v * 1024u;
-                const uint32_t valid = first + 1024u <= sg.n_values
-                                           ? 0xffffffffu
-                                           : live_row_mask((uint32_t)(sg.n_values - first), col.transposed != 0);
-                cnt += (uint32_t)__builtin_popcount(m & valid);
+            const uint64_t first = v * 1024u;
+            uint32_t c = 0;
+            if (first + 1024u <= sg.n_values) {
+                // a full vector: count in-range rows directly (compare + add with carry)
+                auto count = [&](int, uint32_t x) { c += (uint32_t)((x - lo) <= span); };
+                column_walk(col, sg.rmeta, v, stage[threadIdx.x >> 5], live, count);
+            } else {
+                // the segment's partial last vector: row mask, then the rows that exist
+                uint32_t m = 0;
+                auto mark = [&](int r, uint32_t x) { m |= (uint32_t)((x - lo) <= span) << r; };
+                column_walk(col, sg.rmeta, v, stage[threadIdx.x >> 5], live, mark);
+                if (live)
+                    c = (uint32_t)__builtin_popcount(
+                        m & live_row_mask((uint32_t)(sg.n_values - first), col.transposed != 0));
             }
+            cnt += c;
         }
         cnt = dev::wave_sum(cnt);
         if (dev::is_wave_leader()) s_part[threadIdx.x >> 6] = cnt;

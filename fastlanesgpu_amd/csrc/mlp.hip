@@ -152,39 +152,135 @@ __device__ void forward_tile(const float* __restrict feat, uint64_t s0, uint64_t
 }
 
 // ------------------------------------------------------------- inference
+// Inference keeps its own LDS layout: every operand is stored so that an MFMA
+// fragment (4 consecutive k of one row m, or of one column n) is one aligned
+// 8-byte LDS read -- activations row-major, weights transposed (n-major) --
+// and rows are padded by 4 elements so the 16 rows a fragment load touches
+// fall in different banks.
+constexpr int kLdX = kIn + 4, kLdH = kH + 4;
+
+struct alignas(16) InferWave {
+    uint16_t x[16 * kLdX];
+    uint16_t h1[16 * kLdH];
+    uint16_t h2[16 * kLdH];
+    float z[16 * kOutPad];
+};
+struct alignas(16) InferLds {
+    uint16_t w1t[kH * kLdX];       // W1^T: [n][k], n < 64, k < 16
+    uint16_t w2t[kH * kLdH];       // W2^T: [n][k]
+    uint16_t w3t[kOutPad * kLdH];  // W3^T: [n][k], rows n >= 40 zero
+    float b1[kH], b2[kH], b3[kOutPad];
+    InferWave t[4];
+};
+
+__device__ void stage_weights_t(const uint16_t* __restrict wb, InferLds& s) {
+    for (int i = threadIdx.x; i < kIn * kH; i += blockDim.x) {
+        const int k = i / kH, n = i % kH;  // wb[oW1 + k*64 + n] = W1[k][n]
+        s.w1t[n * kLdX + k] = wb[oW1 + i];
+    }
+    for (int i = threadIdx.x; i < kH * kH; i += blockDim.x) {
+        const int k = i / kH, n = i % kH;
+        s.w2t[n * kLdH + k] = wb[oW2 + i];
+    }
+    for (int i = threadIdx.x; i < kH * kOutPad; i += blockDim.x) {
+        const int n = i / kH, k = i % kH;
+        s.w3t[n * kLdH + k] = n < kOut ? wb[oW3 + k * kOut + n] : (uint16_t)0;
+    }
+    for (int i = threadIdx.x; i < kH; i += blockDim.x) {
+        s.b1[i] = from_bf16(wb[oB1 + i]);
+        s.b2[i] = from_bf16(wb[oB2 + i]);
+    }
+    for (int i = threadIdx.x; i < kOutPad; i += blockDim.x) s.b3[i] = i < kOut ? from_bf16(wb[oB3 + i]) : 0.f;
+}
+
+// A fragment (rows m0.., k0..k0+15) of a row-major tile with leading dim ld,
+// and a B fragment (k0.., columns n0..) of a matrix stored transposed.
+__device__ __forceinline__ bf16x4 frag_row(const uint16_t* __restrict a, int ld, int m0, int k0) {
+    const int L = lane64();
+    return *reinterpret_cast<const bf16x4*>(a + (m0 + (L & 15)) * ld + k0 + 4 * (L >> 4));
+}
+
+__device__ __forceinline__ void infer_layer(const uint16_t* __restrict a, int lda, const uint16_t* __restrict wt,
+                                            int ldw, int K, int N, const float* __restrict bias,
+                                            uint16_t* __restrict h, int ldh) {
+    for (int n0 = 0; n0 < N; n0 += 16) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int k0 = 0; k0 < K; k0 += 16) acc = mfma(frag_row(a, lda, 0, k0), frag_row(wt, ldw, n0, k0), acc);
+        for (int i = 0; i < 4; ++i) {
+            const float v = acc[i] + bias[n0 + d_col()];
+            h[d_row(i) * ldh + n0 + d_col()] = to_bf16(v > 0.f ? v : 0.f);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void mlp_forward_kernel(const uint16_t* __restrict wb, const float* __restrict feat,
                                                           uint64_t n, uint8_t* __restrict cls, float* __restrict conf,
                                                           float* __restrict logits) {
-    __shared__ MlpLds s;
-    stage_weights(wb, s);
+    __shared__ InferLds s;
+    stage_weights_t(wb, s);
     __syncthreads();
-    WaveTiles& t = s.t[threadIdx.x >> 6];
+    InferWave& t = s.t[threadIdx.x >> 6];
+    const int L = lane64();
     const uint64_t n_tiles = ceil_div(n, 16);
     const uint64_t waves = (uint64_t)gridDim.x * 4;
     for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < n_tiles; tile += waves) {
         const uint64_t s0 = tile * 16;
-        forward_tile(feat, s0, n, s, t);
-        const int L = lane64();
-        if (L < 16 && s0 + L < n) {
-            // the encoding with the largest probability mass (classes 10e..10e+9),
-            // its most likely width bucket, and that mass as the confidence
-            const float* z = t.z + L * kOutPad;
-            float mx = z[0];
-            for (int c = 1; c < kOut; ++c) mx = fmaxf(mx, z[c]);
-            float mass[4] = {0.f, 0.f, 0.f, 0.f};
-            for (int c = 0; c < kOut; ++c) mass[c / 10] += expf(z[c] - mx);
-            int be = 0;
-            for (int e = 1; e < 4; ++e)
-                if (mass[e] > mass[be]) be = e;
-            int best = 10 * be;
-            for (int c = 10 * be + 1; c < 10 * be + 10; ++c)
-                if (z[c] > z[best]) best = c;
-            const uint64_t row = s0 + L;
-            cls[row] = (uint8_t)best;
-            if (conf) conf[row] = mass[be] / (mass[0] + mass[1] + mass[2] + mass[3]);
-            if (logits)
-                for (int c = 0; c < kOut; ++c) logits[row * kOut + c] = z[c];
+        for (int i = L; i < 16 * kIn; i += 64) {
+            const uint64_t row = s0 + i / kIn;
+            t.x[(i / kIn) * kLdX + (i % kIn)] = to_bf16(row < n ? feat[row * kIn + (i % kIn)] : 0.f);
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        infer_layer(t.x, kLdX, s.w1t, kLdX, kIn, kH, s.b1, t.h1, kLdH);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        infer_layer(t.h1, kLdH, s.w2t, kLdH, kH, kH, s.b2, t.h2, kLdH);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int n0 = 0; n0 < kOutPad; n0 += 16) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            for (int k0 = 0; k0 < kH; k0 += 16) acc = mfma(frag_row(t.h2, kLdH, 0, k0), frag_row(s.w3t, kLdH, n0, k0), acc);
+            for (int i = 0; i < 4; ++i) t.z[d_row(i) * kOutPad + n0 + d_col()] = acc[i] + s.b3[n0 + d_col()];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // softmax over all 64 lanes: lane L takes row r = L % 16 and the ten
+        // classes of encoding g = L / 16; the encoding with the largest mass
+        // wins, its most likely width bucket is the class, its mass the
+        // confidence
+        const int r = L & 15, g = L >> 4;
+        const float* z = t.z + r * kOutPad + 10 * g;
+        float gmax = z[0];
+        int bi = 0;
+        for (int c = 1; c < 10; ++c)
+            if (z[c] > gmax) {
+                gmax = z[c];
+                bi = c;
+            }
+        float rmax = fmaxf(gmax, __shfl_xor(gmax, 16, 64));
+        rmax = fmaxf(rmax, __shfl_xor(rmax, 32, 64));
+        float mass = 0.f;
+        for (int c = 0; c < 10; ++c) mass += expf(z[c] - rmax);
+        float m[4];
+        int b[4];
+        for (int k = 0; k < 4; ++k) {
+            m[k] = __shfl(mass, r + 16 * k, 64);
+            b[k] = __shfl(bi, r + 16 * k, 64);
+        }
+        int be = 0;
+        for (int k = 1; k < 4; ++k)
+            if (m[k] > m[be]) be = k;
+        const uint64_t row = s0 + r;
+        if (g == 0 && row < n) {
+            cls[row] = (uint8_t)(10 * be + b[be]);
+            if (conf) conf[row] = m[be] / (m[0] + m[1] + m[2] + m[3]);
+        }
+        if (logits && row < n)
+            for (int c = 0; c < 10; ++c) logits[row * kOut + 10 * g + c] = z[c];
     }
 }
 

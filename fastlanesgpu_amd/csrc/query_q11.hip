@@ -117,11 +117,14 @@ __device__ __forceinline__ uint32_t valid_mask(uint64_t v, const Q11Dev& a, bool
 // Fast path: compile-time widths for quantity/discount/price, FastLanes delta
 // orderdate.  Live state per work-item: the packed words of the three
 // fixed-width columns (QBW+DBW+PBW VGPRs), one 32-bit date mask, accumulators.
-// FULL: every column's words are loaded for every vector; waves whose date
-// mask is empty skip only the extraction/aggregation ALU work.
+// FULL: every column's words are loaded for every vector -- issued before the
+// orderdate decode, so their latency overlaps the date's dependent
+// vmeta -> words chain; waves whose date mask is empty skip only the
+// extraction/aggregation ALU work.  Holding them across the date decode
+// needs about 96 VGPRs, hence 5 waves per SIMD (at 6 the words spill).
 // PRED: later columns are not even loaded for such waves.
 template <int QBW, int DBW, int PBW, bool PRED>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void q11_fast_kernel(const Q11Dev a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void q11_fast_kernel(const Q11Dev a) {
     uint64_t acc = 0;
     uint32_t cnt = 0;
     const uint32_t l = dev::lane32();
@@ -133,17 +136,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
         const uint64_t v = g * kVecPerBlock + (threadIdx.x >> 5);
         if (v >= a.n_vec) continue;
-        // orderdate first: its per-vector width makes this a (half-wave
-        // divergent) switch whose widest case holds 32 words, so nothing else
-        // is kept live across it.
-        uint32_t m = delta_mask(a.date, v, [=](uint32_t x) { return (x - dlo) < dspan; });
-        if (v == a.n_vec - 1) m &= tail;
-        if (!date_ok || !disc_ok) m = 0;
         uint32_t qw[QBW > 0 ? QBW : 1], dw[DBW > 0 ? DBW : 1], pw[PBW > 0 ? PBW : 1];
         if constexpr (!PRED) {
             dev::load_lane<QBW>(a.qty.words + v * (32u * QBW) + l, qw);
             dev::load_lane<DBW>(a.disc.words + v * (32u * DBW) + l, dw);
             dev::load_lane<PBW>(a.price.words + v * (32u * PBW) + l, pw);
+        }
+        // orderdate: its per-vector width makes this a (half-wave divergent)
+        // switch; widths above 16 take the run-time walk, so at most 16 of
+        // its words are live next to the other columns' words
+        uint32_t m = delta_mask(a.date, v, [=](uint32_t x) { return (x - dlo) < dspan; });
+        if (v == a.n_vec - 1) m &= tail;
+        if (!date_ok || !disc_ok) m = 0;
+        if constexpr (!PRED) {
             pin_words(qw);
             pin_words(dw);
             pin_words(pw);

@@ -6,8 +6,9 @@
 //    (encoding x width bucket).  The size is computed from the statistics the
 //    encoder itself would derive (widths, run count), so this IS the size the
 //    pack step produces.
-//  * features: 16 cheap statistics of a 1/8 sample of the vector (eight
-//    64-byte segments) -- the learned selector's input.
+//  * features: 16 cheap statistics of a 1/8 sample of the vector (four
+//    128-byte lines, the first and the last among them) -- the learned
+//    selector's input.
 //  * mixed column: each vector stored with its own encoding, all in FastLanes
 //    transposed row order so mixed columns of one table stay row-aligned.
 //    plan (chosen encoding's statistics -> size) -> scan -> pack.
@@ -35,25 +36,37 @@ constexpr int kBlock = 256;
 constexpr int kVecPerBlock = kBlock / 32;
 
 __device__ __forceinline__ uint32_t wid(uint32_t x) { return x ? 32u - __clz(x) : 0u; }
+// Half-wave (32-lane) reductions.  Within a 16-lane row DPP moves data at
+// VALU speed (quad_perm swaps at distance 1 and 2, then row_ror:4 and
+// row_ror:8 -- after the quad steps each lane holds its quad, and each
+// rotation doubles the span, so every row lane ends with the row total, for
+// sums too); one lane exchange then joins the two rows of the half-wave.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, false);
+}
+constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kRowRor4 = 0x124, kRowRor8 = 0x128;
+constexpr int kWaveShr1 = 0x138;  // lane i <- lane i-1
+
+template <typename Op>
+__device__ __forceinline__ uint32_t half_reduce(uint32_t x, Op op) {
+    x = op(x, dpp<kQuadXor1>(x));
+    x = op(x, dpp<kQuadXor2>(x));
+    x = op(x, dpp<kRowRor4>(x));
+    x = op(x, dpp<kRowRor8>(x));
+    return op(x, (uint32_t)__shfl_xor(x, 16, 64));
+}
 __device__ __forceinline__ uint32_t h_or(uint32_t x) {
-#pragma unroll
-    for (int o = 16; o > 0; o >>= 1) x |= __shfl_xor(x, o, 64);
-    return x;
+    return half_reduce(x, [](uint32_t a, uint32_t b) { return a | b; });
 }
 __device__ __forceinline__ uint32_t h_min(uint32_t x) {
-#pragma unroll
-    for (int o = 16; o > 0; o >>= 1) x = min(x, (uint32_t)__shfl_xor(x, o, 64));
-    return x;
+    return half_reduce(x, [](uint32_t a, uint32_t b) { return min(a, b); });
 }
 __device__ __forceinline__ uint32_t h_max(uint32_t x) {
-#pragma unroll
-    for (int o = 16; o > 0; o >>= 1) x = max(x, (uint32_t)__shfl_xor(x, o, 64));
-    return x;
+    return half_reduce(x, [](uint32_t a, uint32_t b) { return max(a, b); });
 }
 __device__ __forceinline__ uint32_t h_sum(uint32_t x) {
-#pragma unroll
-    for (int o = 16; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    return x;
+    return half_reduce(x, [](uint32_t a, uint32_t b) { return a + b; });
 }
 
 __device__ __forceinline__ uint32_t chain_slot(uint32_t l) { return 2u * (l & 15u) + (l >> 4); }
@@ -90,7 +103,7 @@ __device__ __forceinline__ VecStats analyze_rows(const uint32_t (&x)[32], uint32
         mn = min(mn, x[j]);
         mx = max(mx, x[j]);
         if (need_delta || need_runs) {
-            const uint32_t up = __shfl_up(x[j], 1, 32);  // element l-1 of chain j
+            const uint32_t up = dpp<kWaveShr1>(x[j]);  // element l-1 of chain j (l >= 1)
             if (l) {
                 dor |= x[j] - up;
                 const uint32_t st = x[j] != up;
@@ -99,7 +112,11 @@ __device__ __forceinline__ VecStats analyze_rows(const uint32_t (&x)[32], uint32
             } else {
                 starts += j ? (uint32_t)(x[j] != last) : 1u;  // across the chain boundary
             }
-            if (need_runs) last = __shfl(x[j], 31, 32);     // element 31 of chain j
+            if (need_runs) {  // element 31 of chain j, of this half-wave's vector
+                const uint32_t e0 = (uint32_t)__builtin_amdgcn_readlane((int)x[j], 31);
+                const uint32_t e1 = (uint32_t)__builtin_amdgcn_readlane((int)x[j], 63);
+                last = (threadIdx.x & 32u) ? e1 : e0;
+            }
         }
     }
     VecStats st{};
@@ -163,15 +180,17 @@ __global__ __launch_bounds__(kBlock) void exhaustive_kernel(const uint32_t* __re
     }
 }
 
-// 16 features of a 1/8 sample: eight 16-row (64-byte) segments, rows
-// 128k..128k+15, so the sample reads 1/8 of the vector's memory lines; lane l
-// takes rows 128*(l/4) + 4*(l%4) + 0..3
+// 16 features of a 1/8 sample: four 32-row (128-byte, one memory line)
+// segments, lines 0, 10, 20 and 31 of the vector (line k*31/3), so the sample
+// reads 1/8 of the vector's lines and includes its first and last rows (a
+// sorted vector's sample spans its whole range); lane l takes rows
+// 32*((l/8)*31/3) + 4*(l%8) + 0..3
 __global__ __launch_bounds__(kBlock) void features_kernel(const uint32_t* __restrict in, uint64_t n, uint64_t n_vec,
                                                           float* __restrict feat) {
     const uint64_t v = (uint64_t)blockIdx.x * kVecPerBlock + (threadIdx.x >> 5);
     if (v >= n_vec) return;
     const uint32_t l = dev::lane32();
-    const uint64_t c0 = v * 1024u + 128u * (l >> 2) + 4u * (l & 3u);
+    const uint64_t c0 = v * 1024u + 32u * (((l >> 3) * 31u) / 3u) + 4u * (l & 7u);
     uint32_t s[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) s[j] = padded(in, n, c0 + j);

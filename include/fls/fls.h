@@ -293,8 +293,8 @@ uint64_t fls_column_bytes(const fls_column* col, uint64_t n_rows);
  * nearest helper is find_bw, ssb_utils.h:222-239).
  * fls_exhaustive_select: for every vector the exact encoded size of all four
  * encodings (d_sizes4[4*v+e], may be NULL) and the smallest class d_class[v].
- * fls_vector_features: 16 float features per vector from a 1/8 sample (rows
- * 128k..128k+15, k < 8: one 64-byte segment in every 512 bytes).           */
+ * fls_vector_features: 16 float features per vector from a 1/8 sample: the
+ * 32-row (128-byte) lines 0, 10, 20 and 31 of the vector.                  */
 int fls_exhaustive_select(const uint32_t* d_in, uint64_t n, uint8_t* d_class, uint32_t* d_sizes4, void* stream);
 int fls_vector_features(const uint32_t* d_in, uint64_t n, float* d_feat, void* stream);
 /* Mixed column from per-vector classes (only each vector's chosen encoding is

@@ -93,13 +93,15 @@ unsigned orc_exhaustive_class(const uint32_t* in) {
 }
 
 /* ---------------------------------------------------------------- features
- * Sample: 32 groups of 4 consecutive rows, group g = rows 128*(g/4) + 4*(g%4)
- * + 0..3, i.e. rows 128k..128k+15 for k = 0..7.                              */
+ * Sample: 32 groups of 4 consecutive rows, group g = rows 32*L(g/8) + 4*(g%8)
+ * + 0..3 with L(k) = k*31/3 = 0, 10, 20, 31: four whole 32-row lines, the
+ * vector's first and last among them.                                       */
+static unsigned sample_row(unsigned g, unsigned j) { return 32 * ((g / 8) * 31 / 3) + 4 * (g % 8) + j; }
 void orc_vector_features(const uint32_t* in, float* f) {
     uint32_t orv = 0, mn = 0xffffffffu, mx = 0, dor = 0, zero = 0, nondec = 0, absmax = 0, dsum = 0, big = 0;
     uint32_t eq = 0, fmin = 0xffffffffu, fmax = 0, wsum = 0, at_min = 0;
     for (unsigned blk = 0; blk < 32; ++blk) {
-        const uint32_t* s = in + 128 * (blk / 4) + 4 * (blk % 4);
+        const uint32_t* s = in + sample_row(blk, 0);
         for (unsigned j = 0; j < 4; ++j) {
             orv |= s[j];
             if (s[j] < mn) mn = s[j];
@@ -121,7 +123,7 @@ void orc_vector_features(const uint32_t* in, float* f) {
         if (s[0] > fmax) fmax = s[0];
     }
     for (unsigned blk = 0; blk < 32; ++blk)
-        for (unsigned j = 0; j < 4; ++j) at_min += in[128 * (blk / 4) + 4 * (blk % 4) + j] == mn;
+        for (unsigned j = 0; j < 4; ++j) at_min += in[sample_row(blk, j)] == mn;
     f[0] = orc_bit_width(orv) / 32.f;
     f[1] = orc_bit_width(mx - mn) / 32.f;
     f[2] = orc_bit_width(dor) / 32.f;
