@@ -267,11 +267,11 @@ __device__ __forceinline__ uint32_t delta_mask_walk(const uint32_t* __restrict i
 // up to 16 take a fully unrolled per-width walk (all words in registers, one
 // extract per row); wider vectors take the run-time walk, which bounds the
 // VGPRs a kernel must reserve for this column at about 16 + a few.
+// m = the vector's vmeta, base = this lane's delta base (already loaded, so a
+// kernel can fetch them one iteration ahead).
 template <typename Pred>
-__device__ __forceinline__ uint32_t delta_mask(const ColDev& c, uint64_t v, Pred pred) {
-    const fls_vmeta m = c.vmeta[v];
+__device__ __forceinline__ uint32_t delta_mask_pre(const ColDev& c, fls_vmeta m, uint32_t base, Pred pred) {
     const uint32_t* in = c.words + FLS_VMETA_OFFSET(m) + dev::lane32();
-    const uint32_t base = c.bases[v * 32u + dev::lane32()];
     const uint32_t bw = FLS_VMETA_BW(m);
     switch (bw) {
 #define FLS_DM_CASE(B) \
@@ -282,6 +282,11 @@ __device__ __forceinline__ uint32_t delta_mask(const ColDev& c, uint64_t v, Pred
 #undef FLS_DM_CASE
         default: return delta_mask_walk(in, bw, base, pred);
     }
+}
+
+template <typename Pred>
+__device__ __forceinline__ uint32_t delta_mask(const ColDev& c, uint64_t v, Pred pred) {
+    return delta_mask_pre(c, c.vmeta[v], c.bases[v * 32u + dev::lane32()], pred);
 }
 
 // ------------------------------------------------------------------ row walks

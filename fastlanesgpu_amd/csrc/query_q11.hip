@@ -119,12 +119,13 @@ __device__ __forceinline__ uint32_t valid_mask(uint64_t v, const Q11Dev& a, bool
 // fixed-width columns (QBW+DBW+PBW VGPRs), one 32-bit date mask, accumulators.
 // FULL: every column's words are loaded for every vector -- issued before the
 // orderdate decode, so their latency overlaps the date's dependent
-// vmeta -> words chain; waves whose date mask is empty skip only the
-// extraction/aggregation ALU work.  Holding them across the date decode
-// needs about 96 VGPRs, hence 5 waves per SIMD (at 6 the words spill).
+// vmeta -> words chain, and the next vector's date vmeta and base are
+// fetched one iteration ahead; waves whose date mask is empty skip only the
+// extraction/aggregation ALU work.  This holds about 127 VGPRs: 4 waves per
+// SIMD, measured faster than 5 (96 VGPRs) and than 6 (where the words spill).
 // PRED: later columns are not even loaded for such waves.
 template <int QBW, int DBW, int PBW, bool PRED>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void q11_fast_kernel(const Q11Dev a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void q11_fast_kernel(const Q11Dev a) {
     uint64_t acc = 0;
     uint32_t cnt = 0;
     const uint32_t l = dev::lane32();
@@ -133,8 +134,24 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     const uint32_t span = a.disc_hi - a.disc_lo;
     // rows of the (possibly partial) last vector, computed once: one VGPR
     const uint32_t tail = valid_mask(a.n_vec - 1, a, true);
+    fls_vmeta next_m = 0;  // the date column's vmeta and base of this work-item's next vector
+    uint32_t next_b = 0;
+    {
+        const uint64_t v0 = (uint64_t)blockIdx.x * kVecPerBlock + (threadIdx.x >> 5);
+        if (v0 < a.n_vec) {
+            next_m = a.date.vmeta[v0];
+            next_b = a.date.bases[v0 * 32u + l];
+        }
+    }
     for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
         const uint64_t v = g * kVecPerBlock + (threadIdx.x >> 5);
+        const fls_vmeta cur_m = next_m;
+        const uint32_t cur_b = next_b;
+        const uint64_t vn = v + (uint64_t)gridDim.x * kVecPerBlock;
+        if (vn < a.n_vec) {
+            next_m = a.date.vmeta[vn];
+            next_b = a.date.bases[vn * 32u + l];
+        }
         if (v >= a.n_vec) continue;
         uint32_t qw[QBW > 0 ? QBW : 1], dw[DBW > 0 ? DBW : 1], pw[PBW > 0 ? PBW : 1];
         if constexpr (!PRED) {
@@ -145,7 +162,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
         // orderdate: its per-vector width makes this a (half-wave divergent)
         // switch; widths above 16 take the run-time walk, so at most 16 of
         // its words are live next to the other columns' words
-        uint32_t m = delta_mask(a.date, v, [=](uint32_t x) { return (x - dlo) < dspan; });
+        uint32_t m = delta_mask_pre(a.date, cur_m, cur_b, [=](uint32_t x) { return (x - dlo) < dspan; });
         if (v == a.n_vec - 1) m &= tail;
         if (!date_ok || !disc_ok) m = 0;
         if constexpr (!PRED) {
