@@ -185,15 +185,8 @@ __global__ __launch_bounds__(kBlock) void exhaustive_kernel(const uint32_t* __re
 // reads 1/8 of the vector's lines and includes its first and last rows (a
 // sorted vector's sample spans its whole range); lane l takes rows
 // 32*((l/8)*31/3) + 4*(l%8) + 0..3
-__global__ __launch_bounds__(kBlock) void features_kernel(const uint32_t* __restrict in, uint64_t n, uint64_t n_vec,
-                                                          float* __restrict feat) {
-    const uint64_t v = (uint64_t)blockIdx.x * kVecPerBlock + (threadIdx.x >> 5);
-    if (v >= n_vec) return;
-    const uint32_t l = dev::lane32();
-    const uint64_t c0 = v * 1024u + 32u * (((l >> 3) * 31u) / 3u) + 4u * (l & 7u);
-    uint32_t s[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) s[j] = padded(in, n, c0 + j);
+// the 16 features of one vector from its sample (4 rows per lane)
+__device__ __forceinline__ void vector_features(const uint32_t (&s)[4], float* __restrict f) {
     uint32_t orv = 0, mn = 0xffffffffu, mx = 0, dor = 0, zero = 0, nondec = 0, absmax = 0, dsum = 0, big = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -218,8 +211,7 @@ __global__ __launch_bounds__(kBlock) void features_kernel(const uint32_t* __rest
     const uint32_t first_min = h_min(s[0]), first_max = h_max(s[0]);
     const uint32_t at_min = h_sum((uint32_t)(s[0] == g_min) + (s[1] == g_min) + (s[2] == g_min) + (s[3] == g_min));
     const uint32_t wsum = h_sum(wid(s[0]) + wid(s[1]) + wid(s[2]) + wid(s[3]));
-    if (l == 0) {
-        float* f = feat + 16 * v;
+    if (dev::lane32() == 0) {
         f[0] = wid(g_or) / 32.f;
         f[1] = wid(g_max - g_min) / 32.f;
         f[2] = wid(g_dor) / 32.f;
@@ -237,6 +229,29 @@ __global__ __launch_bounds__(kBlock) void features_kernel(const uint32_t* __rest
         f[14] = (float)(wid(g_max - g_min) > wid(g_dor));
         f[15] = 1.0f;
     }
+}
+
+// A half-wave takes kFeatVec consecutive vectors and issues all their sample
+// loads (16 independent 16-byte reads per lane) before reducing any of them:
+// one vector's 512 sampled bytes alone are too few to cover HBM latency.
+constexpr int kFeatVec = 4;
+
+__global__ __launch_bounds__(kBlock) void features_kernel(const uint32_t* __restrict in, uint64_t n, uint64_t n_vec,
+                                                          float* __restrict feat) {
+    const uint64_t v0 = ((uint64_t)blockIdx.x * kVecPerBlock + (threadIdx.x >> 5)) * kFeatVec;
+    if (v0 >= n_vec) return;  // uniform over the half-wave, which is all the reductions use
+    const uint32_t l = dev::lane32();
+    const uint32_t off = 32u * (((l >> 3) * 31u) / 3u) + 4u * (l & 7u);
+    uint32_t s[kFeatVec][4];
+#pragma unroll
+    for (int k = 0; k < kFeatVec; ++k) {
+        const uint64_t v = v0 + k < n_vec ? v0 + k : n_vec - 1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) s[k][j] = padded(in, n, v * 1024u + off + j);
+    }
+#pragma unroll
+    for (int k = 0; k < kFeatVec; ++k)
+        if (v0 + k < n_vec) vector_features(s[k], feat + 16 * (v0 + k));
 }
 
 // plan the chosen encoding of every vector: words and metadata.  With a
@@ -551,8 +566,8 @@ int fls_vector_features(const uint32_t* d_in, uint64_t n, float* d_feat, void* s
     if (n == 0) return FLS_OK;
     FLS_RETURN_IF(!d_in || !d_feat, FLS_ERR_NULL);
     const uint64_t n_vec = ceil_div(n, 1024);
-    hipLaunchKernelGGL(features_kernel, dim3((uint32_t)ceil_div(n_vec, kVecPerBlock)), dim3(kBlock), 0,
-                       as_stream(stream), d_in, n, n_vec, d_feat);
+    hipLaunchKernelGGL(features_kernel, dim3((uint32_t)ceil_div(n_vec, (uint64_t)kVecPerBlock * kFeatVec)),
+                       dim3(kBlock), 0, as_stream(stream), d_in, n, n_vec, d_feat);
     return check_launch();
 }
 
