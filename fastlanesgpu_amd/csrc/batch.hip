@@ -34,7 +34,7 @@ namespace fls {
 namespace {
 
 struct SegDev {
-    uint32_t enc, bw;
+    uint32_t enc, bw, transposed, pad;  // transposed: rows in FastLanes transposed order
     uint64_t n_values;
     const uint32_t* words;
     const fls_vmeta* vmeta;
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(const SegDev* __restrict s
         if (t >= n_tiles) break;  // uniform over the block
         const TileDev tile = tiles[t];
         const SegDev sg = segs[tile.seg];
-        const ColDev col{sg.enc, sg.bw, 0u, sg.enc != FLS_ENC_BITPACK, sg.words, sg.vmeta, sg.bases, sg.aux};
+        const ColDev col{sg.enc, sg.bw, 0u, sg.transposed, sg.words, sg.vmeta, sg.bases, sg.aux};
         const uint32_t span = sg.hi - sg.lo;
         uint64_t cnt = 0;
         for (uint32_t vb = 0; vb < tile.nv; vb += VPB) {
@@ -393,8 +393,9 @@ int fls_batch_create(const fls_segment* segs, uint32_t n_segments, fls_batch** o
             delete b;
             return st;
         }
+        const uint32_t tr = c.encoding == FLS_ENC_BITPACK ? (c.transposed ? 1u : 0u) : 1u;
         b->segs.push_back(
-            SegDev{c.encoding, c.bw, g.n_values, c.words, c.vmeta, c.bases, rmeta, c.aux, g.lo, g.hi});
+            SegDev{c.encoding, c.bw, tr, 0u, g.n_values, c.words, c.vmeta, c.bases, rmeta, c.aux, g.lo, g.hi});
         b->info.push_back(fls_segment_info{c.encoding, c.bw, g.n_values, g.lo, g.hi, c.n_words});
         if (c.encoding <= FLS_ENC_RLE) b->count_by_enc[c.encoding]++;
         b->encoded_bytes += 4ull * c.n_words;
@@ -508,7 +509,8 @@ int fls_batch_create_c4(uint64_t seed, uint32_t n_segments, uint32_t log2_min_le
         const uint64_t range = (uint64_t)(mx - mn);
         const uint32_t lo = mn + (uint32_t)(range / 4), hi = mn + (uint32_t)(range * 3 / 4);
         const bool meta = P[i].enc != FLS_ENC_BITPACK;
-        b->segs.push_back(SegDev{P[i].enc, P[i].bw, P[i].len, w, meta ? b->d_vmeta + meta_off[i] : nullptr,
+        b->segs.push_back(SegDev{P[i].enc, P[i].bw, P[i].enc == FLS_ENC_BITPACK ? 0u : 1u, 0u, P[i].len, w,
+                                 meta ? b->d_vmeta + meta_off[i] : nullptr,
                                  meta ? b->d_bases + 32 * meta_off[i] : nullptr,
                                  P[i].enc == FLS_ENC_RLE ? b->d_rmeta + meta_off[i] : nullptr, nullptr, lo, hi});
         const uint64_t seg_words = P[i].enc == FLS_ENC_BITPACK ? ceil_div(P[i].len, 1024) * 32u * P[i].bw : totals[i];

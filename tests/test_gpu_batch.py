@@ -85,7 +85,7 @@ def test_caller_described_segments():
     rng = np.random.default_rng(3)
     segs, want = [], []
     specs = [("bitpack", 5000, 9), ("delta", 70_000, 0), ("rle", 33_333, 14), ("bitpack", 1, 3), ("rle", 1024, 1),
-             ("mixed", 50_000, 13), ("rle_aux", 9000, 20), ("mixed", 700, 31)]
+             ("mixed", 50_000, 13), ("rle_aux", 9000, 20), ("mixed", 700, 31), ("bitpack_t", 3 * 1024 + 777, 13)]
     from fastlanesgpu_amd import selector as sel
 
     keep = []
@@ -107,6 +107,11 @@ def test_caller_described_segments():
             x = rng.integers(0, 2**bw, n).astype(np.uint32)[np.cumsum(starts) - 1]
             w, vm, rm, bs = fls.rle_encode_device(T.to_dev(x), n)
             col = fls.Column(fls.ENC_RLE, n, w, vmeta=vm, bases=bs, aux=rm)
+            rmeta = None
+        elif kind == "bitpack_t":  # bit-packed after transpose_i: its partial vector is masked in that order
+            x = rng.integers(0, 2**bw, n).astype(np.uint32)
+            col = fls.Column(fls.ENC_BITPACK, n, fls.pack_column_device(T.to_dev(x), n, bw, transposed=True), bw=bw,
+                             transposed=True)
             rmeta = None
         elif kind == "bitpack":
             x = rng.integers(0, 2**bw, n).astype(np.uint32)
