@@ -155,6 +155,22 @@ class Column:
         s = self.struct()
         return int(load().fls_column_bytes(ctypes.byref(s), self.n_rows))
 
+    def to_host(self):
+        """Copy the column's device arrays to a HostColumn (for serialization)."""
+        def host(t, dt, count=None):
+            if t is None:
+                return None
+            a = t.detach().cpu().numpy().view(dt)
+            return a[:count].copy() if count is not None else a.copy()
+
+        nv = (self.n_rows + VECTOR_SIZE - 1) // VECTOR_SIZE
+        meta = self.encoding != ENC_BITPACK
+        return HostColumn(self.encoding, self.n_rows, host(self.words, np.uint32, self.n_words), bw=self.bw,
+                          base=self.base, transposed=self.transposed,
+                          vmeta=host(self.vmeta, np.uint64, nv) if meta else None,
+                          bases=host(self.bases, np.uint32, 32 * nv) if self.encoding in (ENC_DELTA, ENC_RLE) else None,
+                          aux=host(self.aux, np.uint64, nv) if self.encoding in (ENC_RLE, ENC_MIXED) else None)
+
 
 def _torch_dev(a):
     import torch
@@ -181,6 +197,55 @@ class HostColumn:
         return Column(self.encoding, self.n_rows, _torch_dev(self.words), bw=self.bw, base=self.base,
                       transposed=self.transposed, vmeta=opt(self.vmeta), bases=opt(self.bases),
                       n_words=self.words.size, aux=opt(self.aux))
+
+    def struct(self):
+        """fls_column over the host arrays (valid while this object lives)."""
+        ptr = lambda a: None if a is None else a.ctypes.data  # noqa: E731
+        return FlsColumn(self.encoding, self.bw, self.base, int(self.transposed), self.words.size,
+                         ptr(self.words), ptr(self.vmeta), ptr(self.bases), ptr(self.aux))
+
+    def to_bytes(self):
+        """The serialized column image (fls_column_serialize): header + sections."""
+        s = self.struct()
+        n = int(load().fls_column_image_bytes(ctypes.byref(s), self.n_rows))
+        if n == 0:
+            raise FlsError("fls_column_image_bytes", -5, load())
+        out = np.zeros(n, np.uint8)
+        call("fls_column_serialize", ctypes.byref(s), self.n_rows, out.ctypes.data, n)
+        return out
+
+    @classmethod
+    def from_bytes(cls, image):
+        """Parse and validate an image (fls_column_deserialize); the arrays are
+        copied out of it."""
+        img = np.ascontiguousarray(np.frombuffer(image, np.uint8) if not isinstance(image, np.ndarray) else image,
+                                   np.uint8)
+        c = FlsColumn()
+        rows = ctypes.c_uint64()
+        call("fls_column_deserialize", img.ctypes.data, img.size, ctypes.byref(c), ctypes.byref(rows))
+        n = rows.value
+        nv = (n + VECTOR_SIZE - 1) // VECTOR_SIZE
+
+        def take(ptr, count, dt):
+            if not ptr or count == 0:
+                return None
+            off = ptr - img.ctypes.data
+            return img[off:off + count * np.dtype(dt).itemsize].view(dt).copy()
+
+        words = take(c.words, c.n_words, np.uint32)
+        words = words if words is not None else np.zeros(0, np.uint32)
+        has_meta = c.encoding != ENC_BITPACK
+        return cls(c.encoding, n, words, bw=c.bw, base=c.base, transposed=bool(c.transposed),
+                   vmeta=take(c.vmeta, nv if has_meta else 0, np.uint64),
+                   bases=take(c.bases, 32 * nv if c.encoding in (ENC_DELTA, ENC_RLE) else 0, np.uint32),
+                   aux=take(c.aux, nv if c.encoding in (ENC_RLE, ENC_MIXED) else 0, np.uint64))
+
+    def save(self, path):
+        self.to_bytes().tofile(path)
+
+    @classmethod
+    def load(cls, path):
+        return cls.from_bytes(np.fromfile(path, np.uint8))
 
     def nbytes(self):
         s = FlsColumn(self.encoding, self.bw, self.base, int(self.transposed), self.words.size)

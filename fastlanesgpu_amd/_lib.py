@@ -52,6 +52,9 @@ class Q11Params(ctypes.Structure):
 COLP = ctypes.POINTER(FlsColumn)
 
 SIGNATURES.update({
+    "fls_column_image_bytes": (c_u64, [COLP, c_u64]),
+    "fls_column_serialize": (c_int, [COLP, c_u64, c_vp, c_u64]),
+    "fls_column_deserialize": (c_int, [c_vp, c_u64, COLP, c_u64p]),
     "fls_query_q11": (c_int, [COLP, COLP, COLP, COLP, c_u64, ctypes.POINTER(Q11Params), c_u32, c_vp, c_vp, c_vp]),
     "fls_column_bytes": (c_u64, [COLP, c_u64]),
     "fls_gen_c1": (c_int, [c_u64, c_u64, c_u32, c_vp, c_vp]),

@@ -38,7 +38,8 @@ enum {
     FLS_ERR_ENCODING = -5,  /* unknown or unsupported encoding */
     FLS_ERR_ARG = -6,       /* other invalid argument */
     FLS_ERR_CAPACITY = -7,  /* output buffer too small */
-    FLS_ERR_NO_DEVICE = -8  /* no usable gfx950 device */
+    FLS_ERR_NO_DEVICE = -8, /* no usable gfx950 device */
+    FLS_ERR_FORMAT = -9     /* serialized column: bad magic, version, sizes or checksum */
 };
 
 #define FLS_VECTOR_SIZE 1024u
@@ -284,6 +285,33 @@ int fls_query_q21(const fls_column* datekey, const fls_column* suppkey, const fl
 /* Encoded bytes a full scan of a column reads for n_rows (packed words, plus
  * per-vector metadata and bases for DELTA).                                   */
 uint64_t fls_column_bytes(const fls_column* col, uint64_t n_rows);
+
+/* ================================================== serialized columns
+ * A self-describing image of one compressed column, for files and the wire
+ * (no reference counterpart; the reference keeps encoded columns in memory
+ * only, with per-vector metadata {bw, base, offset} as in
+ * fls_q21_bitpacked_opt_v4.cu:293-331).  Layout, little-endian:
+ *   fls_column_image header (128 bytes, magic "FLSCOL01", version 1),
+ *   then the sections words (u32 x n_words), vmeta (u64 x n_vmeta),
+ *   bases (u32 x n_bases), aux (u64 x n_aux), each starting at a multiple of
+ *   64 bytes from the image start; checksum = FNV-1a 64 over the sections.
+ * n_vmeta / n_bases / n_aux follow from the encoding and n_rows:
+ *   BITPACK 0 / 0 / 0, DELTA n_vec / 32 n_vec / 0, RLE n_vec / 32 n_vec /
+ *   n_vec (fls_rle_meta), MIXED n_vec / 0 / n_vec.
+ * Serialization reads host memory (copy device columns to the host first);
+ * deserialization validates the header, every size and the checksum, and
+ * returns a column whose pointers are views into the image (no copy).      */
+typedef struct fls_column_image {
+    char magic[8];
+    uint32_t version, encoding, bw, base, transposed, reserved;
+    uint64_t n_rows, n_words, n_vmeta, n_bases, n_aux;
+    uint64_t off_words, off_vmeta, off_bases, off_aux, total_bytes, checksum;
+    uint8_t pad[8];
+} fls_column_image;
+
+uint64_t fls_column_image_bytes(const fls_column* col, uint64_t n_rows);
+int fls_column_serialize(const fls_column* col, uint64_t n_rows, void* out, uint64_t out_bytes);
+int fls_column_deserialize(const void* image, uint64_t image_bytes, fls_column* col, uint64_t* n_rows);
 
 /* ===================================================== encoding selection
  * Classes: 4 encodings (FLS_MIX_*) x 10 width buckets

@@ -383,3 +383,19 @@ def test_learned_q11_exact(trained):
         keep.append(ws)
     assert tot_l <= 1.01 * tot_x, (tot_l, tot_x)
     assert fl.query_q11(*enc, n) == o_q11(cols, n)
+
+
+def test_learned_columns_serialize_and_query(trained):
+    """Learned mixed columns survive device -> image -> device; Q1.1 over the
+    reloaded columns equals the oracle."""
+    mlp, _ = trained
+    n = 300_000 + 17
+    cols = o_gen_c2(29, n)
+    reloaded, keep = [], []
+    for c in cols:
+        ws = sel.MixedWorkspace(n)
+        sel.LearnedSelector(mlp, conf_min=0.9).encode(to_dev(c), n, ws)
+        img = ws.column(n).to_host().to_bytes()
+        reloaded.append(fl.HostColumn.from_bytes(img).to_device())
+        keep.append(ws)
+    assert fl.query_q11(*reloaded, n) == o_q11(cols, n)
