@@ -459,6 +459,97 @@ def q21_groups(d_out, params=None):
     return rows
 
 
+# ----------------------------------------------------- Q3.1 / Q4.1 shapes
+from ._lib import Q31Params, Q41Params, StarDims  # noqa: E402
+
+SUPPLYCOST_BASE = 60000
+
+
+def q31_params(c_region=3, s_region=3, year_lo=1992, year_hi=1997, year_base=1992, n_years=7, n_nations=25,
+               n_slots=4375):
+    """Q3.1 constants (fls_q31.cu:84-85) with the region code of ASIA in the
+    1-based synthetic dimension domain."""
+    return Q31Params(c_region, s_region, year_lo, year_hi, year_base, n_years, n_nations, n_slots)
+
+
+def q41_params(c_region=2, s_region=2, mfgr_lo=1, mfgr_hi=2, year_base=1992, n_years=7, n_slots=175):
+    """Q4.1 constants (fls_q41.cu:97) with AMERICA and MFGR#1/#2 in the 1-based
+    synthetic dimension domain."""
+    return Q41Params(c_region, s_region, mfgr_lo, mfgr_hi, year_base, n_years, n_slots)
+
+
+def gen_star_extra(seed, n, n_part=800000, stream=None):
+    """Supplycost per fact row and manufacturer per part on the device."""
+    import torch
+
+    d = {"supplycost": torch.empty(max(1, n), dtype=torch.int32, device="cuda"),
+         "p_mfgr": torch.empty(max(1, n_part), dtype=torch.int32, device="cuda")}
+    call("fls_gen_star_extra", seed, n, d["supplycost"].data_ptr(), n_part, d["p_mfgr"].data_ptr(),
+         _stream(stream))
+    return d
+
+
+def encode_star_table(datefk, custfk, suppfk, partfk, revenue, supplycost, n_date=2556, n_cust=600000,
+                      n_supp=40000, n_part=800000):
+    """Star-join fact table (date, customer, supplier, part, revenue,
+    supplycost): FOR-packed 1-based keys, revenue at its width, supplycost
+    frame-of-reference from 60000."""
+    return [key_column(datefk, n_date), key_column(custfk, n_cust), key_column(suppfk, n_supp),
+            key_column(partfk, n_part), bitpack_column(revenue), bitpack_column(supplycost, base=SUPPLYCOST_BASE)]
+
+
+def star_dims_struct(d, keys_min=1):
+    p = lambda k: d[k].data_ptr() if k in d else None  # noqa: E731
+    return StarDims(p("s_region"), p("s_nation"), d["n_supp"], p("c_region"), p("c_nation"), d["n_cust"],
+                    p("p_mfgr"), d.get("n_part", 0) if "p_mfgr" in d else 0, p("d_year"), d["n_date"], keys_min)
+
+
+def _star_build(fn, dims, params, stream):
+    import torch
+
+    ds = star_dims_struct(dims)
+    nbytes = int(load().fls_star_workspace_bytes(ctypes.byref(ds)))
+    ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    call(fn, ctypes.byref(ds), ctypes.byref(params), ws.data_ptr(), nbytes, _stream(stream))
+    return ws
+
+
+def q31_build(dims, params=None, stream=None):
+    """Compact Q3.1 workspace (supplier / customer nation codes, year codes)."""
+    return _star_build("fls_q31_build", dims, params or q31_params(), stream)
+
+
+def q41_build(dims, params=None, stream=None):
+    """Compact Q4.1 workspace (supplier codes, customer nations, part bits, years);
+    dims needs "p_mfgr" (from gen_star_extra)."""
+    return _star_build("fls_q41_build", dims, params or q41_params(), stream)
+
+
+def query_q31(date, cust, supp, rev, n_rows, dims, ws, params=None, accumulate=False, d_out=None, stream=None):
+    """Fused decode + three joins + group-by (Q3.1 shape).  Returns the
+    [sums(n_slots) | counts(n_slots)] int64 device tensor."""
+    params = params or q31_params()
+    d_out, ps, pc = _q21_out(params, d_out)
+    cols = [c.struct() for c in (date, cust, supp, rev)]
+    ds = star_dims_struct(dims)
+    call("fls_query_q31", *(ctypes.byref(c) for c in cols), n_rows, ctypes.byref(ds), ws.data_ptr(),
+         ctypes.byref(params), ps, pc, Q_ACCUMULATE if accumulate else 0, _stream(stream))
+    return d_out
+
+
+def query_q41(date, cust, supp, part, rev, cost, n_rows, dims, ws, params=None, accumulate=False, d_out=None,
+              stream=None):
+    """Fused decode + four joins + group-by of revenue - supplycost (Q4.1
+    shape).  Returns the [sums(n_slots) | counts(n_slots)] int64 device tensor."""
+    params = params or q41_params()
+    d_out, ps, pc = _q21_out(params, d_out)
+    cols = [c.struct() for c in (date, cust, supp, part, rev, cost)]
+    ds = star_dims_struct(dims)
+    call("fls_query_q41", *(ctypes.byref(c) for c in cols), n_rows, ctypes.byref(ds), ws.data_ptr(),
+         ctypes.byref(params), ps, pc, Q_ACCUMULATE if accumulate else 0, _stream(stream))
+    return d_out
+
+
 # ----------------------------------------------------------- GPU encoders
 def _dev_u32(n):
     import torch

@@ -144,6 +144,37 @@ SIGNATURES.update({
                               c_u32, c_vp]),
 })
 
+class StarDims(ctypes.Structure):
+    """struct fls_star_dims."""
+    _fields_ = [("s_region", c_vp), ("s_nation", c_vp), ("n_supp", c_u32), ("c_region", c_vp), ("c_nation", c_vp),
+                ("n_cust", c_u32), ("p_mfgr", c_vp), ("n_part", c_u32), ("d_year", c_vp), ("n_date", c_u32),
+                ("keys_min", c_u32)]
+
+
+class Q31Params(ctypes.Structure):
+    """struct fls_q31_params."""
+    _fields_ = [("c_region", c_u32), ("s_region", c_u32), ("year_lo", c_u32), ("year_hi", c_u32),
+                ("year_base", c_u32), ("n_years", c_u32), ("n_nations", c_u32), ("n_slots", c_u32)]
+
+
+class Q41Params(ctypes.Structure):
+    """struct fls_q41_params."""
+    _fields_ = [("c_region", c_u32), ("s_region", c_u32), ("mfgr_lo", c_u32), ("mfgr_hi", c_u32),
+                ("year_base", c_u32), ("n_years", c_u32), ("n_slots", c_u32)]
+
+
+_SDP = ctypes.POINTER(StarDims)
+SIGNATURES.update({
+    "fls_gen_star_extra": (c_int, [c_u64, c_u64, c_vp, c_u32, c_vp, c_vp]),
+    "fls_star_workspace_bytes": (c_u64, [_SDP]),
+    "fls_q31_build": (c_int, [_SDP, ctypes.POINTER(Q31Params), c_vp, c_u64, c_vp]),
+    "fls_query_q31": (c_int, [COLP, COLP, COLP, COLP, c_u64, _SDP, c_vp, ctypes.POINTER(Q31Params), c_vp, c_vp,
+                              c_u32, c_vp]),
+    "fls_q41_build": (c_int, [_SDP, ctypes.POINTER(Q41Params), c_vp, c_u64, c_vp]),
+    "fls_query_q41": (c_int, [COLP, COLP, COLP, COLP, COLP, COLP, c_u64, _SDP, c_vp, ctypes.POINTER(Q41Params),
+                              c_vp, c_vp, c_u32, c_vp]),
+})
+
 SIGNATURES.update({
     "fls_encode_scratch_bytes": (c_u64, [c_u64]),
     "fls_pack_column_device": (c_int, [c_vp, c_u64, c_u32, c_u32, c_u32, c_vp, c_vp]),

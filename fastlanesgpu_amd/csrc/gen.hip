@@ -30,7 +30,8 @@ enum : uint64_t {
     ST_C1 = 1,
     ST_QTY = 11, ST_DISC = 12, ST_PRICE = 13,
     ST_DFK = 21, ST_SFK = 22, ST_PFK = 23, ST_CFK = 24, ST_REV = 25,
-    ST_SREG = 31, ST_SNAT = 32, ST_PCAT = 33, ST_PBRAND = 34, ST_CREG = 35, ST_CNAT = 36
+    ST_SREG = 31, ST_SNAT = 32, ST_PCAT = 33, ST_PBRAND = 34, ST_CREG = 35, ST_CNAT = 36,
+    ST_SCOST = 26, ST_PMFGR = 37
 };
 
 constexpr uint32_t kDays = 2557;  // 1992-01-01 .. 1998-12-31
@@ -97,6 +98,18 @@ __global__ void gen_year_kernel(uint32_t n, uint32_t* __restrict year) {
         year[i] = yyyymmdd(i) / 10000u;
 }
 
+// the columns the Q3.1 / Q4.1 shapes add: lo_supplycost U[60000, 120000),
+// p_mfgr 1 + U{0..4}
+__global__ void gen_star_extra_kernel(uint64_t seed, uint64_t n, uint32_t* __restrict cost, uint32_t n_part,
+                                      uint32_t* __restrict mfgr) {
+    const uint64_t kc = stream_key(seed, ST_SCOST), km = stream_key(seed, ST_PMFGR);
+    const uint64_t m = n > n_part ? n : n_part;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (i < n) cost[i] = 60000u + uniform(kc, i, 60000);
+        if (i < n_part) mfgr[i] = 1u + uniform(km, i, 5);
+    }
+}
+
 uint32_t grid_for(uint64_t n) {
     const uint64_t g = ceil_div(n, 256);
     return (uint32_t)(g < 65536 ? (g ? g : 1) : 65536);
@@ -134,6 +147,16 @@ int fls_gen_c3_fact(uint64_t seed, uint64_t n, uint32_t* d_datefk, uint32_t* d_s
     FLS_RETURN_IF(!d_datefk || !d_suppfk || !d_partfk || !d_revenue, FLS_ERR_NULL);
     hipLaunchKernelGGL(gen_c3_fact_kernel, dim3(grid_for(n)), dim3(256), 0, as_stream(stream), seed, n, d_datefk,
                        d_suppfk, d_partfk, d_custfk, d_revenue);
+    return check_launch();
+}
+
+int fls_gen_star_extra(uint64_t seed, uint64_t n, uint32_t* d_supplycost, uint32_t n_part, uint32_t* d_p_mfgr,
+                       void* stream) {
+    using namespace fls;
+    if (!n && !n_part) return FLS_OK;
+    FLS_RETURN_IF((n && !d_supplycost) || (n_part && !d_p_mfgr), FLS_ERR_NULL);
+    hipLaunchKernelGGL(gen_star_extra_kernel, dim3(grid_for(n > n_part ? n : n_part)), dim3(256), 0, as_stream(stream),
+                       seed, n, d_supplycost, n_part, d_p_mfgr);
     return check_launch();
 }
 

@@ -1,0 +1,458 @@
+// Fused decode + star joins + group-by for the Q3.1 and Q4.1 shapes
+// (reference: probe_v1 and build_hashtable_* of fastlanes/src/ssb/fls_q31.cu
+// and fls_q41.cu).  MI355X-native: the dimension tables are reduced once into
+// a compact workspace (one byte per supplier / customer / date: its nation or
+// year offset, 0xFF when the row is filtered out; one bit per part), the
+// probe kernel stages the small ones in LDS and keeps the customer codes
+// (600 KB) in L2, and the group-by runs in a per-block LDS table (4,375 or
+// 175 slots) flushed once per block: the ~4 M (Q3.1) / ~2 M (Q4.1) qualifying
+// rows of 120 M land in about 150 / 35 live slots, which global per-row
+// atomics would serialise on.
+#include "fls/device/unpack.hpp"
+#include "fls/device/wave.hpp"
+#include "fls_columns.hpp"
+#include "fls_runtime.hpp"
+
+namespace fls {
+namespace {
+
+constexpr uint8_t kNone = 0xFF;
+constexpr uint32_t kMaxSlots = 8192;
+
+// workspace layout (byte offsets, 256-aligned)
+struct StarLayout {
+    uint64_t s_code, c_code, p_bits, y_code, total;
+    uint32_t p_words;
+};
+
+StarLayout star_layout(const fls_star_dims& d) {
+    auto up = [](uint64_t x) { return (x + 255u) & ~(uint64_t)255u; };
+    StarLayout L{};
+    L.p_words = (d.n_part + 31u) / 32u;
+    L.s_code = 0;
+    L.c_code = up(L.s_code + d.n_supp);
+    L.p_bits = up(L.c_code + d.n_cust);
+    L.y_code = up(L.p_bits + 4ull * L.p_words);
+    L.total = up(L.y_code + d.n_date);
+    return L;
+}
+
+__global__ void star_zero_words(uint32_t* __restrict w, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) w[i] = 0u;
+}
+
+// Q3.1 builds (build_hashtable_s/_c/_d, fls_q31.cu:300-372): nation of the
+// qualifying suppliers and customers, year offset of the dates in range
+__global__ void q31_build_kernel(fls_star_dims d, fls_q31_params p, uint8_t* __restrict s_code,
+                                 uint8_t* __restrict c_code, uint8_t* __restrict y_code) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_supp; i += stride)
+        s_code[i] = (d.s_region[i] == p.s_region && d.s_nation[i] < kNone) ? (uint8_t)d.s_nation[i] : kNone;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_cust; i += stride)
+        c_code[i] = (d.c_region[i] == p.c_region && d.c_nation[i] < kNone) ? (uint8_t)d.c_nation[i] : kNone;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_date; i += stride) {
+        const uint32_t y = d.d_year[i];
+        y_code[i] = (y >= p.year_lo && y <= p.year_hi) ? (uint8_t)(y - p.year_base) : kNone;
+    }
+}
+
+// Q4.1 builds (fls_q41.cu:186-271): qualifying suppliers (code 0), parts (a
+// bit), customers (their nation), every date (its year offset)
+__global__ void q41_build_kernel(fls_star_dims d, fls_q41_params p, uint8_t* __restrict s_code,
+                                 uint8_t* __restrict c_code, uint32_t* __restrict p_bits,
+                                 uint8_t* __restrict y_code) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_supp; i += stride)
+        s_code[i] = d.s_region[i] == p.s_region ? (uint8_t)0 : kNone;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_cust; i += stride)
+        c_code[i] = (d.c_region[i] == p.c_region && d.c_nation[i] < kNone) ? (uint8_t)d.c_nation[i] : kNone;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_part; i += stride)
+        if (d.p_mfgr[i] >= p.mfgr_lo && d.p_mfgr[i] <= p.mfgr_hi) atomicOr(&p_bits[i >> 5], 1u << (i & 31u));
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_date; i += stride) {
+        const uint32_t y = d.d_year[i] - p.year_base;  // wraps below year_base: no match
+        y_code[i] = y < kNone ? (uint8_t)y : kNone;
+    }
+}
+
+struct StarDev {
+    ColDev date, cust, supp, part, rev, cost;
+    uint64_t n_rows, n_vec, n_groups;
+    const uint8_t *s_code, *c_code, *y_code;
+    const uint32_t* p_bits;
+    uint32_t n_supp, n_cust, n_part, n_date, keys_min, p_words;
+    uint32_t slot_mul_s, slot_mul_c, n_slots;  // slot = (s*slot_mul_s + c*slot_mul_c + y) % n_slots
+    int64_t* sums;
+    uint64_t* counts;
+};
+
+constexpr int kStarBlock = 1024;
+constexpr int kStarVecs = kStarBlock / 32;
+
+// per-block LDS: group table, then (USE_LDS) the staged dimension codes
+struct StarLds {
+    unsigned long long* sum;
+    uint32_t* cnt;
+    const uint8_t *s_code, *y_code;
+    const uint32_t* p_bits;
+};
+
+template <bool USE_LDS, bool PARTS>
+__device__ __forceinline__ StarLds star_stage(const StarDev& a, uint8_t* lds) {
+    StarLds t;
+    t.sum = reinterpret_cast<unsigned long long*>(lds);
+    t.cnt = reinterpret_cast<uint32_t*>(lds + 8ull * a.n_slots);
+    for (uint32_t i = threadIdx.x; i < a.n_slots; i += blockDim.x) {
+        t.sum[i] = 0ull;
+        t.cnt[i] = 0u;
+    }
+    t.s_code = a.s_code;
+    t.y_code = a.y_code;
+    t.p_bits = a.p_bits;
+    if constexpr (USE_LDS) {
+        uint8_t* q = lds + 12ull * a.n_slots;
+        uint32_t* pb = reinterpret_cast<uint32_t*>(q);  // part bits first: 4-byte aligned
+        if constexpr (PARTS) {
+            for (uint32_t i = threadIdx.x; i < a.p_words; i += blockDim.x) pb[i] = a.p_bits[i];
+            q += 4ull * a.p_words;
+            t.p_bits = pb;
+        }
+        uint8_t* s = q;
+        uint8_t* y = q + a.n_supp;
+        for (uint32_t i = threadIdx.x; i < a.n_supp; i += blockDim.x) s[i] = a.s_code[i];
+        for (uint32_t i = threadIdx.x; i < a.n_date; i += blockDim.x) y[i] = a.y_code[i];
+        t.s_code = s;
+        t.y_code = y;
+    }
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ void star_flush(const StarDev& a, const StarLds& t) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < a.n_slots; i += blockDim.x)
+        if (t.cnt[i]) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(&a.sums[i]), t.sum[i]);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&a.counts[i]), (unsigned long long)t.cnt[i]);
+        }
+}
+
+__device__ __forceinline__ uint8_t code_at(const uint8_t* __restrict code, uint32_t key, uint32_t kmin, uint32_t n) {
+    const uint32_t k = key - kmin;
+    return k < n ? code[k] : kNone;
+}
+
+// One byte per row r of a lane, four rows per register (r compile-time).
+__device__ __forceinline__ void put8(uint32_t (&p)[8], int r, uint32_t v) { p[r >> 2] |= v << (8 * (r & 3)); }
+__device__ __forceinline__ uint32_t get8(const uint32_t (&p)[8], int r) { return (p[r >> 2] >> (8 * (r & 3))) & 0xFFu; }
+
+// Q3.1 probe: supplier nation (LDS), customer nation (L2), year offset (LDS).
+// Fixed widths issue all four columns' loads before the first probe (bytes in
+// flight); the runtime-width path (32 registers per unpacked column) loads
+// each column where it is first used, so at most two are held at once.
+template <int DBW, int CBW, int SBW, int RBW, bool USE_LDS>
+__global__ __launch_bounds__(kStarBlock) void q31_kernel(const StarDev a) {
+    constexpr bool kEarly = DBW >= 0 && CBW >= 0 && SBW >= 0 && RBW >= 0;
+    extern __shared__ __align__(16) uint8_t lds[];
+    const StarLds t = star_stage<USE_LDS, false>(a, lds);
+    const uint32_t tail = live_row_mask((uint32_t)(a.n_rows - (a.n_vec - 1) * 1024u), a.rev.transposed != 0);
+    const uint32_t km = a.keys_min;
+    for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
+        const uint64_t v = g * kStarVecs + (threadIdx.x >> 5);
+        if (v >= a.n_vec) continue;
+        LaneCol<SBW> sc;
+        LaneCol<CBW> cc;
+        LaneCol<DBW> dc;
+        LaneCol<RBW> rc;
+        sc.load(a.supp, v);
+        if constexpr (kEarly) {
+            cc.load(a.cust, v);
+            dc.load(a.date, v);
+            rc.load(a.rev, v);
+        }
+        uint32_t m = 0, s8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            const uint32_t s = code_at(t.s_code, sc.get(r), km, a.n_supp);
+            put8(s8, r, s);
+            m |= (uint32_t)(s != kNone) << r;
+        }
+        if (v == a.n_vec - 1) m &= tail;
+        if constexpr (kEarly) {
+            cc.pin();
+            dc.pin();
+            rc.pin();
+        }
+        if (!__any(m != 0)) continue;
+        if constexpr (!kEarly) cc.load(a.cust, v);
+        uint32_t c8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            if (__any((m >> r) & 1u)) {
+                if ((m >> r) & 1u) {
+                    const uint32_t c = code_at(a.c_code, cc.get(r), km, a.n_cust);
+                    put8(c8, r, c);
+                    if (c == kNone) m &= ~(1u << r);
+                }
+            }
+        }
+        if (!__any(m != 0)) continue;
+        if constexpr (!kEarly) {
+            dc.load(a.date, v);
+            rc.load(a.rev, v);
+        }
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            if ((m >> r) & 1u) {
+                const uint32_t y = code_at(t.y_code, dc.get(r), km, a.n_date);
+                if (y != kNone) {
+                    const uint32_t slot = (get8(s8, r) * a.slot_mul_s + get8(c8, r) * a.slot_mul_c + y) % a.n_slots;
+                    atomicAdd(&t.sum[slot], (unsigned long long)rc.get(r));
+                    atomicAdd(&t.cnt[slot], 1u);
+                }
+            }
+        }
+    }
+    star_flush(a, t);
+}
+
+// Q4.1 probe: supplier and part pass (LDS), customer nation (L2), year offset
+// (LDS), revenue - supplycost.  Fixed widths load the last four columns while
+// the first two are probed; the runtime-width path loads two at a time.
+template <int DBW, int CBW, int SBW, int PBW, int RBW, int KBW, bool USE_LDS>
+__global__ __launch_bounds__(kStarBlock) void q41_kernel(const StarDev a) {
+    constexpr bool kEarly = DBW >= 0 && CBW >= 0 && SBW >= 0 && PBW >= 0 && RBW >= 0 && KBW >= 0;
+    extern __shared__ __align__(16) uint8_t lds[];
+    const StarLds t = star_stage<USE_LDS, true>(a, lds);
+    const uint32_t tail = live_row_mask((uint32_t)(a.n_rows - (a.n_vec - 1) * 1024u), a.rev.transposed != 0);
+    const uint32_t km = a.keys_min;
+    for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
+        const uint64_t v = g * kStarVecs + (threadIdx.x >> 5);
+        if (v >= a.n_vec) continue;
+        uint32_t m = 0;
+        {
+            LaneCol<SBW> sc;
+            LaneCol<PBW> pc;
+            sc.load(a.supp, v);
+            pc.load(a.part, v);
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                const uint32_t p = pc.get(r) - km;
+                const uint32_t hit = (code_at(t.s_code, sc.get(r), km, a.n_supp) != kNone) &
+                                     (p < a.n_part ? (t.p_bits[p >> 5] >> (p & 31u)) & 1u : 0u);
+                m |= hit << r;
+            }
+        }
+        if (v == a.n_vec - 1) m &= tail;
+        LaneCol<CBW> cc;
+        LaneCol<DBW> dc;
+        LaneCol<RBW> rc;
+        LaneCol<KBW> kc;
+        if constexpr (kEarly) {
+            cc.load(a.cust, v);
+            dc.load(a.date, v);
+            rc.load(a.rev, v);
+            kc.load(a.cost, v);
+            cc.pin();
+            dc.pin();
+            rc.pin();
+            kc.pin();
+            if (!__any(m != 0)) continue;
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                if ((m >> r) & 1u) {
+                    const uint32_t cn = code_at(a.c_code, cc.get(r), km, a.n_cust);
+                    const uint32_t y = code_at(t.y_code, dc.get(r), km, a.n_date);
+                    if (cn != kNone && y != kNone) {
+                        const uint32_t slot = (cn * a.slot_mul_c + y) % a.n_slots;
+                        const int64_t profit = (int64_t)rc.get(r) - (int64_t)kc.get(r);
+                        atomicAdd(&t.sum[slot], (unsigned long long)profit);
+                        atomicAdd(&t.cnt[slot], 1u);
+                    }
+                }
+            }
+        } else {
+            if (!__any(m != 0)) continue;
+            cc.load(a.cust, v);
+            dc.load(a.date, v);
+            uint32_t c8[8] = {0, 0, 0, 0, 0, 0, 0, 0}, y8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                if ((m >> r) & 1u) {
+                    const uint32_t cn = code_at(a.c_code, cc.get(r), km, a.n_cust);
+                    const uint32_t y = code_at(t.y_code, dc.get(r), km, a.n_date);
+                    put8(c8, r, cn);
+                    put8(y8, r, y);
+                    if (cn == kNone || y == kNone) m &= ~(1u << r);
+                }
+            }
+            if (!__any(m != 0)) continue;
+            rc.load(a.rev, v);
+            kc.load(a.cost, v);
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                if ((m >> r) & 1u) {
+                    const uint32_t slot = (get8(c8, r) * a.slot_mul_c + get8(y8, r)) % a.n_slots;
+                    const int64_t profit = (int64_t)rc.get(r) - (int64_t)kc.get(r);
+                    atomicAdd(&t.sum[slot], (unsigned long long)profit);
+                    atomicAdd(&t.cnt[slot], 1u);
+                }
+            }
+        }
+    }
+    star_flush(a, t);
+}
+
+template <typename Kernel>
+int launch_star(Kernel k, size_t lds, const StarDev& a, hipStream_t s) {
+    FLS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+    uint32_t grid = 0;
+    if (const int st = resident_grid_of(reinterpret_cast<const void*>(k), kStarBlock, lds, a.n_groups, &grid))
+        return st;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kStarBlock), lds, s, a);
+    return check_launch();
+}
+
+int star_columns(StarDev& a, const fls_column* const* cols, ColDev* const* outs, int n, uint64_t n_rows) {
+    for (int i = 0; i < n; ++i) {
+        if (!cols[i]) return FLS_ERR_NULL;
+        if (const int st = to_dev_column(*cols[i], n_rows, outs[i], false)) return st;
+        if ((cols[i]->transposed != 0) != (cols[0]->transposed != 0)) return FLS_ERR_ARG;
+    }
+    a.n_rows = n_rows;
+    a.n_vec = ceil_div(n_rows, 1024);
+    a.n_groups = ceil_div(a.n_vec, kStarVecs);
+    return FLS_OK;
+}
+
+int star_workspace(StarDev& a, const fls_star_dims* dims, const void* d_ws) {
+    const StarLayout L = star_layout(*dims);
+    const char* ws = static_cast<const char*>(d_ws);
+    a.s_code = reinterpret_cast<const uint8_t*>(ws + L.s_code);
+    a.c_code = reinterpret_cast<const uint8_t*>(ws + L.c_code);
+    a.p_bits = reinterpret_cast<const uint32_t*>(ws + L.p_bits);
+    a.y_code = reinterpret_cast<const uint8_t*>(ws + L.y_code);
+    a.n_supp = dims->n_supp;
+    a.n_cust = dims->n_cust;
+    a.n_part = dims->n_part;
+    a.n_date = dims->n_date;
+    a.p_words = L.p_words;
+    a.keys_min = dims->keys_min;
+    return FLS_OK;
+}
+
+int star_outputs(StarDev& a, uint32_t n_slots, int64_t* d_sums, uint64_t* d_counts, uint32_t flags, hipStream_t s) {
+    if (!d_sums || !d_counts) return FLS_ERR_NULL;
+    if (n_slots == 0 || n_slots > kMaxSlots) return FLS_ERR_ARG;
+    a.n_slots = n_slots;
+    a.sums = d_sums;
+    a.counts = d_counts;
+    if (!(flags & FLS_Q_ACCUMULATE)) {
+        if (const int st = zero_u64(reinterpret_cast<uint64_t*>(d_sums), n_slots, s)) return st;
+        return zero_u64(d_counts, n_slots, s);
+    }
+    return FLS_OK;
+}
+
+constexpr size_t kLdsBudget = 150u * 1024u;
+
+}  // namespace
+}  // namespace fls
+
+extern "C" {
+
+uint64_t fls_star_workspace_bytes(const fls_star_dims* dims) {
+    return dims ? fls::star_layout(*dims).total : 0;
+}
+
+int fls_q31_build(const fls_star_dims* dims, const fls_q31_params* params, void* d_workspace, uint64_t ws_bytes,
+                  void* stream) {
+    using namespace fls;
+    FLS_RETURN_IF(!dims || !params || !d_workspace, FLS_ERR_NULL);
+    FLS_RETURN_IF(!dims->s_region || !dims->s_nation || !dims->c_region || !dims->c_nation || !dims->d_year,
+                  FLS_ERR_NULL);
+    FLS_RETURN_IF(params->year_hi < params->year_lo || params->year_lo < params->year_base ||
+                      params->year_hi - params->year_base >= kNone,
+                  FLS_ERR_ARG);
+    const StarLayout L = star_layout(*dims);
+    FLS_RETURN_IF(ws_bytes < L.total, FLS_ERR_CAPACITY);
+    uint8_t* ws = static_cast<uint8_t*>(d_workspace);
+    hipLaunchKernelGGL(q31_build_kernel, dim3(1024), dim3(256), 0, as_stream(stream), *dims, *params, ws + L.s_code,
+                       ws + L.c_code, ws + L.y_code);
+    return check_launch();
+}
+
+int fls_query_q31(const fls_column* datekey, const fls_column* custkey, const fls_column* suppkey,
+                  const fls_column* revenue, uint64_t n_rows, const fls_star_dims* dims, const void* d_workspace,
+                  const fls_q31_params* params, int64_t* d_sums, uint64_t* d_counts, uint32_t flags, void* stream) {
+    using namespace fls;
+    FLS_RETURN_IF(!dims || !d_workspace || !params, FLS_ERR_NULL);
+    StarDev a{};
+    const fls_column* cols[4] = {datekey, custkey, suppkey, revenue};
+    ColDev* outs[4] = {&a.date, &a.cust, &a.supp, &a.rev};
+    if (const int st = star_columns(a, cols, outs, 4, n_rows)) return st;
+    hipStream_t s = as_stream(stream);
+    if (const int st = star_outputs(a, params->n_slots, d_sums, d_counts, flags, s)) return st;
+    if (n_rows == 0) return FLS_OK;
+    star_workspace(a, dims, d_workspace);
+    a.slot_mul_s = params->n_nations * params->n_years;
+    a.slot_mul_c = params->n_years;
+    const size_t table = 12ull * a.n_slots;
+    const size_t staged = table + dims->n_supp + dims->n_date;
+    const bool in_lds = staged <= kLdsBudget;
+    const size_t lds = in_lds ? staged : table;
+    const bool fast = a.date.bw == 12 && a.cust.bw == 20 && a.supp.bw == 16 && a.rev.bw == 24;
+    if (fast)
+        return in_lds ? launch_star(q31_kernel<12, 20, 16, 24, true>, lds, a, s)
+                      : launch_star(q31_kernel<12, 20, 16, 24, false>, lds, a, s);
+    return in_lds ? launch_star(q31_kernel<-1, -1, -1, -1, true>, lds, a, s)
+                  : launch_star(q31_kernel<-1, -1, -1, -1, false>, lds, a, s);
+}
+
+int fls_q41_build(const fls_star_dims* dims, const fls_q41_params* params, void* d_workspace, uint64_t ws_bytes,
+                  void* stream) {
+    using namespace fls;
+    FLS_RETURN_IF(!dims || !params || !d_workspace, FLS_ERR_NULL);
+    FLS_RETURN_IF(!dims->s_region || !dims->c_region || !dims->c_nation || !dims->p_mfgr || !dims->d_year,
+                  FLS_ERR_NULL);
+    const StarLayout L = star_layout(*dims);
+    FLS_RETURN_IF(ws_bytes < L.total, FLS_ERR_CAPACITY);
+    uint8_t* ws = static_cast<uint8_t*>(d_workspace);
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(star_zero_words, dim3(256), dim3(256), 0, s, reinterpret_cast<uint32_t*>(ws + L.p_bits),
+                       L.p_words);
+    hipLaunchKernelGGL(q41_build_kernel, dim3(1024), dim3(256), 0, s, *dims, *params, ws + L.s_code, ws + L.c_code,
+                       reinterpret_cast<uint32_t*>(ws + L.p_bits), ws + L.y_code);
+    return check_launch();
+}
+
+int fls_query_q41(const fls_column* datekey, const fls_column* custkey, const fls_column* suppkey,
+                  const fls_column* partkey, const fls_column* revenue, const fls_column* supplycost, uint64_t n_rows,
+                  const fls_star_dims* dims, const void* d_workspace, const fls_q41_params* params, int64_t* d_sums,
+                  uint64_t* d_counts, uint32_t flags, void* stream) {
+    using namespace fls;
+    FLS_RETURN_IF(!dims || !d_workspace || !params, FLS_ERR_NULL);
+    StarDev a{};
+    const fls_column* cols[6] = {datekey, custkey, suppkey, partkey, revenue, supplycost};
+    ColDev* outs[6] = {&a.date, &a.cust, &a.supp, &a.part, &a.rev, &a.cost};
+    if (const int st = star_columns(a, cols, outs, 6, n_rows)) return st;
+    hipStream_t s = as_stream(stream);
+    if (const int st = star_outputs(a, params->n_slots, d_sums, d_counts, flags, s)) return st;
+    if (n_rows == 0) return FLS_OK;
+    star_workspace(a, dims, d_workspace);
+    a.slot_mul_s = 0;
+    a.slot_mul_c = params->n_years;
+    const size_t table = 12ull * a.n_slots;
+    const size_t staged = table + 4ull * a.p_words + dims->n_supp + dims->n_date;
+    const bool in_lds = staged <= kLdsBudget;
+    const size_t lds = in_lds ? staged : table;
+    const bool fast = a.date.bw == 12 && a.cust.bw == 20 && a.supp.bw == 16 && a.part.bw == 20 && a.rev.bw == 24 &&
+                      a.cost.bw == 16;
+    if (fast)
+        return in_lds ? launch_star(q41_kernel<12, 20, 16, 20, 24, 16, true>, lds, a, s)
+                      : launch_star(q41_kernel<12, 20, 16, 20, 24, 16, false>, lds, a, s);
+    return in_lds ? launch_star(q41_kernel<-1, -1, -1, -1, -1, -1, true>, lds, a, s)
+                  : launch_star(q41_kernel<-1, -1, -1, -1, -1, -1, false>, lds, a, s);
+}
+
+}  // extern "C"

@@ -282,6 +282,75 @@ int fls_query_q21(const fls_column* datekey, const fls_column* suppkey, const fl
                   const fls_column* revenue, uint64_t n_rows, const fls_q21_dims* dims, const void* d_workspace,
                   const fls_q21_params* params, int64_t* d_sums, uint64_t* d_counts, uint32_t flags, void* stream);
 
+/* ============================================ star joins (Q3.1, Q4.1)
+ * Dense-key dimension columns for the three- and four-way star joins: row i of
+ * each dimension holds key keys_min + i.  Nations must be below 255 (rows with
+ * a larger nation do not match).  Unused columns may be NULL (p_mfgr for
+ * Q3.1; s_nation and p_mfgr's part count for Q4.1 are ignored when unused).  */
+typedef struct fls_star_dims {
+    const uint32_t* s_region;
+    const uint32_t* s_nation;
+    uint32_t n_supp;
+    const uint32_t* c_region;
+    const uint32_t* c_nation;
+    uint32_t n_cust;
+    const uint32_t* p_mfgr;
+    uint32_t n_part;
+    const uint32_t* d_year;
+    uint32_t n_date;
+    uint32_t keys_min;
+} fls_star_dims;
+
+/* Shape of fls_q31.cu:28-94 (probe_v1) with builds fls_q31.cu:300-372: rows
+ * whose customer has c_region == c_region, whose supplier has s_region ==
+ * s_region and whose date's year is in [year_lo, year_hi] add revenue to
+ *   slot = (s_nation * n_nations * n_years + c_nation * n_years
+ *           + (year - year_base)) % n_slots                 (fls_q31.cu:84-85)
+ * The reference's constants: n_nations 25, n_years 7, year_base 1992,
+ * n_slots 4375.                                                              */
+typedef struct fls_q31_params {
+    uint32_t c_region, s_region;
+    uint32_t year_lo, year_hi;
+    uint32_t year_base, n_years, n_nations;
+    uint32_t n_slots;
+} fls_q31_params;
+
+/* Shape of fls_q41.cu:27-107 (probe_v1) with builds fls_q41.cu:186-271: rows
+ * whose supplier has s_region == s_region, whose customer has c_region ==
+ * c_region and whose part has mfgr_lo <= p_mfgr <= mfgr_hi add
+ * revenue - supplycost (signed) to
+ *   slot = (c_nation * n_years + (year - year_base)) % n_slots  (fls_q41.cu:97)
+ * Dates whose year is outside [year_base, year_base + 254] do not match (every
+ * SSB date is 1992-1998).  The reference's constants: n_years 7, year_base
+ * 1992, n_slots 175.                                                          */
+typedef struct fls_q41_params {
+    uint32_t c_region, s_region;
+    uint32_t mfgr_lo, mfgr_hi;
+    uint32_t year_base, n_years;
+    uint32_t n_slots;
+} fls_q41_params;
+
+/* MI355X-native builds: one byte per supplier, customer and date (nation or
+ * year offset, 0xFF for a filtered row) and one bit per part, in a
+ * caller-owned workspace of fls_star_workspace_bytes() bytes.  The probes
+ * stage the supplier and date codes (and part bits) in LDS, read the customer
+ * codes through L2, and group in a per-block LDS table flushed once per block.
+ * d_sums / d_counts hold n_slots (<= 8192) entries each, overwritten unless
+ * flags has FLS_Q_ACCUMULATE.  Keys outside [keys_min, keys_min + n) do not
+ * match.  All fact columns are BITPACK with one row order.                   */
+uint64_t fls_star_workspace_bytes(const fls_star_dims* dims);
+int fls_q31_build(const fls_star_dims* dims, const fls_q31_params* params, void* d_workspace, uint64_t ws_bytes,
+                  void* stream);
+int fls_query_q31(const fls_column* datekey, const fls_column* custkey, const fls_column* suppkey,
+                  const fls_column* revenue, uint64_t n_rows, const fls_star_dims* dims, const void* d_workspace,
+                  const fls_q31_params* params, int64_t* d_sums, uint64_t* d_counts, uint32_t flags, void* stream);
+int fls_q41_build(const fls_star_dims* dims, const fls_q41_params* params, void* d_workspace, uint64_t ws_bytes,
+                  void* stream);
+int fls_query_q41(const fls_column* datekey, const fls_column* custkey, const fls_column* suppkey,
+                  const fls_column* partkey, const fls_column* revenue, const fls_column* supplycost,
+                  uint64_t n_rows, const fls_star_dims* dims, const void* d_workspace, const fls_q41_params* params,
+                  int64_t* d_sums, uint64_t* d_counts, uint32_t flags, void* stream);
+
 /* Encoded bytes a full scan of a column reads for n_rows (packed words, plus
  * per-vector metadata and bases for DELTA).                                   */
 uint64_t fls_column_bytes(const fls_column* col, uint64_t n_rows);
@@ -465,7 +534,9 @@ void fls_batch_destroy(fls_batch* b);
  *       supplier U[0,40000), part U[0,800000), customer U[0,600000) (may be
  *       NULL), revenue U[0,10^7); dimensions: year of each date row, supplier
  *       region U{1..5} / nation U{1..25}, part category U{1..25} / brand
- *       U{1..1000}, customer region / nation.                                */
+ *       U{1..1000}, customer region / nation.
+ *   star extra (Q3.1 / Q4.1 columns the c3 table lacks): supplycost of each
+ *       fact row U[60000, 120000), manufacturer of each part U{1..5}.       */
 int fls_gen_c1(uint64_t seed, uint64_t n, uint32_t bw, uint32_t* d_out, void* stream);
 int fls_gen_c2(uint64_t seed, uint64_t n, uint32_t* d_orderdate, uint32_t* d_quantity, uint32_t* d_discount,
                uint32_t* d_price, void* stream);
@@ -474,6 +545,8 @@ int fls_gen_c3_fact(uint64_t seed, uint64_t n, uint32_t* d_datefk, uint32_t* d_s
 int fls_gen_c3_dims(uint64_t seed, uint32_t n_date, uint32_t n_supp, uint32_t n_part, uint32_t n_cust,
                     uint32_t* d_year, uint32_t* d_s_region, uint32_t* d_s_nation, uint32_t* d_p_category,
                     uint32_t* d_p_brand, uint32_t* d_c_region, uint32_t* d_c_nation, void* stream);
+int fls_gen_star_extra(uint64_t seed, uint64_t n, uint32_t* d_supplycost, uint32_t n_part, uint32_t* d_p_mfgr,
+                       void* stream);
 
 #ifdef __cplusplus
 }

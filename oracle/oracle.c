@@ -241,7 +241,8 @@ enum {
     ST_C1 = 1,
     ST_QTY = 11, ST_DISC = 12, ST_PRICE = 13,
     ST_DFK = 21, ST_SFK = 22, ST_PFK = 23, ST_CFK = 24, ST_REV = 25,
-    ST_SREG = 31, ST_SNAT = 32, ST_PCAT = 33, ST_PBRAND = 34, ST_CREG = 35, ST_CNAT = 36
+    ST_SREG = 31, ST_SNAT = 32, ST_PCAT = 33, ST_PBRAND = 34, ST_CREG = 35, ST_CNAT = 36,
+    ST_SCOST = 26, ST_PMFGR = 37
 };
 
 void orc_gen_c1(uint64_t seed, uint64_t n, unsigned bw, uint32_t* v) {
@@ -297,6 +298,11 @@ void orc_gen_c3_dims(uint64_t seed, uint32_t n_date, uint32_t n_supp, uint32_t n
     }
 }
 
+void orc_gen_star_extra(uint64_t seed, uint64_t n, uint32_t* supplycost, uint32_t n_part, uint32_t* p_mfgr) {
+    for (uint64_t i = 0; i < n; ++i) supplycost[i] = 60000u + orc_uniform(seed, ST_SCOST, i, 60000);
+    for (uint32_t p = 0; p < n_part; ++p) p_mfgr[p] = 1u + orc_uniform(seed, ST_PMFGR, p, 5);
+}
+
 /* ==================================================================== queries */
 int64_t orc_q11(const uint32_t* orderdate, const uint32_t* quantity, const uint32_t* discount,
                 const uint32_t* price, uint64_t n, uint32_t date_gt, uint32_t date_lt, uint32_t qty_lt,
@@ -330,6 +336,48 @@ void orc_q21(const uint32_t* datefk, const uint32_t* suppfk, const uint32_t* par
         uint32_t brand = p_brand[partfk[i]];
         uint32_t slot = (brand * 7 + (year - 1992)) % Q21_SLOTS; /* fls_q21.cu:71 */
         sums[slot] += (int64_t)revenue[i];
+        counts[slot] += 1;
+    }
+}
+
+/* Q3.1 shape, fls_q31.cu:28-94 (probe_v1) and 300-372 (builds): suppliers of
+ * s_region == reg_s give s_nation, customers of c_region == reg_c give
+ * c_nation, dates with year_lo <= d_year <= year_hi give the year; slot
+ * (s_nation*25*7 + c_nation*7 + year - 1992) % 4375 (fls_q31.cu:84).
+ * Foreign keys are 0-based row indices of the dimension columns.          */
+void orc_q31(const uint32_t* datefk, const uint32_t* custfk, const uint32_t* suppfk, const uint32_t* revenue,
+             uint64_t n, const uint32_t* d_year, const uint32_t* s_region, const uint32_t* s_nation,
+             const uint32_t* c_region, const uint32_t* c_nation, uint32_t reg_c, uint32_t reg_s, uint32_t year_lo,
+             uint32_t year_hi, int64_t* sums, uint64_t* counts) {
+    memset(sums, 0, sizeof(int64_t) * 4375);
+    memset(counts, 0, sizeof(uint64_t) * 4375);
+    for (uint64_t i = 0; i < n; ++i) {
+        if (s_region[suppfk[i]] != reg_s) continue;
+        if (c_region[custfk[i]] != reg_c) continue;
+        const uint32_t year = d_year[datefk[i]];
+        if (year < year_lo || year > year_hi) continue;
+        const uint32_t slot = (s_nation[suppfk[i]] * 25 * 7 + c_nation[custfk[i]] * 7 + (year - 1992)) % 4375;
+        sums[slot] += (int64_t)revenue[i];
+        counts[slot] += 1;
+    }
+}
+
+/* Q4.1 shape, fls_q41.cu:27-107 (probe_v1) and 186-271 (builds): suppliers of
+ * s_region == reg_s and parts with mfgr_lo <= p_mfgr <= mfgr_hi pass, customers
+ * of c_region == reg_c give c_nation, every date gives its year; slot
+ * (c_nation*7 + year - 1992) % 175 (fls_q41.cu:97), sum of revenue - supplycost. */
+void orc_q41(const uint32_t* datefk, const uint32_t* custfk, const uint32_t* suppfk, const uint32_t* partfk,
+             const uint32_t* revenue, const uint32_t* supplycost, uint64_t n, const uint32_t* d_year,
+             const uint32_t* s_region, const uint32_t* c_region, const uint32_t* c_nation, const uint32_t* p_mfgr,
+             uint32_t reg_c, uint32_t reg_s, uint32_t mfgr_lo, uint32_t mfgr_hi, int64_t* sums, uint64_t* counts) {
+    memset(sums, 0, sizeof(int64_t) * 175);
+    memset(counts, 0, sizeof(uint64_t) * 175);
+    for (uint64_t i = 0; i < n; ++i) {
+        if (s_region[suppfk[i]] != reg_s) continue;
+        if (c_region[custfk[i]] != reg_c) continue;
+        if (p_mfgr[partfk[i]] < mfgr_lo || p_mfgr[partfk[i]] > mfgr_hi) continue;
+        const uint32_t slot = (c_nation[custfk[i]] * 7 + (d_year[datefk[i]] - 1992)) % 175;
+        sums[slot] += (int64_t)revenue[i] - (int64_t)supplycost[i];
         counts[slot] += 1;
     }
 }

@@ -25,7 +25,8 @@
  *   orc_rle_*                          format defined by this project (the
  *                                      reference's RLE is a CPU sketch):
  *                                      round trip + brute-force run counting.
- *   orc_q11 / orc_q21                  pinned: textbook semantics checked against
+ *   orc_q11 / orc_q21 / orc_q31 / orc_q41
+ *                                      pinned: textbook semantics checked against
  *                                      numpy brute force on small tables.
  *   orc_gen_* / orc_rand*              definitions of the synthetic inputs.
  *   orc_mix_* / orc_exhaustive_class   the encoders above run literally; sizes
@@ -128,6 +129,21 @@ int64_t orc_q11(const uint32_t* orderdate, const uint32_t* quantity, const uint3
 void orc_q21(const uint32_t* datefk, const uint32_t* suppfk, const uint32_t* partfk, const uint32_t* revenue,
              uint64_t n, const uint32_t* d_year, const uint32_t* s_region, const uint32_t* p_category,
              const uint32_t* p_brand, uint32_t cat, uint32_t reg, int64_t* sums, uint64_t* counts);
+
+/* Extra columns of the Q3.1 / Q4.1 shapes: lo_supplycost U[60000, 120000),
+ * p_mfgr 1 + U{0..4}.                                                          */
+void orc_gen_star_extra(uint64_t seed, uint64_t n, uint32_t* supplycost, uint32_t n_part, uint32_t* p_mfgr);
+/* Scalar Q3.1 shape (fls_q31.cu:28-94,300-372): sums/counts[4375].           */
+void orc_q31(const uint32_t* datefk, const uint32_t* custfk, const uint32_t* suppfk, const uint32_t* revenue,
+             uint64_t n, const uint32_t* d_year, const uint32_t* s_region, const uint32_t* s_nation,
+             const uint32_t* c_region, const uint32_t* c_nation, uint32_t reg_c, uint32_t reg_s, uint32_t year_lo,
+             uint32_t year_hi, int64_t* sums, uint64_t* counts);
+/* Scalar Q4.1 shape (fls_q41.cu:27-107,186-271): sums/counts[175] of
+ * revenue - supplycost.                                                       */
+void orc_q41(const uint32_t* datefk, const uint32_t* custfk, const uint32_t* suppfk, const uint32_t* partfk,
+             const uint32_t* revenue, const uint32_t* supplycost, uint64_t n, const uint32_t* d_year,
+             const uint32_t* s_region, const uint32_t* c_region, const uint32_t* c_nation, const uint32_t* p_mfgr,
+             uint32_t reg_c, uint32_t reg_s, uint32_t mfgr_lo, uint32_t mfgr_hi, int64_t* sums, uint64_t* counts);
 
 /* ---------------------------------------------------------------- batched segments
  * Definition of the batched-segments workload (the GPU builder generates the

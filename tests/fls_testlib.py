@@ -67,6 +67,10 @@ def oracle():
     o.orc_q11.argtypes = [U32P, U32P, U32P, U32P, u64, c_u, c_u, c_u, c_u, c_u, ctypes.POINTER(u64)]
     o.orc_q11.restype = ctypes.c_int64
     o.orc_q21.argtypes = [U32P, U32P, U32P, U32P, u64, U32P, U32P, U32P, U32P, c_u, c_u, I64P, U64P]
+    o.orc_gen_star_extra.argtypes = [u64, u64, U32P, c_u, U32P]
+    o.orc_q31.argtypes = [U32P, U32P, U32P, U32P, u64, U32P, U32P, U32P, U32P, U32P, c_u, c_u, c_u, c_u, I64P, U64P]
+    o.orc_q41.argtypes = [U32P, U32P, U32P, U32P, U32P, U32P, u64, U32P, U32P, U32P, U32P, U32P, c_u, c_u, c_u, c_u,
+                          I64P, U64P]
     _oracle = o
     return o
 
@@ -233,6 +237,43 @@ def o_q21(fact, dims, n, cat=12, reg=2):
     datefk, suppfk, partfk, rev = [np.ascontiguousarray(x, np.uint32) for x in fact]
     oracle().orc_q21(datefk, suppfk, partfk, rev, n, dims["d_year"], dims["s_region"], dims["p_category"],
                      dims["p_brand"], cat, reg, sums, counts)
+    return sums, counts
+
+
+STAR_NAMES = ("datefk", "custfk", "suppfk", "partfk", "revenue", "supplycost")
+
+
+def o_gen_star(seed, n, n_date=2556, n_supp=40000, n_part=800000, n_cust=600000):
+    """Star-join fact columns (0-based keys) and dimensions, as the GPU
+    generators write them: fact dict over STAR_NAMES, dims dict with p_mfgr."""
+    o = oracle()
+    f = {k: np.zeros(max(1, n), np.uint32) for k in STAR_NAMES}
+    o.orc_gen_c3_fact(seed, n, f["datefk"], f["suppfk"], f["partfk"], f["custfk"].ctypes.data, f["revenue"])
+    dims = [np.zeros(max(1, k), np.uint32) for k in (n_date, n_supp, n_supp, n_part, n_part, n_cust, n_cust)]
+    o.orc_gen_c3_dims(seed, n_date, n_supp, n_part, n_cust, *dims)
+    names = ["d_year", "s_region", "s_nation", "p_category", "p_brand", "c_region", "c_nation"]
+    d = dict(zip(names, dims))
+    d["p_mfgr"] = np.zeros(max(1, n_part), np.uint32)
+    o.orc_gen_star_extra(seed, n, f["supplycost"], n_part, d["p_mfgr"])
+    return {k: v[:n] for k, v in f.items()}, d
+
+
+def o_q31(f, d, n, reg_c=3, reg_s=3, year_lo=1992, year_hi=1997):
+    sums = np.zeros(4375, np.int64)
+    counts = np.zeros(4375, np.uint64)
+    c = [np.ascontiguousarray(f[k], np.uint32) for k in ("datefk", "custfk", "suppfk", "revenue")]
+    oracle().orc_q31(*c, n, d["d_year"], d["s_region"], d["s_nation"], d["c_region"], d["c_nation"], reg_c, reg_s,
+                     year_lo, year_hi, sums, counts)
+    return sums, counts
+
+
+def o_q41(f, d, n, reg_c=2, reg_s=2, mfgr_lo=1, mfgr_hi=2):
+    sums = np.zeros(175, np.int64)
+    counts = np.zeros(175, np.uint64)
+    c = [np.ascontiguousarray(f[k], np.uint32) for k in ("datefk", "custfk", "suppfk", "partfk", "revenue",
+                                                          "supplycost")]
+    oracle().orc_q41(*c, n, d["d_year"], d["s_region"], d["c_region"], d["c_nation"], d["p_mfgr"], reg_c, reg_s,
+                     mfgr_lo, mfgr_hi, sums, counts)
     return sums, counts
 
 

@@ -1,0 +1,170 @@
+"""GPU parity of the fused decode + star joins + group-by (Q3.1 and Q4.1
+shapes) against the oracle."""
+import numpy as np
+import pytest
+
+import fastlanesgpu_amd as fls
+import fls_testlib as T
+
+pytestmark = pytest.mark.gpu
+SMALL = dict(n_date=2556, n_supp=400, n_part=8000, n_cust=600)
+DIM_KEYS = ("d_year", "s_region", "s_nation", "c_region", "c_nation", "p_mfgr")
+
+
+def small_table(seed, n, sizes=SMALL):
+    f, d = T.o_gen_star(seed, n, **sizes)
+    f["suppfk"] %= sizes["n_supp"]
+    f["partfk"] %= sizes["n_part"]
+    f["custfk"] %= sizes["n_cust"]
+    return f, d
+
+
+def dev_dims(d_np, sizes):
+    torch = T.torch_cuda()
+    d = {k: torch.from_numpy(d_np[k].view(np.int32).copy()).cuda() for k in DIM_KEYS}
+    d.update(sizes)
+    return d
+
+
+FAST = (12, 20, 16, 20, 24, 16)  # the widths of the 120M-row table: fixed-width kernels
+
+
+def encode(f, sizes, widths=None, transposed=False):
+    """Minimal widths (widths None) or the given widths, in either row order."""
+    if widths is None and not transposed:
+        return fls.encode_star_table(*(f[k] for k in T.STAR_NAMES), n_date=sizes["n_date"],
+                                     n_cust=sizes["n_cust"], n_supp=sizes["n_supp"], n_part=sizes["n_part"])
+    widths = widths or [max(1, int(sizes[k] - 1).bit_length()) for k in ("n_date", "n_cust", "n_supp", "n_part")] \
+        + [25, 17]
+    n = f["datefk"].size
+    cols = [fls.HostColumn(fls.ENC_BITPACK, n, fls.pack_column(f[k], bw, transpose=transposed), bw=bw, base=1,
+                           transposed=transposed)
+            for k, bw in zip(("datefk", "custfk", "suppfk", "partfk"), widths)]
+    cols.append(fls.bitpack_column(f["revenue"], widths[4], transposed=transposed))
+    cols.append(fls.bitpack_column(f["supplycost"], widths[5], transposed=transposed, base=fls.SUPPLYCOST_BASE))
+    return cols
+
+
+def run_q31(f, d_np, sizes, n, enc=None):
+    d = dev_dims(d_np, sizes)
+    cols = [c.to_device() for c in (enc or encode(f, sizes))]
+    ws = fls.q31_build(d)
+    return fls.query_q31(cols[0], cols[1], cols[2], cols[4], n, d, ws).cpu().numpy()
+
+
+def run_q41(f, d_np, sizes, n, enc=None):
+    d = dev_dims(d_np, sizes)
+    cols = [c.to_device() for c in (enc or encode(f, sizes))]
+    ws = fls.q41_build(d)
+    return fls.query_q41(*cols, n, d, ws).cpu().numpy()
+
+
+def check(out, want):
+    sums, counts = want
+    k = sums.size
+    np.testing.assert_array_equal(out[k:].view(np.uint64), counts)
+    np.testing.assert_array_equal(out[:k], sums)
+
+
+def test_generators_match_oracle():
+    n, n_part = 70_001, 80_000
+    fd = fls.gen_c3_fact(6, n, with_customer=True)
+    ex = fls.gen_star_extra(6, n, n_part)
+    f, d = T.o_gen_star(6, n, n_part=n_part)
+    np.testing.assert_array_equal(fd["custfk"].cpu().numpy().view(np.uint32), f["custfk"])
+    np.testing.assert_array_equal(ex["supplycost"].cpu().numpy().view(np.uint32), f["supplycost"])
+    np.testing.assert_array_equal(ex["p_mfgr"].cpu().numpy().view(np.uint32), d["p_mfgr"])
+
+
+@pytest.mark.parametrize("n", [1, 700, 1024, 50_000 + 17])
+def test_q31_small(n):
+    f, d = small_table(31, n)
+    check(run_q31(f, d, SMALL, n, encode(f, SMALL, FAST)), T.o_q31(f, d, n))
+
+
+@pytest.mark.parametrize("n", [1, 700, 1024, 50_000 + 17])
+def test_q41_small(n):
+    f, d = small_table(41, n)
+    check(run_q41(f, d, SMALL, n, encode(f, SMALL, FAST)), T.o_q41(f, d, n))
+
+
+@pytest.mark.parametrize("transposed", [False, True])
+def test_runtime_widths_and_row_order(transposed):
+    # minimal key widths (9-13 bits here) take the run-time-width kernels
+    n = 30_000 + 5
+    f, d = small_table(32, n)
+    enc = encode(f, SMALL, None, transposed)
+    assert enc[2].bw == 9
+    check(run_q31(f, d, SMALL, n, enc), T.o_q31(f, d, n))
+    check(run_q41(f, d, SMALL, n, enc), T.o_q41(f, d, n))
+
+
+def test_transposed_fixed_widths():
+    n = 20_000 + 3
+    f, d = small_table(33, n)
+    enc = encode(f, SMALL, FAST, transposed=True)
+    check(run_q31(f, d, SMALL, n, enc), T.o_q31(f, d, n))
+    check(run_q41(f, d, SMALL, n, enc), T.o_q41(f, d, n))
+
+
+@pytest.mark.parametrize("sizes,widths", [
+    # supplier (Q3.1) and part (Q4.1) codes too large to stage: run-time widths
+    (dict(n_date=2556, n_supp=120_000, n_part=1_100_000, n_cust=600), None),
+    # part bits too large to stage next to the others (Q4.1), fixed widths
+    (dict(n_date=2556, n_supp=40_000, n_part=1 << 20, n_cust=600), FAST),
+])
+def test_dimensions_beyond_lds(sizes, widths):
+    n = 40_000 + 9
+    f, d = small_table(34, n, sizes)
+    enc = encode(f, sizes, widths)
+    check(run_q31(f, d, sizes, n, enc), T.o_q31(f, d, n))
+    check(run_q41(f, d, sizes, n, enc), T.o_q41(f, d, n))
+
+
+def test_accumulate_and_errors():
+    torch = T.torch_cuda()
+    n = 10_000
+    f, d_np = small_table(35, n)
+    d = dev_dims(d_np, SMALL)
+    cols = [c.to_device() for c in encode(f, SMALL)]
+    ws = fls.q41_build(d)
+    out = fls.query_q41(*cols, n, d, ws)
+    fls.query_q41(*cols, n, d, ws, accumulate=True, d_out=out)
+    sums, counts = T.o_q41(f, d_np, n)
+    check(out.cpu().numpy(), (2 * sums, 2 * counts))
+    with pytest.raises(fls.FlsError):
+        fls.query_q41(*cols, n, d, ws, params=fls.q41_params(n_slots=0))
+    with pytest.raises(fls.FlsError):  # one column in another row order
+        bad = fls.bitpack_column(f["revenue"], 24, transposed=True).to_device()
+        fls.query_q41(cols[0], cols[1], cols[2], cols[3], bad, cols[5], n, d, ws)
+    with pytest.raises(fls.FlsError):  # workspace too small
+        small = torch.empty(16, dtype=torch.uint8, device="cuda")
+        ds = fls.star_dims_struct(d)
+        fls.call("fls_q41_build", fls.ctypes.byref(ds), fls.ctypes.byref(fls.q41_params()), small.data_ptr(), 16,
+                 None)
+
+
+def test_full_size():
+    torch = T.torch_cuda()
+    n = 120_000_000
+    fd = fls.gen_c3_fact(3, n, with_customer=True)
+    ex = fls.gen_star_extra(3, n)
+    f = {k: fd[k].cpu().numpy().view(np.uint32) for k in ("datefk", "custfk", "suppfk", "partfk", "revenue")}
+    f["supplycost"] = ex["supplycost"].cpu().numpy().view(np.uint32)
+    del fd
+    dims = fls.gen_c3_dims(3)
+    dims["p_mfgr"] = ex.pop("p_mfgr")
+    d_np = {k: dims[k].cpu().numpy().view(np.uint32) for k in DIM_KEYS}
+    enc = encode(f, fls.C3_DIMS)
+    assert [c.bw for c in enc] == [12, 20, 16, 20, 24, 16]
+    cols = [c.to_device() for c in enc]
+    del enc
+    want31 = T.o_q31(f, d_np, n)
+    assert want31[1].sum() > 3_000_000  # 1/5 * 1/5 * 6/7 of the rows
+    out = fls.query_q31(cols[0], cols[1], cols[2], cols[4], n, dims, fls.q31_build(dims))
+    check(out.cpu().numpy(), want31)
+    want41 = T.o_q41(f, d_np, n)
+    assert want41[1].sum() > 1_500_000  # 1/5 * 1/5 * 2/5 of the rows
+    out = fls.query_q41(*cols, n, dims, fls.q41_build(dims))
+    check(out.cpu().numpy(), want41)
+    torch.cuda.synchronize()

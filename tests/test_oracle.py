@@ -222,6 +222,70 @@ def test_q21_brute_force():
     assert set(d_year.tolist()) == set(range(1992, 1999))
 
 
+def _star_small(seed, n):
+    sizes = dict(n_date=2556, n_supp=400, n_part=8000, n_cust=600)
+    f, d = T.o_gen_star(seed, n, **sizes)
+    f["suppfk"] %= sizes["n_supp"]
+    f["partfk"] %= sizes["n_part"]
+    f["custfk"] %= sizes["n_cust"]
+    return f, d
+
+
+def test_q31_brute_force():
+    # group by (supplier nation, customer nation, year) in plain Python, then
+    # decode each non-empty oracle slot back to its group: with nations 1..25,
+    # V = 25*s_nation + c_nation runs over 26..650 and the slot keeps V mod 625,
+    # so V = q below 26 means q + 625 (e.g. (24, 25) wraps to slots 0..6)
+    n = 40_000
+    f, d = _star_small(21, n)
+    sums, counts = T.o_q31(f, d, n)
+    want, want_n = {}, {}
+    for i in range(n):
+        s, c, y = int(f["suppfk"][i]), int(f["custfk"][i]), int(d["d_year"][f["datefk"][i]])
+        if d["s_region"][s] == 3 and d["c_region"][c] == 3 and 1992 <= y <= 1997:
+            key = (int(d["s_nation"][s]), int(d["c_nation"][c]), y)
+            want[key] = want.get(key, 0) + int(f["revenue"][i])
+            want_n[key] = want_n.get(key, 0) + 1
+    got, got_n = {}, {}
+    for slot in np.nonzero(counts)[0]:
+        y = 1992 + int(slot) % 7
+        q = int(slot) // 7
+        v = q if q >= 26 else q + 625
+        cn = (v - 1) % 25 + 1
+        key = ((v - cn) // 25, cn, y)
+        got[key] = int(sums[slot])
+        got_n[key] = int(counts[slot])
+    assert got == want and got_n == want_n and len(want) > 200
+    assert max(k[2] for k in want) == 1997  # 1998 is filtered out
+
+
+def test_q41_brute_force():
+    n = 40_000
+    f, d = _star_small(22, n)
+    sums, counts = T.o_q41(f, d, n)
+    want = {}
+    for i in range(n):
+        s, c, p = int(f["suppfk"][i]), int(f["custfk"][i]), int(f["partfk"][i])
+        if d["s_region"][s] == 2 and d["c_region"][c] == 2 and 1 <= d["p_mfgr"][p] <= 2:
+            key = (int(d["c_nation"][c]), int(d["d_year"][f["datefk"][i]]))
+            want[key] = want.get(key, 0) + int(f["revenue"][i]) - int(f["supplycost"][i])
+    got = {}
+    for slot in np.nonzero(counts)[0]:
+        cn = int(slot) // 7
+        got[(cn if cn > 0 else 25, 1992 + int(slot) % 7)] = int(sums[slot])
+    assert got == want and len(want) > 50
+    assert (f["revenue"] < f["supplycost"]).any()  # some rows add a negative profit
+
+
+def test_star_extra_ranges():
+    f, d = T.o_gen_star(5, 100_000, n_part=50_000)
+    assert f["supplycost"].min() >= 60000 and f["supplycost"].max() < 120000
+    assert f["supplycost"].max() - f["supplycost"].min() > 59000
+    assert set(np.unique(d["p_mfgr"]).tolist()) == {1, 2, 3, 4, 5}
+    # custfk is drawn by the same generator as the Q2.1 table's keys
+    assert f["custfk"].max() < 600000 and len(np.unique(f["custfk"])) > 90_000
+
+
 def test_rle_roundtrip_and_run_count():
     o = T.oracle()
     for n in (1024, 1000, 1, 513):
