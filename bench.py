@@ -209,6 +209,48 @@ def bench_c3(steps, warmup, seed):
             "groups": int((res_c[7000:] > 0).sum().item()), "paths_agree": bool(torch.equal(res_c, out))}
 
 
+def bench_star(steps, warmup, seed):
+    """Fused decode + star joins + group-by (Q3.1 and Q4.1 shapes), 120M rows."""
+    import torch
+
+    import fastlanesgpu_amd as fls
+
+    n = 120_000_000
+    fd = fls.gen_c3_fact(seed, n, with_customer=True)
+    ex = fls.gen_star_extra(seed, n)
+    f = [fd[k].cpu().numpy().view("uint32") for k in ("datefk", "custfk", "suppfk", "partfk", "revenue")]
+    f.append(ex.pop("supplycost").cpu().numpy().view("uint32"))
+    del fd
+    enc = fls.encode_star_table(*f)
+    del f
+    cols = [c.to_device() for c in enc]
+    dims = fls.gen_c3_dims(seed)
+    dims["p_mfgr"] = ex.pop("p_mfgr")
+    res = {"workload": "120M fact rows, dimensions 2556 dates / 600K customers / 40K suppliers / 800K parts; "
+                       "keys at 12/20/16/20 bits, revenue 24, supplycost 16 (FOR)"}
+    q31_cols = [cols[0], cols[1], cols[2], cols[4]]
+    for name, run, build, used, n_slots in (
+            ("q31", lambda ws, out: fls.query_q31(*q31_cols, n, dims, ws, accumulate=True, d_out=out),
+             fls.q31_build, [0, 1, 2, 4], 4375),
+            ("q41", lambda ws, out: fls.query_q41(*cols, n, dims, ws, accumulate=True, d_out=out),
+             fls.q41_build, [0, 1, 2, 3, 4, 5], 175)):
+        ws = build(dims)
+        out = torch.zeros(2 * n_slots, dtype=torch.int64, device="cuda")
+        _, per = time_kernel(lambda: run(ws, out), out, steps, warmup)
+        ms = statistics.mean(per)
+        nbytes = sum(enc[i].nbytes() for i in used)
+        one = torch.zeros_like(out)
+        run(ws, one)
+        total = torch.zeros_like(out)
+        for _ in range(3):
+            run(ws, total)
+        res[name] = {"encoded_bytes": nbytes, "kernel_ms": ms, "rows_per_s": n / (ms * 1e-3),
+                     "GBs": nbytes / (ms * 1e-3) / 1e9, "frac_hbm": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "groups": int((one[n_slots:] > 0).sum().item()), "rows_qualifying": int(one[n_slots:].sum()),
+                     "repeat_consistent": bool(torch.equal(total, 3 * one))}
+    return res
+
+
 def bench_c4(steps, seed):
     """4096 uneven segments (bitpack/delta/RLE) decoded + range-filtered:
     one graph-replayed persistent launch vs one launch per segment."""
@@ -543,6 +585,8 @@ def main():
         line["secondary"] = {"c1_unpack": bench_c1(args.steps, args.warmup, args.seed)}
         log("[bench] secondary: probe + group-by")
         line["secondary"]["c3_probe_groupby"] = bench_c3(max(5, args.steps // 2), 2, args.seed)
+        log("[bench] secondary: star joins")
+        line["secondary"]["star_joins"] = bench_star(max(5, args.steps // 2), 2, args.seed)
         log("[bench] secondary: batched segments")
         line["secondary"]["c4_batched"] = bench_c4(max(5, args.steps // 5), args.seed)
         if not args.no_c5:

@@ -69,6 +69,87 @@ __device__ __forceinline__ void decode_delta(const ColDev& c, uint64_t v, uint32
     dev::rsum_lane(x, c.bases[v * 32u + dev::lane32()]);
 }
 
+// Value k of a linear LSB-first bit-packing at width bw (bw <= 32).
+__device__ __forceinline__ uint32_t linear_extract(const uint32_t* __restrict w, uint32_t k, uint32_t bw) {
+    if (bw == 0) return 0u;
+    const uint32_t bit = k * bw, i = bit >> 5, s = bit & 31u;
+    const uint32_t lo = w[i];
+    const uint32_t hi = (s + bw > 32u) ? w[i + 1] : 0u;
+    const uint32_t mask = bw >= 32 ? 0xffffffffu : ((1u << bw) - 1u);
+    return __builtin_amdgcn_alignbit(hi, lo, s) & mask;
+}
+
+// Any column encoding (BITPACK, DELTA, RLE, MIXED) decoded into registers with
+// no LDS and no wave barrier, for kernels whose LDS holds dimension tables.
+// Every encoding reduces to one unpack at a run-time width, then a FOR add, or
+// the in-register rsum (delta, and the run index of RLE), and for RLE a gather
+// of the run values straight from the column (a vector's run values are at
+// most 1024 packed values, so the gathers hit the cache after the first).
+// Row order: normal for BITPACK unless c.transposed, transposed otherwise.
+__device__ __forceinline__ void decode_regs(const ColDev& c, uint64_t v, uint32_t (&x)[32]) {
+    const uint32_t l = dev::lane32();
+    const uint32_t* p;
+    uint32_t w, add = 0, seed = 0, nr = 0, gbase = 0, gbw = 0;
+    bool rsum = false;
+    const uint32_t* vals = nullptr;
+    if (c.enc == FLS_ENC_BITPACK) {
+        p = c.words + v * (32u * c.bw);
+        w = c.bw;
+        add = c.base;
+    } else if (c.enc == FLS_ENC_MIXED) {
+        const fls_vmeta m = c.vmeta[v];
+        const uint32_t enc = (uint32_t)(m >> 6) & 3u, wm = (uint32_t)m & 63u;
+        const uint64_t ax = c.aux[v];
+        const uint32_t* q = c.words + FLS_VMETA_OFFSET(m);
+        p = q;
+        w = wm;
+        if (enc == FLS_MIX_FOR) {
+            add = (uint32_t)ax;
+        } else if (enc == FLS_MIX_DELTA || enc == FLS_MIX_RLE) {
+            rsum = true;
+            seed = q[l];
+            p = q + 32u;
+            if (enc == FLS_MIX_RLE) {
+                const uint32_t ibw = (uint32_t)(ax >> 48) & 1u, n_field = (uint32_t)(ax >> 32) & 0xffffu;
+                w = ibw;
+                vals = q + 32u + 32u * ibw;
+                gbase = (uint32_t)ax;
+                gbw = wm;
+                nr = n_field < 1024u ? n_field : 1024u;
+            }
+        }
+    } else {  // DELTA, RLE: (offset, width) per vector, 32 bases per vector
+        const fls_vmeta m = c.vmeta[v];
+        p = c.words + FLS_VMETA_OFFSET(m);
+        w = FLS_VMETA_BW(m);
+        rsum = true;
+        seed = c.bases[v * 32u + l];
+        if (c.enc == FLS_ENC_RLE) {
+            const fls_rle_meta rm = reinterpret_cast<const fls_rle_meta*>(c.aux)[v];
+            vals = p + 32u * w;
+            gbase = rm.vbase;
+            gbw = rm.vbw;
+            nr = rm.n_runs < 1024u ? rm.n_runs : 1024u;
+        }
+    }
+    dev::unpack_device(p, x, w);
+    if (rsum) dev::rsum_lane(x, seed);
+    if (vals) {
+        const uint32_t last = nr ? nr - 1u : 0u;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            x[r] = gbase + linear_extract(vals, min(x[r], last), gbw);
+            // one row's gather at a time: left free, the scheduler hoists all
+            // 32 rows' loads (64 addresses of 64 bits) and the kernels that
+            // hold other columns' codes beside this one spill
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    } else if (add) {
+#pragma unroll
+        for (int r = 0; r < 32; ++r) x[r] += add;
+    }
+}
+
 // Keep the asm-visible value of each word live here: the loads that produced
 // them are then issued (and complete) before any data-dependent branch that
 // follows, so a full scan reads every byte even when a wave skips its ALU work.
@@ -78,17 +159,23 @@ __device__ __forceinline__ void pin_words(uint32_t (&w)[N]) {
     for (int k = 0; k < N; ++k) asm volatile("" : "+v"(w[k]));
 }
 
-// One lane of one vector of a fixed-width column, held for row-by-row use.
-// BW >= 0: the packed words (BW registers), each row extracted on demand.
-// BW < 0: width known only at run time; the 32 rows are decoded up front.
+// One lane of one vector of a column, held for row-by-row use.
+// BW >= 0: a BITPACK column of that width: the packed words (BW registers),
+// each row extracted on demand.
+// BW < 0: any encoding and width, known only at run time: the 32 rows are
+// decoded up front by decode_regs.
 template <int BW>
 struct LaneCol {
     uint32_t w[BW > 0 ? BW : (BW == 0 ? 1 : 32)];
     uint32_t base;
     __device__ __forceinline__ void load(const ColDev& c, uint64_t v) {
-        base = c.base;
-        if constexpr (BW >= 0) dev::load_lane<BW>(c.words + v * (32u * BW) + dev::lane32(), w);
-        else dev::unpack_device(c.words + v * (32u * c.bw), w, c.bw);
+        if constexpr (BW >= 0) {
+            base = c.base;
+            dev::load_lane<BW>(c.words + v * (32u * BW) + dev::lane32(), w);
+        } else {
+            base = 0;
+            decode_regs(c, v, w);
+        }
     }
     __device__ __forceinline__ uint32_t get(int r) const {
         if constexpr (BW >= 0) return dev::extract_at<BW>(w, r) + base;
@@ -119,16 +206,6 @@ __device__ __forceinline__ uint32_t live_row_mask(uint32_t live, bool transposed
     const uint32_t part = 0x11111111u & ((1u << (4u * s)) - 1u);  // s <= 7
     const uint32_t sh = q < 4 ? (0x3120u >> (4u * q)) & 3u : 0u;
     return m | (part << sh);
-}
-
-// Value k of a linear LSB-first bit-packing at width bw (bw <= 32).
-__device__ __forceinline__ uint32_t linear_extract(const uint32_t* __restrict w, uint32_t k, uint32_t bw) {
-    if (bw == 0) return 0u;
-    const uint32_t bit = k * bw, i = bit >> 5, s = bit & 31u;
-    const uint32_t lo = w[i];
-    const uint32_t hi = (s + bw > 32u) ? w[i + 1] : 0u;
-    const uint32_t mask = bw >= 32 ? 0xffffffffu : ((1u << bw) - 1u);
-    return __builtin_amdgcn_alignbit(hi, lo, s) & mask;
 }
 
 // FastLanes RLE decode of vector v, one lane, into 32 rows (transposed order).

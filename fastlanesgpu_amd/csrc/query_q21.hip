@@ -219,6 +219,78 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
     }
 }
 
+// Two 16-bit values per register, row r in half r & 1 of register r >> 1.
+__device__ __forceinline__ void put16(uint32_t (&p)[16], int r, uint32_t v) { p[r >> 1] |= v << (16 * (r & 1)); }
+__device__ __forceinline__ uint32_t get16(const uint32_t (&p)[16], int r) { return (p[r >> 1] >> (16 * (r & 1))) & 0xFFFFu; }
+
+// Compact-table probe over columns of any encoding and width: one column
+// decoded at a time (decode_regs, 32 registers), with only the row mask and
+// the rows' brands and years (16 bits each) carried between columns.
+template <bool USE_LDS>
+__global__ __launch_bounds__(1024) void q21_any_kernel(const Q21Dev a) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t* supp_bm = a.supp_bm;
+    const uint32_t* part_bm = a.part_bm;
+    const uint16_t* year_tab = a.year;
+    if constexpr (USE_LDS) {
+        uint32_t* s = lds;
+        uint32_t* p = lds + a.supp_words;
+        uint16_t* y = reinterpret_cast<uint16_t*>(p + a.part_words);
+        for (uint32_t i = threadIdx.x; i < a.supp_words; i += blockDim.x) s[i] = a.supp_bm[i];
+        for (uint32_t i = threadIdx.x; i < a.part_words; i += blockDim.x) p[i] = a.part_bm[i];
+        for (uint32_t i = threadIdx.x; i < a.n_date; i += blockDim.x) y[i] = a.year[i];
+        __syncthreads();
+        supp_bm = s;
+        part_bm = p;
+        year_tab = y;
+    }
+    const uint32_t tail = live_row_mask((uint32_t)(a.n_rows - (a.n_vec - 1) * 1024u), a.rev.transposed != 0);
+    const uint32_t km = a.keys_min;
+    for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
+        const uint64_t v = g * 32u + (threadIdx.x >> 5);
+        if (v >= a.n_vec) continue;
+        uint32_t x[32];
+        uint32_t m = (v == a.n_vec - 1) ? tail : 0xFFFFFFFFu;
+        decode_regs(a.supp, v, x);
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            const uint32_t k = x[r] - km;
+            const uint32_t hit = k < a.n_supp ? (supp_bm[k >> 5] >> (k & 31u)) & 1u : 0u;
+            m &= ~((hit ^ 1u) << r);
+        }
+        if (!__any(m != 0)) continue;
+        decode_regs(a.part, v, x);
+        uint32_t b16[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            const uint32_t k = x[r] - km;
+            const uint32_t hit = k < a.n_part ? (part_bm[k >> 5] >> (k & 31u)) & 1u : 0u;
+            m &= ~((hit ^ 1u) << r);
+        }
+        if (!__any(m != 0)) continue;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            const bool live = (m >> r) & 1u;
+            put16(b16, r, live ? a.brand[x[r] - km] : 0u);  // passed the part bitmap: key in range
+            if ((r & 7) == 7) __builtin_amdgcn_sched_barrier(0);  // 8 rows' loads in flight
+        }
+        decode_regs(a.date, v, x);
+        uint32_t y16[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            const uint32_t k = x[r] - km;
+            const bool ok = k < a.n_date;
+            put16(y16, r, ok ? year_tab[k] : 0u);
+            if (!ok) m &= ~(1u << r);
+        }
+        if (!__any(m != 0)) continue;
+        decode_regs(a.rev, v, x);
+#pragma unroll
+        for (int r = 0; r < 32; ++r)
+            if ((m >> r) & 1u) group_add(a, get16(b16, r), get16(y16, r), x[r]);
+    }
+}
+
 template <typename Kernel>
 int launch_resident_q21(Kernel k, int block, size_t lds, const Q21Dev& a, hipStream_t s) {
     int per_cu = 0;
@@ -242,16 +314,19 @@ int launch_compact(const Q21Dev& a, bool in_lds, size_t lds, hipStream_t s) {
     return launch_resident_q21(q21_compact_kernel<DBW, SBW, PBW, RBW, false>, 1024, 0, a, s);
 }
 
+// allow_var: the compact probe decodes any encoding (decode_regs); the
+// reference-table probe reads BITPACK columns only.  Either way the four
+// columns decode in one row order.
 int fill_fact(Q21Dev& a, const fls_column* date, const fls_column* supp, const fls_column* part,
-              const fls_column* rev, uint64_t n_rows) {
+              const fls_column* rev, uint64_t n_rows, bool allow_var) {
     if (!date || !supp || !part || !rev) return FLS_ERR_NULL;
     int st;
-    if ((st = to_dev_column(*date, n_rows, &a.date, false)) != FLS_OK) return st;
-    if ((st = to_dev_column(*supp, n_rows, &a.supp, false)) != FLS_OK) return st;
-    if ((st = to_dev_column(*part, n_rows, &a.part, false)) != FLS_OK) return st;
-    if ((st = to_dev_column(*rev, n_rows, &a.rev, false)) != FLS_OK) return st;
-    if (date->transposed != supp->transposed || date->transposed != part->transposed ||
-        date->transposed != rev->transposed)
+    if ((st = to_dev_column(*date, n_rows, &a.date, allow_var)) != FLS_OK) return st;
+    if ((st = to_dev_column(*supp, n_rows, &a.supp, allow_var)) != FLS_OK) return st;
+    if ((st = to_dev_column(*part, n_rows, &a.part, allow_var)) != FLS_OK) return st;
+    if ((st = to_dev_column(*rev, n_rows, &a.rev, allow_var)) != FLS_OK) return st;
+    if (a.date.transposed != a.supp.transposed || a.date.transposed != a.part.transposed ||
+        a.date.transposed != a.rev.transposed)
         return FLS_ERR_ARG;
     a.n_rows = n_rows;
     a.n_vec = ceil_div(n_rows, 1024);
@@ -299,7 +374,7 @@ int fls_query_q21_pht(const fls_column* datekey, const fls_column* suppkey, cons
     FLS_RETURN_IF(!ht || !ht->ht_s || !ht->ht_p || !ht->ht_d, FLS_ERR_NULL);
     FLS_RETURN_IF(!ht->s_len || !ht->p_len || !ht->d_len, FLS_ERR_ARG);
     Q21Dev a{};
-    int st = fill_fact(a, datekey, suppkey, partkey, revenue, n_rows);
+    int st = fill_fact(a, datekey, suppkey, partkey, revenue, n_rows, false);
     if (st) return st;
     hipStream_t s = as_stream(stream);
     if ((st = fill_groups(a, params, d_sums, d_counts, flags, s)) != FLS_OK) return st;
@@ -348,7 +423,7 @@ int fls_query_q21(const fls_column* datekey, const fls_column* suppkey, const fl
     using namespace fls;
     FLS_RETURN_IF(!dims || !d_workspace, FLS_ERR_NULL);
     Q21Dev a{};
-    int st = fill_fact(a, datekey, suppkey, partkey, revenue, n_rows);
+    int st = fill_fact(a, datekey, suppkey, partkey, revenue, n_rows, true);
     if (st) return st;
     hipStream_t s = as_stream(stream);
     if ((st = fill_groups(a, params, d_sums, d_counts, flags, s)) != FLS_OK) return st;
@@ -368,9 +443,16 @@ int fls_query_q21(const fls_column* datekey, const fls_column* suppkey, const fl
     a.n_groups = ceil_div(a.n_vec, 32);
     const size_t lds = 4ull * (L.supp_words + L.part_words) + 2ull * dims->n_date + 16u;
     const bool in_lds = lds <= 150u * 1024u;
-    const bool fast = a.date.bw == 12 && a.supp.bw == 16 && a.part.bw == 20 && a.rev.bw == 24;
+    const bool fast = a.date.enc == FLS_ENC_BITPACK && a.supp.enc == FLS_ENC_BITPACK &&
+                      a.part.enc == FLS_ENC_BITPACK && a.rev.enc == FLS_ENC_BITPACK && a.date.bw == 12 &&
+                      a.supp.bw == 16 && a.part.bw == 20 && a.rev.bw == 24;
     if (fast) return launch_compact<12, 16, 20, 24>(a, in_lds, lds, s);
-    return launch_compact<-1, -1, -1, -1>(a, in_lds, lds, s);
+    if (in_lds) {
+        auto k = q21_any_kernel<true>;
+        FLS_HIP_CHECK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        return launch_resident_q21(k, 1024, lds, a, s);
+    }
+    return launch_resident_q21(q21_any_kernel<false>, 1024, 0, a, s);
 }
 
 }  // extern "C"

@@ -86,6 +86,8 @@ struct StarDev {
 };
 
 constexpr int kStarBlock = 1024;
+constexpr int kB31 = 16;  // rows per batch of dimension lookups
+constexpr int kB41 = 8;
 constexpr int kStarVecs = kStarBlock / 32;
 
 // per-block LDS: group table, then (USE_LDS) the staged dimension codes
@@ -141,17 +143,70 @@ __device__ __forceinline__ uint8_t code_at(const uint8_t* __restrict code, uint3
     return k < n ? code[k] : kNone;
 }
 
+// Table entries for rows [r0, r0 + B) of a lane: table[key - kmin] for the
+// rows live in m whose key is in range, `none` for the others.  Every load is
+// issued (a dead row reads entry 0) before any result is used, so the B
+// lookups wait for one memory round trip instead of B dependent ones.
+template <int B, typename T, typename Col>
+__device__ __forceinline__ void lookup_batch(const Col& col, const T* __restrict table, uint32_t kmin, uint32_t n,
+                                             uint32_t m, int r0, uint32_t none, uint32_t (&out)[B]) {
+    uint32_t idx[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+        const uint32_t k = col.get(r0 + j) - kmin;
+        idx[j] = (((m >> (r0 + j)) & 1u) && k < n) ? k : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+        const uint32_t x = table[idx[j] != 0xFFFFFFFFu ? idx[j] : 0u];
+        out[j] = idx[j] != 0xFFFFFFFFu ? x : none;
+    }
+}
+
+template <int B>
+constexpr uint32_t batch_mask(int r0) {
+    return (B >= 32 ? 0xFFFFFFFFu : ((1u << B) - 1u)) << r0;
+}
+
 // One byte per row r of a lane, four rows per register (r compile-time).
 __device__ __forceinline__ void put8(uint32_t (&p)[8], int r, uint32_t v) { p[r >> 2] |= v << (8 * (r & 3)); }
 __device__ __forceinline__ uint32_t get8(const uint32_t (&p)[8], int r) { return (p[r >> 2] >> (8 * (r & 3))) & 0xFFu; }
 
-// Q3.1 probe: supplier nation (LDS), customer nation (L2), year offset (LDS).
-// Fixed widths issue all four columns' loads before the first probe (bytes in
-// flight); the runtime-width path (32 registers per unpacked column) loads
-// each column where it is first used, so at most two are held at once.
+// A lane's 32 decoded rows (any encoding, decode_regs) seen as a column.
+struct RegCol {
+    uint32_t x[32];
+    __device__ __forceinline__ void load(const ColDev& c, uint64_t v) { decode_regs(c, v, x); }
+    __device__ __forceinline__ uint32_t get(int r) const { return x[r]; }
+};
+
+// Batched code lookups of the rows live in m: each row's code is kept in
+// `codes`, and rows whose code is kNone (filtered, or key out of range) leave m.
+template <int B, typename Col>
+__device__ __forceinline__ void filter_codes(const Col& col, const uint8_t* __restrict table, uint32_t kmin,
+                                             uint32_t n, uint32_t& m, uint32_t (&codes)[8]) {
+#pragma unroll
+    for (int rb = 0; rb < 32; rb += B) {
+        if (!__any((m & batch_mask<B>(rb)) != 0)) continue;
+        uint32_t cv[B];
+        lookup_batch<B>(col, table, kmin, n, m, rb, kNone, cv);
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            put8(codes, rb + j, cv[j]);
+            if (cv[j] == kNone) m &= ~(1u << (rb + j));
+        }
+    }
+}
+
+__device__ __forceinline__ void group_add_lds(const StarLds& t, uint32_t slot, uint64_t value) {
+    atomicAdd(&t.sum[slot], (unsigned long long)value);
+    atomicAdd(&t.cnt[slot], 1u);
+}
+
+// Q3.1 probe at fixed widths: all four columns' loads are issued before the
+// first probe (bytes in flight); supplier nation (LDS) -> customer nation
+// (L2) -> year offset (LDS) -> LDS group table.
 template <int DBW, int CBW, int SBW, int RBW, bool USE_LDS>
 __global__ __launch_bounds__(kStarBlock) void q31_kernel(const StarDev a) {
-    constexpr bool kEarly = DBW >= 0 && CBW >= 0 && SBW >= 0 && RBW >= 0;
     extern __shared__ __align__(16) uint8_t lds[];
     const StarLds t = star_stage<USE_LDS, false>(a, lds);
     const uint32_t tail = live_row_mask((uint32_t)(a.n_rows - (a.n_vec - 1) * 1024u), a.rev.transposed != 0);
@@ -164,11 +219,9 @@ __global__ __launch_bounds__(kStarBlock) void q31_kernel(const StarDev a) {
         LaneCol<DBW> dc;
         LaneCol<RBW> rc;
         sc.load(a.supp, v);
-        if constexpr (kEarly) {
-            cc.load(a.cust, v);
-            dc.load(a.date, v);
-            rc.load(a.rev, v);
-        }
+        cc.load(a.cust, v);
+        dc.load(a.date, v);
+        rc.load(a.rev, v);
         uint32_t m = 0, s8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
@@ -177,50 +230,69 @@ __global__ __launch_bounds__(kStarBlock) void q31_kernel(const StarDev a) {
             m |= (uint32_t)(s != kNone) << r;
         }
         if (v == a.n_vec - 1) m &= tail;
-        if constexpr (kEarly) {
-            cc.pin();
-            dc.pin();
-            rc.pin();
-        }
+        cc.pin();
+        dc.pin();
+        rc.pin();
         if (!__any(m != 0)) continue;
-        if constexpr (!kEarly) cc.load(a.cust, v);
         uint32_t c8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-        for (int r = 0; r < 32; ++r) {
-            if (__any((m >> r) & 1u)) {
-                if ((m >> r) & 1u) {
-                    const uint32_t c = code_at(a.c_code, cc.get(r), km, a.n_cust);
-                    put8(c8, r, c);
-                    if (c == kNone) m &= ~(1u << r);
-                }
-            }
-        }
+        filter_codes<kB31>(cc, a.c_code, km, a.n_cust, m, c8);
         if (!__any(m != 0)) continue;
-        if constexpr (!kEarly) {
-            dc.load(a.date, v);
-            rc.load(a.rev, v);
-        }
 #pragma unroll
-        for (int r = 0; r < 32; ++r) {
-            if ((m >> r) & 1u) {
-                const uint32_t y = code_at(t.y_code, dc.get(r), km, a.n_date);
-                if (y != kNone) {
-                    const uint32_t slot = (get8(s8, r) * a.slot_mul_s + get8(c8, r) * a.slot_mul_c + y) % a.n_slots;
-                    atomicAdd(&t.sum[slot], (unsigned long long)rc.get(r));
-                    atomicAdd(&t.cnt[slot], 1u);
-                }
+        for (int rb = 0; rb < 32; rb += kB31) {
+            if (!__any((m & batch_mask<kB31>(rb)) != 0)) continue;
+            uint32_t yv[kB31];
+            lookup_batch<kB31>(dc, t.y_code, km, a.n_date, m, rb, kNone, yv);
+#pragma unroll
+            for (int j = 0; j < kB31; ++j) {
+                const int r = rb + j;
+                if (yv[j] != kNone)
+                    group_add_lds(t, (get8(s8, r) * a.slot_mul_s + get8(c8, r) * a.slot_mul_c + yv[j]) % a.n_slots,
+                                  rc.get(r));
             }
         }
     }
     star_flush(a, t);
 }
 
-// Q4.1 probe: supplier and part pass (LDS), customer nation (L2), year offset
-// (LDS), revenue - supplycost.  Fixed widths load the last four columns while
-// the first two are probed; the runtime-width path loads two at a time.
+// Q3.1 probe over columns of any encoding and width: one column decoded at a
+// time (32 registers), only the per-row codes carried between them.
+template <bool USE_LDS>
+__global__ __launch_bounds__(kStarBlock) void q31_any_kernel(const StarDev a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const StarLds t = star_stage<USE_LDS, false>(a, lds);
+    const uint32_t tail = live_row_mask((uint32_t)(a.n_rows - (a.n_vec - 1) * 1024u), a.rev.transposed != 0);
+    const uint32_t km = a.keys_min;
+    for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
+        const uint64_t v = g * kStarVecs + (threadIdx.x >> 5);
+        if (v >= a.n_vec) continue;
+        RegCol col;
+        uint32_t m = (v == a.n_vec - 1) ? tail : 0xFFFFFFFFu;
+        uint32_t s8[8] = {0, 0, 0, 0, 0, 0, 0, 0}, c8[8] = {0, 0, 0, 0, 0, 0, 0, 0}, y8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        col.load(a.supp, v);
+        filter_codes<kB31>(col, t.s_code, km, a.n_supp, m, s8);
+        if (!__any(m != 0)) continue;
+        col.load(a.cust, v);
+        filter_codes<kB31>(col, a.c_code, km, a.n_cust, m, c8);
+        if (!__any(m != 0)) continue;
+        col.load(a.date, v);
+        filter_codes<kB31>(col, t.y_code, km, a.n_date, m, y8);
+        if (!__any(m != 0)) continue;
+        col.load(a.rev, v);
+#pragma unroll
+        for (int r = 0; r < 32; ++r)
+            if ((m >> r) & 1u)
+                group_add_lds(
+                    t, (get8(s8, r) * a.slot_mul_s + get8(c8, r) * a.slot_mul_c + get8(y8, r)) % a.n_slots,
+                    col.get(r));
+    }
+    star_flush(a, t);
+}
+
+// Q4.1 probe at fixed widths: supplier and part pass (LDS) while the last four
+// columns load, then customer nation (L2) and year offset (LDS) in batches,
+// revenue - supplycost into the LDS group table.
 template <int DBW, int CBW, int SBW, int PBW, int RBW, int KBW, bool USE_LDS>
 __global__ __launch_bounds__(kStarBlock) void q41_kernel(const StarDev a) {
-    constexpr bool kEarly = DBW >= 0 && CBW >= 0 && SBW >= 0 && PBW >= 0 && RBW >= 0 && KBW >= 0;
     extern __shared__ __align__(16) uint8_t lds[];
     const StarLds t = star_stage<USE_LDS, true>(a, lds);
     const uint32_t tail = live_row_mask((uint32_t)(a.n_rows - (a.n_vec - 1) * 1024u), a.rev.transposed != 0);
@@ -247,57 +319,80 @@ __global__ __launch_bounds__(kStarBlock) void q41_kernel(const StarDev a) {
         LaneCol<DBW> dc;
         LaneCol<RBW> rc;
         LaneCol<KBW> kc;
-        if constexpr (kEarly) {
-            cc.load(a.cust, v);
-            dc.load(a.date, v);
-            rc.load(a.rev, v);
-            kc.load(a.cost, v);
-            cc.pin();
-            dc.pin();
-            rc.pin();
-            kc.pin();
-            if (!__any(m != 0)) continue;
+        cc.load(a.cust, v);
+        dc.load(a.date, v);
+        rc.load(a.rev, v);
+        kc.load(a.cost, v);
+        cc.pin();
+        dc.pin();
+        rc.pin();
+        kc.pin();
+        if (!__any(m != 0)) continue;
 #pragma unroll
-            for (int r = 0; r < 32; ++r) {
-                if ((m >> r) & 1u) {
-                    const uint32_t cn = code_at(a.c_code, cc.get(r), km, a.n_cust);
-                    const uint32_t y = code_at(t.y_code, dc.get(r), km, a.n_date);
-                    if (cn != kNone && y != kNone) {
-                        const uint32_t slot = (cn * a.slot_mul_c + y) % a.n_slots;
-                        const int64_t profit = (int64_t)rc.get(r) - (int64_t)kc.get(r);
-                        atomicAdd(&t.sum[slot], (unsigned long long)profit);
-                        atomicAdd(&t.cnt[slot], 1u);
-                    }
-                }
-            }
-        } else {
-            if (!__any(m != 0)) continue;
-            cc.load(a.cust, v);
-            dc.load(a.date, v);
-            uint32_t c8[8] = {0, 0, 0, 0, 0, 0, 0, 0}, y8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int rb = 0; rb < 32; rb += kB41) {
+            if (!__any((m & batch_mask<kB41>(rb)) != 0)) continue;
+            uint32_t cv[kB41], yv[kB41];
+            lookup_batch<kB41>(cc, a.c_code, km, a.n_cust, m, rb, kNone, cv);
+            lookup_batch<kB41>(dc, t.y_code, km, a.n_date, m, rb, kNone, yv);
 #pragma unroll
-            for (int r = 0; r < 32; ++r) {
-                if ((m >> r) & 1u) {
-                    const uint32_t cn = code_at(a.c_code, cc.get(r), km, a.n_cust);
-                    const uint32_t y = code_at(t.y_code, dc.get(r), km, a.n_date);
-                    put8(c8, r, cn);
-                    put8(y8, r, y);
-                    if (cn == kNone || y == kNone) m &= ~(1u << r);
-                }
-            }
-            if (!__any(m != 0)) continue;
-            rc.load(a.rev, v);
-            kc.load(a.cost, v);
-#pragma unroll
-            for (int r = 0; r < 32; ++r) {
-                if ((m >> r) & 1u) {
-                    const uint32_t slot = (get8(c8, r) * a.slot_mul_c + get8(y8, r)) % a.n_slots;
-                    const int64_t profit = (int64_t)rc.get(r) - (int64_t)kc.get(r);
-                    atomicAdd(&t.sum[slot], (unsigned long long)profit);
-                    atomicAdd(&t.cnt[slot], 1u);
-                }
+            for (int j = 0; j < kB41; ++j) {
+                const int r = rb + j;
+                if (cv[j] != kNone && yv[j] != kNone)
+                    group_add_lds(t, (cv[j] * a.slot_mul_c + yv[j]) % a.n_slots,
+                                  (uint64_t)((int64_t)rc.get(r) - (int64_t)kc.get(r)));
             }
         }
+    }
+    star_flush(a, t);
+}
+
+// Q4.1 probe over columns of any encoding and width, one column at a time.
+// Revenue and supplycost enter the group sums in two passes (+revenue with the
+// row count, then -supplycost): the 64-bit wrapping sums are the same.
+template <bool USE_LDS>
+__global__ __launch_bounds__(kStarBlock) void q41_any_kernel(const StarDev a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const StarLds t = star_stage<USE_LDS, true>(a, lds);
+    const uint32_t tail = live_row_mask((uint32_t)(a.n_rows - (a.n_vec - 1) * 1024u), a.rev.transposed != 0);
+    const uint32_t km = a.keys_min;
+    for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
+        const uint64_t v = g * kStarVecs + (threadIdx.x >> 5);
+        if (v >= a.n_vec) continue;
+        RegCol col;
+        uint32_t m = (v == a.n_vec - 1) ? tail : 0xFFFFFFFFu;
+        uint32_t s8[8] = {0, 0, 0, 0, 0, 0, 0, 0}, c8[8] = {0, 0, 0, 0, 0, 0, 0, 0}, y8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        col.load(a.supp, v);
+        filter_codes<kB31>(col, t.s_code, km, a.n_supp, m, s8);
+        if (!__any(m != 0)) continue;
+        col.load(a.part, v);
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            const uint32_t p = col.get(r) - km;
+            const uint32_t hit = p < a.n_part ? (t.p_bits[p >> 5] >> (p & 31u)) & 1u : 0u;
+            m &= ~((hit ^ 1u) << r);
+        }
+        if (!__any(m != 0)) continue;
+        col.load(a.cust, v);
+        filter_codes<kB31>(col, a.c_code, km, a.n_cust, m, c8);
+        if (!__any(m != 0)) continue;
+        col.load(a.date, v);
+        filter_codes<kB31>(col, t.y_code, km, a.n_date, m, y8);
+        if (!__any(m != 0)) continue;
+        uint32_t slot16[16];  // slot of each row (< kMaxSlots), two per register
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            slot16[k] = ((get8(c8, 2 * k) * a.slot_mul_c + get8(y8, 2 * k)) % a.n_slots) |
+                        (((get8(c8, 2 * k + 1) * a.slot_mul_c + get8(y8, 2 * k + 1)) % a.n_slots) << 16);
+        col.load(a.rev, v);
+#pragma unroll
+        for (int r = 0; r < 32; ++r)
+            if ((m >> r) & 1u) group_add_lds(t, (slot16[r >> 1] >> (16 * (r & 1))) & 0xFFFFu, col.get(r));
+        col.load(a.cost, v);
+#pragma unroll
+        for (int r = 0; r < 32; ++r)
+            if ((m >> r) & 1u)
+                atomicAdd(&t.sum[(slot16[r >> 1] >> (16 * (r & 1))) & 0xFFFFu],
+                          (unsigned long long)(0ull - (uint64_t)col.get(r)));
     }
     star_flush(a, t);
 }
@@ -313,16 +408,24 @@ int launch_star(Kernel k, size_t lds, const StarDev& a, hipStream_t s) {
     return check_launch();
 }
 
+// Fact columns of any encoding, all in one row order (DELTA, RLE and MIXED
+// columns decode in transposed order, as does BITPACK with transposed set).
 int star_columns(StarDev& a, const fls_column* const* cols, ColDev* const* outs, int n, uint64_t n_rows) {
     for (int i = 0; i < n; ++i) {
         if (!cols[i]) return FLS_ERR_NULL;
-        if (const int st = to_dev_column(*cols[i], n_rows, outs[i], false)) return st;
-        if ((cols[i]->transposed != 0) != (cols[0]->transposed != 0)) return FLS_ERR_ARG;
+        if (const int st = to_dev_column(*cols[i], n_rows, outs[i], true)) return st;
+        if ((outs[i]->transposed != 0) != (outs[0]->transposed != 0)) return FLS_ERR_ARG;
     }
     a.n_rows = n_rows;
     a.n_vec = ceil_div(n_rows, 1024);
     a.n_groups = ceil_div(a.n_vec, kStarVecs);
     return FLS_OK;
+}
+
+bool all_bitpack(ColDev* const* outs, int n) {
+    for (int i = 0; i < n; ++i)
+        if (outs[i]->enc != FLS_ENC_BITPACK) return false;
+    return true;
 }
 
 int star_workspace(StarDev& a, const fls_star_dims* dims, const void* d_ws) {
@@ -371,6 +474,7 @@ int fls_q31_build(const fls_star_dims* dims, const fls_q31_params* params, void*
     FLS_RETURN_IF(!dims || !params || !d_workspace, FLS_ERR_NULL);
     FLS_RETURN_IF(!dims->s_region || !dims->s_nation || !dims->c_region || !dims->c_nation || !dims->d_year,
                   FLS_ERR_NULL);
+    FLS_RETURN_IF(!dims->n_supp || !dims->n_cust || !dims->n_date, FLS_ERR_ARG);
     FLS_RETURN_IF(params->year_hi < params->year_lo || params->year_lo < params->year_base ||
                       params->year_hi - params->year_base >= kNone,
                   FLS_ERR_ARG);
@@ -387,6 +491,7 @@ int fls_query_q31(const fls_column* datekey, const fls_column* custkey, const fl
                   const fls_q31_params* params, int64_t* d_sums, uint64_t* d_counts, uint32_t flags, void* stream) {
     using namespace fls;
     FLS_RETURN_IF(!dims || !d_workspace || !params, FLS_ERR_NULL);
+    FLS_RETURN_IF(!dims->n_supp || !dims->n_cust || !dims->n_date, FLS_ERR_ARG);
     StarDev a{};
     const fls_column* cols[4] = {datekey, custkey, suppkey, revenue};
     ColDev* outs[4] = {&a.date, &a.cust, &a.supp, &a.rev};
@@ -401,12 +506,12 @@ int fls_query_q31(const fls_column* datekey, const fls_column* custkey, const fl
     const size_t staged = table + dims->n_supp + dims->n_date;
     const bool in_lds = staged <= kLdsBudget;
     const size_t lds = in_lds ? staged : table;
-    const bool fast = a.date.bw == 12 && a.cust.bw == 20 && a.supp.bw == 16 && a.rev.bw == 24;
+    const bool fast = all_bitpack(outs, 4) && a.date.bw == 12 && a.cust.bw == 20 && a.supp.bw == 16 &&
+                      a.rev.bw == 24;
     if (fast)
         return in_lds ? launch_star(q31_kernel<12, 20, 16, 24, true>, lds, a, s)
                       : launch_star(q31_kernel<12, 20, 16, 24, false>, lds, a, s);
-    return in_lds ? launch_star(q31_kernel<-1, -1, -1, -1, true>, lds, a, s)
-                  : launch_star(q31_kernel<-1, -1, -1, -1, false>, lds, a, s);
+    return in_lds ? launch_star(q31_any_kernel<true>, lds, a, s) : launch_star(q31_any_kernel<false>, lds, a, s);
 }
 
 int fls_q41_build(const fls_star_dims* dims, const fls_q41_params* params, void* d_workspace, uint64_t ws_bytes,
@@ -415,6 +520,7 @@ int fls_q41_build(const fls_star_dims* dims, const fls_q41_params* params, void*
     FLS_RETURN_IF(!dims || !params || !d_workspace, FLS_ERR_NULL);
     FLS_RETURN_IF(!dims->s_region || !dims->c_region || !dims->c_nation || !dims->p_mfgr || !dims->d_year,
                   FLS_ERR_NULL);
+    FLS_RETURN_IF(!dims->n_supp || !dims->n_cust || !dims->n_date || !dims->n_part, FLS_ERR_ARG);
     const StarLayout L = star_layout(*dims);
     FLS_RETURN_IF(ws_bytes < L.total, FLS_ERR_CAPACITY);
     uint8_t* ws = static_cast<uint8_t*>(d_workspace);
@@ -432,6 +538,7 @@ int fls_query_q41(const fls_column* datekey, const fls_column* custkey, const fl
                   uint64_t* d_counts, uint32_t flags, void* stream) {
     using namespace fls;
     FLS_RETURN_IF(!dims || !d_workspace || !params, FLS_ERR_NULL);
+    FLS_RETURN_IF(!dims->n_supp || !dims->n_cust || !dims->n_date || !dims->n_part, FLS_ERR_ARG);
     StarDev a{};
     const fls_column* cols[6] = {datekey, custkey, suppkey, partkey, revenue, supplycost};
     ColDev* outs[6] = {&a.date, &a.cust, &a.supp, &a.part, &a.rev, &a.cost};
@@ -446,13 +553,12 @@ int fls_query_q41(const fls_column* datekey, const fls_column* custkey, const fl
     const size_t staged = table + 4ull * a.p_words + dims->n_supp + dims->n_date;
     const bool in_lds = staged <= kLdsBudget;
     const size_t lds = in_lds ? staged : table;
-    const bool fast = a.date.bw == 12 && a.cust.bw == 20 && a.supp.bw == 16 && a.part.bw == 20 && a.rev.bw == 24 &&
-                      a.cost.bw == 16;
+    const bool fast = all_bitpack(outs, 6) && a.date.bw == 12 && a.cust.bw == 20 && a.supp.bw == 16 &&
+                      a.part.bw == 20 && a.rev.bw == 24 && a.cost.bw == 16;
     if (fast)
         return in_lds ? launch_star(q41_kernel<12, 20, 16, 20, 24, 16, true>, lds, a, s)
                       : launch_star(q41_kernel<12, 20, 16, 20, 24, 16, false>, lds, a, s);
-    return in_lds ? launch_star(q41_kernel<-1, -1, -1, -1, -1, -1, true>, lds, a, s)
-                  : launch_star(q41_kernel<-1, -1, -1, -1, -1, -1, false>, lds, a, s);
+    return in_lds ? launch_star(q41_any_kernel<true>, lds, a, s) : launch_star(q41_any_kernel<false>, lds, a, s);
 }
 
 }  // extern "C"

@@ -284,9 +284,10 @@ int fls_query_q21(const fls_column* datekey, const fls_column* suppkey, const fl
 
 /* ============================================ star joins (Q3.1, Q4.1)
  * Dense-key dimension columns for the three- and four-way star joins: row i of
- * each dimension holds key keys_min + i.  Nations must be below 255 (rows with
- * a larger nation do not match).  Unused columns may be NULL (p_mfgr for
- * Q3.1; s_nation and p_mfgr's part count for Q4.1 are ignored when unused).  */
+ * each dimension holds key keys_min + i, and each used dimension holds at
+ * least one row (FLS_ERR_ARG otherwise).  Nations must be below 255 (rows with
+ * a larger nation do not match).  Columns a shape does not use may be NULL:
+ * p_mfgr (with n_part 0) for Q3.1, s_nation for Q4.1.                        */
 typedef struct fls_star_dims {
     const uint32_t* s_region;
     const uint32_t* s_nation;

@@ -394,3 +394,37 @@ def pad_vectors(a):
 
 def o_uniform_py(seed, stream, i, rng_range):
     return int(oracle().orc_uniform(seed, stream, i, rng_range))
+
+
+def encode_any(x, kind):
+    """Device column of the uint32 values x in one of the formats the probe
+    kernels decode at run time (all in transposed row order):
+      "delta"      FastLanes delta (GPU encoder)
+      "rle"        FastLanes RLE (GPU encoder)
+      "for"        MIXED with every vector FOR: a width, base and offset per
+                   vector, the layout of the per-vector FOR key columns
+      "exhaustive" MIXED with each vector's smallest encoding
+      "bitpack_t"  BITPACK at the smallest width, transposed"""
+    import fastlanesgpu_amd as fls
+    from fastlanesgpu_amd import selector as sel
+
+    x = np.ascontiguousarray(x, np.uint32)
+    n = x.size
+    d = to_dev(x)
+    if kind == "delta":
+        w, vm, bs = fls.delta_encode_device(d, n)
+        return fls.Column(fls.ENC_DELTA, n, w, vmeta=vm, bases=bs)
+    if kind == "rle":
+        w, vm, rm, bs = fls.rle_encode_device(d, n)
+        return fls.Column(fls.ENC_RLE, n, w, vmeta=vm, bases=bs, aux=rm)
+    if kind in ("for", "exhaustive"):
+        ws = sel.MixedWorkspace(n)
+        if kind == "for":
+            ws.cls.fill_(sel.MIX_FOR * 10)
+            sel.mixed_encode_classes(d, n, ws)
+        else:
+            sel.mixed_encode_exhaustive(d, n, ws)
+        return ws.column(n)
+    if kind == "bitpack_t":
+        return fls.bitpack_column(x, transposed=True).to_device()
+    raise ValueError(kind)

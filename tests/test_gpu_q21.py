@@ -72,6 +72,26 @@ def test_q21_runtime_widths():
     check(out_p, want)
 
 
+@pytest.mark.parametrize("layout", [("delta", "for", "rle", "exhaustive"), ("for", "rle", "for", "delta")])
+def test_q21_any_encoding_columns(layout):
+    # the compact probe decodes delta, RLE and mixed (per-vector FOR) columns
+    # at run time; rows sorted by date give the delta and RLE columns runs
+    n = 50_000 + 3
+    fact, dims = T.o_gen_c3(13, n, **SMALL)
+    fact[1] %= SMALL["n_supp"]
+    fact[2] %= SMALL["n_part"]
+    order = np.argsort(fact[0], kind="stable")
+    fact = [x[order] for x in fact]
+    vals = [fact[0] + 1, fact[1] + 1, fact[2] + 1, fact[3]]
+    cols = [T.encode_any(x, kind) for x, kind in zip(vals, layout)]
+    d = dev_dims(dims, SMALL)
+    out = fls.query_q21(*cols, n, d, fls.q21_build(d)).cpu().numpy()
+    check(out, T.o_q21(fact, dims, n))
+    with pytest.raises(fls.FlsError):  # the reference-table probe reads BITPACK only
+        pht, keep = fls.q21_build_pht(d)
+        fls.query_q21_pht(*cols, n, pht)
+
+
 def test_q21_full_size_and_result_rows():
     n = 120_000_000
     fd = fls.gen_c3_fact(1, n)

@@ -121,6 +121,34 @@ def test_dimensions_beyond_lds(sizes, widths):
     check(run_q41(f, d, sizes, n, enc), T.o_q41(f, d, n))
 
 
+ANY_LAYOUTS = [
+    # (date, cust, supp, part, revenue, supplycost); keys are encoded 1-based
+    ("delta", "for", "rle", "exhaustive", "bitpack_t", "for"),
+    ("for", "exhaustive", "bitpack_t", "for", "rle", "delta"),
+]
+
+
+@pytest.mark.parametrize("layout", ANY_LAYOUTS)
+@pytest.mark.parametrize("sort_by_date", [False, True])
+def test_any_encoding_columns(layout, sort_by_date):
+    # per-vector FOR keys (fls_q41_bitpacked_opt_v4.cu:29-120 reads custkey
+    # and orderdate with a width, base and offset per vector), delta, RLE and
+    # mixed columns take the run-time-encoding kernels; sorting the fact rows
+    # by date gives the delta and RLE columns long runs
+    n = 60_000 + 77
+    f, d = small_table(36, n)
+    if sort_by_date:
+        order = np.argsort(f["datefk"], kind="stable")
+        f = {k: v[order] for k, v in f.items()}
+    vals = [f["datefk"] + 1, f["custfk"] + 1, f["suppfk"] + 1, f["partfk"] + 1, f["revenue"], f["supplycost"]]
+    cols = [T.encode_any(x, kind) for x, kind in zip(vals, layout)]
+    dd = dev_dims(d, SMALL)
+    out = fls.query_q31(cols[0], cols[1], cols[2], cols[4], n, dd, fls.q31_build(dd)).cpu().numpy()
+    check(out, T.o_q31(f, d, n))
+    out = fls.query_q41(*cols, n, dd, fls.q41_build(dd)).cpu().numpy()
+    check(out, T.o_q41(f, d, n))
+
+
 def test_accumulate_and_errors():
     torch = T.torch_cuda()
     n = 10_000
@@ -137,6 +165,8 @@ def test_accumulate_and_errors():
     with pytest.raises(fls.FlsError):  # one column in another row order
         bad = fls.bitpack_column(f["revenue"], 24, transposed=True).to_device()
         fls.query_q41(cols[0], cols[1], cols[2], cols[3], bad, cols[5], n, d, ws)
+    with pytest.raises(fls.FlsError):  # an empty dimension
+        fls.query_q41(*cols, n, dict(d, n_cust=0), ws)
     with pytest.raises(fls.FlsError):  # workspace too small
         small = torch.empty(16, dtype=torch.uint8, device="cuda")
         ds = fls.star_dims_struct(d)
