@@ -184,6 +184,16 @@ struct LaneCol {
     __device__ __forceinline__ void pin() { pin_words(w); }
 };
 
+// Bit key - kmin of a bitmap of n >= 1 bits, 0 for keys out of range.  The
+// load reads word 0 for those and masks the result, so it is unconditional and
+// a lane's 32 rows' loads issue back to back (a load under a range test
+// compiles to a branch and a wait per row).
+__device__ __forceinline__ uint32_t bit_at(const uint32_t* __restrict bits, uint32_t key, uint32_t kmin, uint32_t n) {
+    const uint32_t k = key - kmin;
+    const uint32_t kk = k < n ? k : 0u;
+    return (bits[kk >> 5] >> (kk & 31u)) & (k < n ? 1u : 0u);
+}
+
 // Row mask (bit r) of the rows of this lane that exist in a vector holding
 // `live` <= 1024 rows, in normal or transposed row order.  Closed forms (no
 // per-row index table, which the compiler would hoist into 32 registers):

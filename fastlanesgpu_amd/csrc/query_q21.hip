@@ -24,6 +24,7 @@
 #include "fls/device/unpack.hpp"
 #include "fls/device/wave.hpp"
 #include "fls_columns.hpp"
+#include "fls_fastdiv.hpp"
 #include "fls_runtime.hpp"
 
 namespace fls {
@@ -113,12 +114,13 @@ struct Q21Dev {
     uint32_t supp_words, part_words;
     // group-by
     uint32_t year_base, n_years, n_slots;
+    FastDiv slot_div;  // n_slots
     int64_t* sums;
     uint64_t* counts;
 };
 
 __device__ __forceinline__ void group_add(const Q21Dev& a, uint32_t brand, uint32_t year, uint32_t revenue) {
-    const uint32_t slot = (brand * a.n_years + (year - a.year_base)) % a.n_slots;
+    const uint32_t slot = fast_mod(brand * a.n_years + (year - a.year_base), a.slot_div);
     atomicAdd((unsigned long long*)&a.sums[slot], (unsigned long long)revenue);
     atomicAdd((unsigned long long*)&a.counts[slot], 1ull);
 }
@@ -183,38 +185,51 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
         dc.load(a.date, v);
         rc.load(a.rev, v);
         // supplier semi-join on every row
+        const uint32_t km = a.keys_min;
         uint32_t m = 0;
 #pragma unroll
-        for (int r = 0; r < 32; ++r) {
-            const uint32_t s = sc.get(r) - a.keys_min;
-            const uint32_t hit = s < a.n_supp ? (supp_bm[s >> 5] >> (s & 31u)) & 1u : 0u;
-            m |= hit << r;
-        }
+        for (int r = 0; r < 32; ++r) m |= bit_at(supp_bm, sc.get(r), km, a.n_supp) << r;
         if (v == a.n_vec - 1) m &= tail;
         pc.pin();
         dc.pin();
         rc.pin();
         if (!__any(m != 0)) continue;
-        // part semi-join on the surviving rows (rows no lane of the wave
-        // still holds are skipped as a whole)
+        // part semi-join: every row from the LDS bitmap; from L2 only the rows
+        // that passed the supplier (rows no lane of the wave holds skipped)
+        if constexpr (USE_LDS) {
 #pragma unroll
-        for (int r = 0; r < 32; ++r) {
-            if (__any((m >> r) & 1u)) {
-                const uint32_t p = pc.get(r) - a.keys_min;
-                const bool live = (m >> r) & 1u;
-                const uint32_t hit = (live && p < a.n_part) ? (part_bm[p >> 5] >> (p & 31u)) & 1u : 0u;
-                if (!hit) m &= ~(1u << r);
+            for (int r = 0; r < 32; ++r) m &= ~((bit_at(part_bm, pc.get(r), km, a.n_part) ^ 1u) << r);
+        } else {
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                if (__any((m >> r) & 1u)) {
+                    const uint32_t p = pc.get(r) - km;
+                    const bool live = (m >> r) & 1u;
+                    const uint32_t hit = (live && p < a.n_part) ? (part_bm[p >> 5] >> (p & 31u)) & 1u : 0u;
+                    if (!hit) m &= ~(1u << r);
+                }
             }
         }
         if (!__any(m != 0)) continue;
-        // group-by on the rows that passed both
+        // group-by on the rows that passed both: brand (L2) and year lookups
+        // of 8 rows issued together, then the global atomics
 #pragma unroll
-        for (int r = 0; r < 32; ++r) {
-            if ((m >> r) & 1u) {
-                const uint32_t p = pc.get(r) - a.keys_min;
-                const uint32_t d = dc.get(r) - a.keys_min;
-                if (d < a.n_date) group_add(a, a.brand[p], year_tab[d], rc.get(r));
+        for (int rb = 0; rb < 32; rb += 8) {
+            if (!__any(((m >> rb) & 0xFFu) != 0)) continue;
+            uint32_t bv[8], yv[8], ok = 0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int r = rb + j;
+                const bool live = (m >> r) & 1u;  // passed the part bitmap: key in range
+                const uint32_t p = pc.get(r) - km, d = dc.get(r) - km;
+                const bool in = live && d < a.n_date;
+                ok |= (uint32_t)in << j;
+                bv[j] = a.brand[live ? p : 0u];
+                yv[j] = year_tab[in ? d : 0u];
             }
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if ((ok >> j) & 1u) group_add(a, bv[j], yv[j], rc.get(rb + j));
         }
     }
 }
@@ -254,18 +269,14 @@ __global__ __launch_bounds__(1024) void q21_any_kernel(const Q21Dev a) {
         decode_regs(a.supp, v, x);
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
-            const uint32_t k = x[r] - km;
-            const uint32_t hit = k < a.n_supp ? (supp_bm[k >> 5] >> (k & 31u)) & 1u : 0u;
-            m &= ~((hit ^ 1u) << r);
+            m &= ~((bit_at(supp_bm, x[r], km, a.n_supp) ^ 1u) << r);
         }
         if (!__any(m != 0)) continue;
         decode_regs(a.part, v, x);
         uint32_t b16[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
-            const uint32_t k = x[r] - km;
-            const uint32_t hit = k < a.n_part ? (part_bm[k >> 5] >> (k & 31u)) & 1u : 0u;
-            m &= ~((hit ^ 1u) << r);
+            m &= ~((bit_at(part_bm, x[r], km, a.n_part) ^ 1u) << r);
         }
         if (!__any(m != 0)) continue;
 #pragma unroll
@@ -340,6 +351,7 @@ int fill_groups(Q21Dev& a, const fls_q21_params* p, int64_t* d_sums, uint64_t* d
     a.year_base = p->year_base;
     a.n_years = p->n_years;
     a.n_slots = p->n_slots;
+    a.slot_div = make_fastdiv(p->n_slots);
     a.sums = d_sums;
     a.counts = d_counts;
     if (!(flags & FLS_Q_ACCUMULATE)) {
@@ -403,6 +415,7 @@ int fls_q21_build(const fls_q21_dims* dims, const fls_q21_params* params, void* 
     using namespace fls;
     FLS_RETURN_IF(!dims || !params || !d_workspace, FLS_ERR_NULL);
     FLS_RETURN_IF(!dims->s_region || !dims->p_category || !dims->p_brand || !dims->d_year, FLS_ERR_NULL);
+    FLS_RETURN_IF(!dims->n_supp || !dims->n_part || !dims->n_date, FLS_ERR_ARG);
     const Q21Layout L = q21_layout(dims->n_supp, dims->n_part, dims->n_date);
     FLS_RETURN_IF(ws_bytes < L.total, FLS_ERR_CAPACITY);
     char* ws = static_cast<char*>(d_workspace);
@@ -422,6 +435,7 @@ int fls_query_q21(const fls_column* datekey, const fls_column* suppkey, const fl
                   const fls_q21_params* params, int64_t* d_sums, uint64_t* d_counts, uint32_t flags, void* stream) {
     using namespace fls;
     FLS_RETURN_IF(!dims || !d_workspace, FLS_ERR_NULL);
+    FLS_RETURN_IF(!dims->n_supp || !dims->n_part || !dims->n_date, FLS_ERR_ARG);
     Q21Dev a{};
     int st = fill_fact(a, datekey, suppkey, partkey, revenue, n_rows, true);
     if (st) return st;

@@ -11,6 +11,7 @@
 #include "fls/device/unpack.hpp"
 #include "fls/device/wave.hpp"
 #include "fls_columns.hpp"
+#include "fls_fastdiv.hpp"
 #include "fls_runtime.hpp"
 
 namespace fls {
@@ -81,6 +82,7 @@ struct StarDev {
     const uint32_t* p_bits;
     uint32_t n_supp, n_cust, n_part, n_date, keys_min, p_words;
     uint32_t slot_mul_s, slot_mul_c, n_slots;  // slot = (s*slot_mul_s + c*slot_mul_c + y) % n_slots
+    FastDiv slot_div;                          // n_slots
     int64_t* sums;
     uint64_t* counts;
 };
@@ -138,9 +140,12 @@ __device__ __forceinline__ void star_flush(const StarDev& a, const StarLds& t) {
         }
 }
 
-__device__ __forceinline__ uint8_t code_at(const uint8_t* __restrict code, uint32_t key, uint32_t kmin, uint32_t n) {
+// Code of a key (kNone out of range), branch-free like bit_at.  Every
+// dimension holds at least one row.
+__device__ __forceinline__ uint32_t code_at(const uint8_t* __restrict code, uint32_t key, uint32_t kmin, uint32_t n) {
     const uint32_t k = key - kmin;
-    return k < n ? code[k] : kNone;
+    const uint32_t x = code[k < n ? k : 0u];
+    return k < n ? x : kNone;
 }
 
 // Table entries for rows [r0, r0 + B) of a lane: table[key - kmin] for the
@@ -246,7 +251,7 @@ __global__ __launch_bounds__(kStarBlock) void q31_kernel(const StarDev a) {
             for (int j = 0; j < kB31; ++j) {
                 const int r = rb + j;
                 if (yv[j] != kNone)
-                    group_add_lds(t, (get8(s8, r) * a.slot_mul_s + get8(c8, r) * a.slot_mul_c + yv[j]) % a.n_slots,
+                    group_add_lds(t, fast_mod(get8(s8, r) * a.slot_mul_s + get8(c8, r) * a.slot_mul_c + yv[j], a.slot_div),
                                   rc.get(r));
             }
         }
@@ -282,7 +287,7 @@ __global__ __launch_bounds__(kStarBlock) void q31_any_kernel(const StarDev a) {
         for (int r = 0; r < 32; ++r)
             if ((m >> r) & 1u)
                 group_add_lds(
-                    t, (get8(s8, r) * a.slot_mul_s + get8(c8, r) * a.slot_mul_c + get8(y8, r)) % a.n_slots,
+                    t, fast_mod(get8(s8, r) * a.slot_mul_s + get8(c8, r) * a.slot_mul_c + get8(y8, r), a.slot_div),
                     col.get(r));
     }
     star_flush(a, t);
@@ -308,9 +313,8 @@ __global__ __launch_bounds__(kStarBlock) void q41_kernel(const StarDev a) {
             pc.load(a.part, v);
 #pragma unroll
             for (int r = 0; r < 32; ++r) {
-                const uint32_t p = pc.get(r) - km;
                 const uint32_t hit = (code_at(t.s_code, sc.get(r), km, a.n_supp) != kNone) &
-                                     (p < a.n_part ? (t.p_bits[p >> 5] >> (p & 31u)) & 1u : 0u);
+                                     bit_at(t.p_bits, pc.get(r), km, a.n_part);
                 m |= hit << r;
             }
         }
@@ -338,7 +342,7 @@ __global__ __launch_bounds__(kStarBlock) void q41_kernel(const StarDev a) {
             for (int j = 0; j < kB41; ++j) {
                 const int r = rb + j;
                 if (cv[j] != kNone && yv[j] != kNone)
-                    group_add_lds(t, (cv[j] * a.slot_mul_c + yv[j]) % a.n_slots,
+                    group_add_lds(t, fast_mod(cv[j] * a.slot_mul_c + yv[j], a.slot_div),
                                   (uint64_t)((int64_t)rc.get(r) - (int64_t)kc.get(r)));
             }
         }
@@ -367,9 +371,7 @@ __global__ __launch_bounds__(kStarBlock) void q41_any_kernel(const StarDev a) {
         col.load(a.part, v);
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
-            const uint32_t p = col.get(r) - km;
-            const uint32_t hit = p < a.n_part ? (t.p_bits[p >> 5] >> (p & 31u)) & 1u : 0u;
-            m &= ~((hit ^ 1u) << r);
+            m &= ~((bit_at(t.p_bits, col.get(r), km, a.n_part) ^ 1u) << r);
         }
         if (!__any(m != 0)) continue;
         col.load(a.cust, v);
@@ -381,8 +383,8 @@ __global__ __launch_bounds__(kStarBlock) void q41_any_kernel(const StarDev a) {
         uint32_t slot16[16];  // slot of each row (< kMaxSlots), two per register
 #pragma unroll
         for (int k = 0; k < 16; ++k)
-            slot16[k] = ((get8(c8, 2 * k) * a.slot_mul_c + get8(y8, 2 * k)) % a.n_slots) |
-                        (((get8(c8, 2 * k + 1) * a.slot_mul_c + get8(y8, 2 * k + 1)) % a.n_slots) << 16);
+            slot16[k] = fast_mod(get8(c8, 2 * k) * a.slot_mul_c + get8(y8, 2 * k), a.slot_div) |
+                        (fast_mod(get8(c8, 2 * k + 1) * a.slot_mul_c + get8(y8, 2 * k + 1), a.slot_div) << 16);
         col.load(a.rev, v);
 #pragma unroll
         for (int r = 0; r < 32; ++r)
@@ -448,6 +450,7 @@ int star_outputs(StarDev& a, uint32_t n_slots, int64_t* d_sums, uint64_t* d_coun
     if (!d_sums || !d_counts) return FLS_ERR_NULL;
     if (n_slots == 0 || n_slots > kMaxSlots) return FLS_ERR_ARG;
     a.n_slots = n_slots;
+    a.slot_div = make_fastdiv(n_slots);
     a.sums = d_sums;
     a.counts = d_counts;
     if (!(flags & FLS_Q_ACCUMULATE)) {

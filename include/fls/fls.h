@@ -258,7 +258,8 @@ int fls_query_q21_pht(const fls_column* datekey, const fls_column* suppkey, cons
                       const fls_column* revenue, uint64_t n_rows, const fls_q21_pht* ht, const fls_q21_params* params,
                       int64_t* d_sums, uint64_t* d_counts, uint32_t flags, void* stream);
 
-/* Dense-key dimension columns: row i holds key keys_min + i.                 */
+/* Dense-key dimension columns: row i holds key keys_min + i; each holds at
+ * least one row (FLS_ERR_ARG otherwise).                                      */
 typedef struct fls_q21_dims {
     const uint32_t* s_region;
     uint32_t n_supp;
