@@ -251,6 +251,38 @@ def bench_star(steps, warmup, seed):
     return res
 
 
+def bench_materialize(steps, warmup, seed):
+    """Decode-to-materialize: 120M-row delta orderdate to plain values in row
+    order (untransposed through LDS) and in stored order; transpose_o alone."""
+    import torch
+
+    import fastlanesgpu_amd as fls
+
+    n = 120_000_000
+    od = fls.gen_c2(seed, n)[0]
+    w, vm, bs = fls.delta_encode_device(od, n)
+    col = fls.Column(fls.ENC_DELTA, n, w, vmeta=vm, bases=bs)
+    enc_bytes = col.nbytes()
+    n_vec = (n + 1023) // 1024
+    res = {"workload": "120M-row orderdate column (sorted dates), FastLanes delta; output 4 B per row",
+           "encoded_bytes": enc_bytes}
+    out = torch.empty(n_vec * 1024, dtype=torch.int32, device="cuda")
+    for name, order in (("rows", fls.ORDER_ROWS), ("stored", fls.ORDER_STORED)):
+        _, per = time_kernel(lambda: fls.decode_column(col, n, order, out=out), out, steps, warmup)
+        ms = statistics.mean(per)
+        moved = enc_bytes + 4 * (n if order == fls.ORDER_ROWS else n_vec * 1024)
+        res[name] = {"kernel_ms": ms, "rows_per_s": n / (ms * 1e-3), "GBs_read_plus_write": moved / (ms * 1e-3) / 1e9,
+                     "frac_hbm": moved / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    res["rows_equal_generated"] = bool(torch.equal(fls.decode_column(col, n, fls.ORDER_ROWS, out=out), od))
+    src = out.clone()
+    dst = torch.empty_like(out)
+    _, per = time_kernel(lambda: fls.transpose_vectors(src, n_vec, inverse=True, out=dst), dst, steps, warmup)
+    ms = statistics.mean(per)
+    res["transpose_o"] = {"kernel_ms": ms, "GBs_read_plus_write": 8 * n_vec * 1024 / (ms * 1e-3) / 1e9,
+                          "frac_hbm": 8 * n_vec * 1024 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    return res
+
+
 def bench_c4(steps, seed):
     """4096 uneven segments (bitpack/delta/RLE) decoded + range-filtered:
     one graph-replayed persistent launch vs one launch per segment."""
@@ -587,6 +619,8 @@ def main():
         line["secondary"]["c3_probe_groupby"] = bench_c3(max(5, args.steps // 2), 2, args.seed)
         log("[bench] secondary: star joins")
         line["secondary"]["star_joins"] = bench_star(max(5, args.steps // 2), 2, args.seed)
+        log("[bench] secondary: materialize")
+        line["secondary"]["materialize"] = bench_materialize(max(5, args.steps // 2), 2, args.seed)
         log("[bench] secondary: batched segments")
         line["secondary"]["c4_batched"] = bench_c4(max(5, args.steps // 5), args.seed)
         if not args.no_c5:

@@ -550,6 +550,32 @@ def query_q41(date, cust, supp, part, rev, cost, n_rows, dims, ws, params=None, 
     return d_out
 
 
+# ------------------------------------------------ untranspose, materialize
+ORDER_STORED, ORDER_ROWS = 0, 1
+
+
+def transpose_vectors(d_in, n_vec, inverse=False, out=None, stream=None):
+    """transpose_i (or, inverse, transpose_o) of n_vec whole 1024-value vectors
+    on the device; returns the output tensor."""
+    if out is None:
+        out = _dev_u32(max(1, n_vec * VECTOR_SIZE))
+    call("fls_transpose_u32", dptr(d_in), n_vec, 1 if inverse else 0, dptr(out), _stream(stream))
+    return out
+
+
+def decode_column(col, n_rows=None, order=ORDER_ROWS, out=None, stream=None):
+    """Decode a device column of any encoding to plain uint32 values (int32
+    tensor): original row order (n_rows values) or the stored order (whole
+    vectors)."""
+    n_rows = col.n_rows if n_rows is None else n_rows
+    n = n_rows if order == ORDER_ROWS else (n_rows + VECTOR_SIZE - 1) // VECTOR_SIZE * VECTOR_SIZE
+    if out is None:
+        out = _dev_u32(max(1, n))
+    s = col.struct()
+    call("fls_decode_column_u32", ctypes.byref(s), n_rows, order, dptr(out), _stream(stream))
+    return out[:n]
+
+
 # ----------------------------------------------------------- GPU encoders
 def _dev_u32(n):
     import torch

@@ -163,6 +163,11 @@ class Q41Params(ctypes.Structure):
                 ("year_base", c_u32), ("n_years", c_u32), ("n_slots", c_u32)]
 
 
+SIGNATURES.update({
+    "fls_transpose_u32": (c_int, [c_vp, c_u64, c_u32, c_vp, c_vp]),
+    "fls_decode_column_u32": (c_int, [COLP, c_u64, c_u32, c_vp, c_vp]),
+})
+
 _SDP = ctypes.POINTER(StarDims)
 SIGNATURES.update({
     "fls_gen_star_extra": (c_int, [c_u64, c_u64, c_vp, c_u32, c_vp, c_vp]),

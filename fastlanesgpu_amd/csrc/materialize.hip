@@ -1,0 +1,164 @@
+// Untranspose and decode-to-materialize on the GPU.
+//
+// transpose_i / transpose_o (fastlanes/src/transpose.cpp:4110-5135 and the
+// inverse that transpose.cpp:5136-6161 is meant to be) over whole 1024-value
+// vectors: each vector is staged in LDS with coalesced loads and written back
+// through the permutation with coalesced stores.
+//
+// fls_decode_column_u32 turns a compressed column of any encoding into plain
+// values for a downstream operator (the role of Crystal's BlockStore,
+// crystal-opt/store.cuh:57-69, after a decode): in the order the column is
+// stored, or in original row order, which untransposes DELTA / RLE / MIXED /
+// transposed BITPACK columns on the fly so they line up with plain columns.
+#include "fls/device/unpack.hpp"
+#include "fls/device/wave.hpp"
+#include "fls_columns.hpp"
+#include "fls_runtime.hpp"
+
+namespace fls {
+namespace {
+
+constexpr int kTrBlock = 256;                 // 4 waves, one vector each
+constexpr int kTrVecs = kTrBlock / 64;
+constexpr uint32_t kTrPad = 1024u + 16u;      // words per staged vector
+
+// LDS slot of index k: one padding word per 64 keeps the strided gathers of
+// the permutation (stride 64 words) on distinct banks.
+__device__ __forceinline__ uint32_t tr_slot(uint32_t k) { return k + (k >> 6); }
+
+template <bool INVERSE>
+__global__ __launch_bounds__(kTrBlock) void transpose_kernel(const uint32_t* __restrict in, uint64_t n_vec,
+                                                             uint32_t* __restrict out) {
+    __shared__ uint32_t stage[kTrVecs][kTrPad];
+    const uint32_t w = threadIdx.x >> 6, t = threadIdx.x & 63u;
+    uint32_t* st = stage[w];
+    for (uint64_t g = blockIdx.x; g * kTrVecs < n_vec; g += gridDim.x) {
+        const uint64_t v = g * kTrVecs + w;
+        if (v < n_vec) {  // wave-uniform: one vector per wave
+            const uint32_t* src = in + v * 1024u;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) st[tr_slot(t + 64u * k)] = src[t + 64u * k];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (v < n_vec) {
+            uint32_t* dst = out + v * 1024u;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const uint32_t p = t + 64u * k;
+                // transpose_i: position p holds original index untranspose_index(p);
+                // transpose_o: original index p sits at position transpose_index(p)
+                dst[p] = st[tr_slot(INVERSE ? dev::transpose_index(p) : dev::untranspose_index(p))];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+constexpr int kMatBlock = 256;          // 4 waves, two vectors per wave
+constexpr int kMatVecs = kMatBlock / 32;
+constexpr uint32_t kRowPad = 33u;       // words per 32 rows of a staged vector
+
+// One vector per 32 lanes: decode (any encoding) into the lane's 32 rows,
+// then store.  ROWS with a transposed column: row r of lane l is original row
+// untranspose_index(l + 32 r); the vector is staged in LDS by original row
+// (33 words per 32 rows) and read back by rows 32 r + l, so the stores are
+// coalesced.  ROWS writes rows < n_rows only; stored order writes whole
+// vectors.
+template <bool ROWS>
+__global__ __launch_bounds__(kMatBlock) void materialize_kernel(const ColDev c, uint64_t n_rows, uint64_t n_vec,
+                                                                uint32_t* __restrict out) {
+    __shared__ uint32_t stage[ROWS ? kMatVecs : 1][ROWS ? 32u * kRowPad : 1u];
+    const uint32_t l = dev::lane32();
+    const uint32_t slot = threadIdx.x >> 5;
+    for (uint64_t g = blockIdx.x; g * kMatVecs < n_vec; g += gridDim.x) {
+        const uint64_t v = g * kMatVecs + slot;
+        const bool live = v < n_vec;
+        uint32_t x[32];
+        if (live) decode_regs(c, v, x);
+        uint32_t* dst = out + v * 1024u;
+        if constexpr (!ROWS) {
+            if (live) {
+#pragma unroll
+                for (int r = 0; r < 32; ++r) dst[32u * r + l] = x[r];
+            }
+        } else {
+            const uint32_t rows = live ? (uint32_t)min<uint64_t>(1024u, n_rows - v * 1024u) : 0u;
+            if (!c.transposed) {
+#pragma unroll
+                for (int r = 0; r < 32; ++r)
+                    if (32u * r + l < rows) dst[32u * r + l] = x[r];
+            } else {
+                uint32_t* st = stage[slot];
+                if (live) {
+#pragma unroll
+                    for (int r = 0; r < 32; ++r) {
+                        const uint32_t i = dev::untranspose_index(l + 32u * r);
+                        st[i + (i >> 5)] = x[r];
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+                for (int r = 0; r < 32; ++r)
+                    if (32u * r + l < rows) dst[32u * r + l] = st[kRowPad * r + l];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        }
+    }
+}
+
+}  // namespace
+}  // namespace fls
+
+extern "C" {
+
+int fls_transpose_u32(const uint32_t* d_in, uint64_t n_vec, uint32_t inverse, uint32_t* d_out, void* stream) {
+    using namespace fls;
+    if (n_vec == 0) return FLS_OK;
+    FLS_RETURN_IF(!d_in || !d_out, FLS_ERR_NULL);
+    FLS_RETURN_IF(d_in == d_out, FLS_ERR_ARG);
+    uint32_t grid = 0;
+    const uint64_t work = ceil_div(n_vec, kTrVecs);
+    if (inverse) {
+        if (const int st = resident_grid(transpose_kernel<true>, kTrBlock, work, &grid)) return st;
+        hipLaunchKernelGGL(transpose_kernel<true>, dim3(grid), dim3(kTrBlock), 0, as_stream(stream), d_in, n_vec,
+                           d_out);
+    } else {
+        if (const int st = resident_grid(transpose_kernel<false>, kTrBlock, work, &grid)) return st;
+        hipLaunchKernelGGL(transpose_kernel<false>, dim3(grid), dim3(kTrBlock), 0, as_stream(stream), d_in, n_vec,
+                           d_out);
+    }
+    return check_launch();
+}
+
+int fls_decode_column_u32(const fls_column* col, uint64_t n_rows, uint32_t order, uint32_t* d_out, void* stream) {
+    using namespace fls;
+    FLS_RETURN_IF(!col, FLS_ERR_NULL);
+    FLS_RETURN_IF(order != FLS_ORDER_STORED && order != FLS_ORDER_ROWS, FLS_ERR_ARG);
+    ColDev c{};
+    if (const int st = to_dev_column(*col, n_rows, &c, true)) return st;
+    if (n_rows == 0) return FLS_OK;
+    FLS_RETURN_IF(!d_out, FLS_ERR_NULL);
+    const uint64_t n_vec = ceil_div(n_rows, 1024);
+    const uint64_t work = ceil_div(n_vec, kMatVecs);
+    uint32_t grid = 0;
+    if (order == FLS_ORDER_ROWS) {
+        if (const int st = resident_grid(materialize_kernel<true>, kMatBlock, work, &grid)) return st;
+        hipLaunchKernelGGL(materialize_kernel<true>, dim3(grid), dim3(kMatBlock), 0, as_stream(stream), c, n_rows,
+                           n_vec, d_out);
+    } else {
+        if (const int st = resident_grid(materialize_kernel<false>, kMatBlock, work, &grid)) return st;
+        hipLaunchKernelGGL(materialize_kernel<false>, dim3(grid), dim3(kMatBlock), 0, as_stream(stream), c, n_rows,
+                           n_vec, d_out);
+    }
+    return check_launch();
+}
+
+}  // extern "C"

@@ -141,6 +141,14 @@ __device__ __forceinline__ void rsum_lane(uint32_t (&v)[32], uint32_t base) {
     }
 }
 
+// Transposed position of original index k = 64a + 8b + c inside the vector:
+// transpose_i's k -> a + 16*rev3(b) + 128*c (rev3 reverses 3 bits).
+__host__ __device__ constexpr uint32_t transpose_index(uint32_t k) {
+    uint32_t a = k >> 6, b = (k >> 3) & 7u, c = k & 7u;
+    uint32_t rb = ((b & 1u) << 2) | (b & 2u) | ((b >> 2) & 1u);
+    return a + 16u * rb + 128u * c;
+}
+
 // Original (pre-transpose) index inside the vector of transposed position
 // p = lane + 32*r: inverse of transpose_i's k -> a + 16*rev3(b) + 128*c.
 __host__ __device__ constexpr uint32_t untranspose_index(uint32_t p) {

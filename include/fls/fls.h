@@ -353,6 +353,24 @@ int fls_query_q41(const fls_column* datekey, const fls_column* custkey, const fl
                   uint64_t n_rows, const fls_star_dims* dims, const void* d_workspace, const fls_q41_params* params,
                   int64_t* d_sums, uint64_t* d_counts, uint32_t flags, void* stream);
 
+/* ============================================ untranspose and materialize
+ * transpose_i (inverse == 0) or transpose_o (inverse != 0) of n_vec whole
+ * vectors, d_in != d_out: the unified transposed layout of
+ * fastlanes/src/transpose.cpp:4110-5135 and its true inverse (the reference's
+ * transpose_o at transpose.cpp:5136-6161 repeats transpose_i's permutation).  */
+int fls_transpose_u32(const uint32_t* d_in, uint64_t n_vec, uint32_t inverse, uint32_t* d_out, void* stream);
+
+/* Decode a column of any encoding into plain values for a downstream operator
+ * (a decode followed by Crystal's BlockStore, crystal-opt/store.cuh:57-69):
+ *   FLS_ORDER_STORED  the column's own row order (transposed for DELTA, RLE,
+ *                     MIXED and transposed BITPACK); whole vectors, so d_out
+ *                     holds ceil(n_rows / 1024) * 1024 values;
+ *   FLS_ORDER_ROWS    original row order (transposed columns are untransposed
+ *                     on the fly); exactly n_rows values.                    */
+#define FLS_ORDER_STORED 0u
+#define FLS_ORDER_ROWS 1u
+int fls_decode_column_u32(const fls_column* col, uint64_t n_rows, uint32_t order, uint32_t* d_out, void* stream);
+
 /* Encoded bytes a full scan of a column reads for n_rows (packed words, plus
  * per-vector metadata and bases for DELTA).                                   */
 uint64_t fls_column_bytes(const fls_column* col, uint64_t n_rows);
