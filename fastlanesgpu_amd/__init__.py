@@ -729,3 +729,63 @@ def batch_from_segments(segments):
     h = ctypes.c_void_p()
     call("fls_batch_create", arr, len(segments), ctypes.byref(h))
     return Batch(h.value, keep)
+
+
+# ------------------------------------- tile-based competitor formats (comparison)
+class TileColumn:
+    """A GPU-FOR / GPU-DFOR / GPU-RFOR encoded column (host numpy arrays, or
+    device tensors after to_device()).  kind: "for", "dfor" or "rfor"."""
+
+    def __init__(self, kind, n, words, starts, rl_words=None, rl_starts=None):
+        self.kind, self.n, self.words, self.starts = kind, n, words, starts
+        self.rl_words, self.rl_starts = rl_words, rl_starts
+
+    def nbytes(self):
+        extra = 0 if self.rl_words is None else 4 * (len(self.rl_words) + len(self.rl_starts))
+        return 4 * (len(self.words) + len(self.starts)) + extra
+
+    def to_device(self):
+        import torch
+
+        dev = lambda a: None if a is None else torch.from_numpy(np.ascontiguousarray(a).view(np.int32)).cuda()  # noqa
+        return TileColumn(self.kind, self.n, dev(self.words), dev(self.starts), dev(self.rl_words),
+                          dev(self.rl_starts))
+
+
+def tile_encode(values, kind):
+    """Host encode of uint32 values into the tile-based format `kind`."""
+    x = _u32(values)
+    n = x.size
+    cap = int(load().fls_tile_max_words(n))
+    if kind == "rfor":
+        nt = (n + 511) // 512
+        v, r = np.zeros(cap, np.uint32), np.zeros(cap, np.uint32)
+        vs, rs = np.zeros(nt + 1, np.uint32), np.zeros(nt + 1, np.uint32)
+        nv, nr = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        call("fls_tile_rfor_encode_u32", x.ctypes.data, n, v.ctypes.data, vs.ctypes.data, r.ctypes.data,
+             rs.ctypes.data, ctypes.byref(nv), ctypes.byref(nr))
+        return TileColumn(kind, n, v[:nv.value], vs, r[:nr.value], rs)
+    nb = (n + 127) // 128 if kind == "for" else 4 * ((n + 511) // 512)
+    out, starts = np.zeros(cap, np.uint32), np.zeros(nb + 1, np.uint32)
+    nw = ctypes.c_uint64(0)
+    call(f"fls_tile_{kind}_encode_u32", x.ctypes.data, n, out.ctypes.data, starts.ctypes.data, ctypes.byref(nw))
+    return TileColumn(kind, n, out[:nw.value], starts)
+
+
+def tile_decode(col, out=None, d_sum=None, values=True, stream=None):
+    """Decode a device TileColumn: returns (values tensor or None, sum tensor or None)."""
+    import torch
+
+    if values and out is None:
+        out = torch.empty(max(1, col.n), dtype=torch.int32, device="cuda")
+    if not values:
+        out = None
+    if d_sum is None:
+        d_sum = torch.zeros(1, dtype=torch.int64, device="cuda")
+    if col.kind == "rfor":
+        call("fls_tile_rfor_decode_u32", dptr(col.words), dptr(col.starts), dptr(col.rl_words), dptr(col.rl_starts),
+             col.n, dptr(out), dptr(d_sum), _stream(stream))
+    else:
+        call(f"fls_tile_{col.kind}_decode_u32", dptr(col.words), dptr(col.starts), col.n, dptr(out), dptr(d_sum),
+             _stream(stream))
+    return (out[:col.n] if out is not None else None), d_sum

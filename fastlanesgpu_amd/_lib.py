@@ -168,6 +168,16 @@ SIGNATURES.update({
     "fls_decode_column_u32": (c_int, [COLP, c_u64, c_u32, c_vp, c_vp]),
 })
 
+SIGNATURES.update({
+    "fls_tile_max_words": (c_u64, [c_u64]),
+    "fls_tile_for_encode_u32": (c_int, [c_vp, c_u64, c_vp, c_vp, c_vp]),
+    "fls_tile_dfor_encode_u32": (c_int, [c_vp, c_u64, c_vp, c_vp, c_vp]),
+    "fls_tile_rfor_encode_u32": (c_int, [c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "fls_tile_for_decode_u32": (c_int, [c_vp, c_vp, c_u64, c_vp, c_vp, c_vp]),
+    "fls_tile_dfor_decode_u32": (c_int, [c_vp, c_vp, c_u64, c_vp, c_vp, c_vp]),
+    "fls_tile_rfor_decode_u32": (c_int, [c_vp, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp]),
+})
+
 _SDP = ctypes.POINTER(StarDims)
 SIGNATURES.update({
     "fls_gen_star_extra": (c_int, [c_u64, c_u64, c_vp, c_u32, c_vp, c_vp]),

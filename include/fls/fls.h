@@ -371,6 +371,36 @@ int fls_transpose_u32(const uint32_t* d_in, uint64_t n_vec, uint32_t inverse, ui
 #define FLS_ORDER_ROWS 1u
 int fls_decode_column_u32(const fls_column* col, uint64_t n_rows, uint32_t order, uint32_t* d_out, void* stream);
 
+/* ============================== tile-based competitor formats (comparison)
+ * GPU-FOR, GPU-DFOR and GPU-RFOR of the tile-based compression baseline the
+ * reference carries (tile_based/, Shanbhag et al.), word-compatible with its
+ * encoders: bin_pack (tile_based_bench_bitpack.cu:6-83), deltaBinPack
+ * (tile_based_bench_delta.cu:8-112) and rleBinPack (rlebinpack.cpp:9-146).
+ * A 4-word header {block size, 4, total count, first value}, then blocks;
+ * starts[] holds every block's word offset and the end: ceil(n/128) + 1
+ * entries for FOR, 4 ceil(n/512) + 1 for DFOR (4 blocks behind each 512-value
+ * tile's first value), ceil(n/512) + 1 for RFOR (one run-value and one
+ * run-length block per tile).  Every block has one width; width 32 is one
+ * value per word (the reference loops shift by 32 there, undefined in C++).
+ * Inputs are padded to whole blocks with the last value.  Encoders run on the
+ * host into buffers of fls_tile_max_words(n) words.                         */
+uint64_t fls_tile_max_words(uint64_t n);
+int fls_tile_for_encode_u32(const uint32_t* in, uint64_t n, uint32_t* out, uint32_t* starts, uint64_t* n_words);
+int fls_tile_dfor_encode_u32(const uint32_t* in, uint64_t n, uint32_t* out, uint32_t* starts, uint64_t* n_words);
+int fls_tile_rfor_encode_u32(const uint32_t* in, uint64_t n, uint32_t* values, uint32_t* val_starts,
+                             uint32_t* run_lengths, uint32_t* rl_starts, uint64_t* n_val_words, uint64_t* n_rl_words);
+/* Device decoders (LoadBinPack / LoadDBinPack / LoadRBinPack,
+ * tile_based/src/include/binpack_kernel.cuh:20-84, deltabinpack_kernel.cuh:
+ * 39-103, rlebinpack_kernel.cuh:20-146): n values to d_out and/or their sum to
+ * *d_sum (zeroed first); at least one of the two non-NULL.                   */
+int fls_tile_for_decode_u32(const uint32_t* d_data, const uint32_t* d_starts, uint64_t n, uint32_t* d_out,
+                            uint64_t* d_sum, void* stream);
+int fls_tile_dfor_decode_u32(const uint32_t* d_data, const uint32_t* d_starts, uint64_t n, uint32_t* d_out,
+                             uint64_t* d_sum, void* stream);
+int fls_tile_rfor_decode_u32(const uint32_t* d_values, const uint32_t* d_val_starts, const uint32_t* d_run_lengths,
+                             const uint32_t* d_rl_starts, uint64_t n, uint32_t* d_out, uint64_t* d_sum,
+                             void* stream);
+
 /* Encoded bytes a full scan of a column reads for n_rows (packed words, plus
  * per-vector metadata and bases for DELTA).                                   */
 uint64_t fls_column_bytes(const fls_column* col, uint64_t n_rows);

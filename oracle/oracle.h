@@ -205,4 +205,17 @@ double orc_mlp_grad(const double* w, const float* feat, const uint8_t* label, ui
 #ifdef __cplusplus
 }
 #endif
+/* Tile-based competitor formats (tile.c): GPU-FOR, GPU-DFOR, GPU-RFOR.  Output
+ * buffers hold orc_tile_max_words(n) words; starts hold ceil(n/128) + 1 (FOR,
+ * DFOR: 4 per 512-value tile + 1) or ceil(n/512) + 1 (RFOR) entries.         */
+uint64_t orc_tile_max_words(uint64_t n);
+uint64_t orc_tile_for_encode(const uint32_t* in, uint64_t n, uint32_t* out, uint32_t* starts);
+void orc_tile_for_decode(const uint32_t* data, const uint32_t* starts, uint64_t n, uint32_t* out);
+uint64_t orc_tile_dfor_encode(const uint32_t* in, uint64_t n, uint32_t* out, uint32_t* starts);
+void orc_tile_dfor_decode(const uint32_t* data, const uint32_t* starts, uint64_t n, uint32_t* out);
+void orc_tile_rfor_encode(const uint32_t* in, uint64_t n, uint32_t* values, uint32_t* val_starts,
+                          uint32_t* run_lengths, uint32_t* rl_starts, uint64_t* n_val_words, uint64_t* n_rl_words);
+void orc_tile_rfor_decode(const uint32_t* values, const uint32_t* val_starts, const uint32_t* run_lengths,
+                          const uint32_t* rl_starts, uint64_t n, uint32_t* out);
+
 #endif

@@ -68,6 +68,15 @@ def oracle():
     o.orc_q11.restype = ctypes.c_int64
     o.orc_q21.argtypes = [U32P, U32P, U32P, U32P, u64, U32P, U32P, U32P, U32P, c_u, c_u, I64P, U64P]
     o.orc_gen_star_extra.argtypes = [u64, u64, U32P, c_u, U32P]
+    o.orc_tile_max_words.argtypes = [u64]
+    o.orc_tile_max_words.restype = u64
+    for name in ("orc_tile_for_encode", "orc_tile_dfor_encode"):
+        getattr(o, name).argtypes = [U32P, u64, U32P, U32P]
+        getattr(o, name).restype = u64
+    for name in ("orc_tile_for_decode", "orc_tile_dfor_decode"):
+        getattr(o, name).argtypes = [U32P, U32P, u64, U32P]
+    o.orc_tile_rfor_encode.argtypes = [U32P, u64, U32P, U32P, U32P, U32P, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+    o.orc_tile_rfor_decode.argtypes = [U32P, U32P, U32P, U32P, u64, U32P]
     o.orc_q31.argtypes = [U32P, U32P, U32P, U32P, u64, U32P, U32P, U32P, U32P, U32P, c_u, c_u, c_u, c_u, I64P, U64P]
     o.orc_q41.argtypes = [U32P, U32P, U32P, U32P, U32P, U32P, u64, U32P, U32P, U32P, U32P, U32P, c_u, c_u, c_u, c_u,
                           I64P, U64P]
@@ -428,3 +437,43 @@ def encode_any(x, kind):
     if kind == "bitpack_t":
         return fls.bitpack_column(x, transposed=True).to_device()
     raise ValueError(kind)
+
+
+# ----------------------------------------------------- tile-based formats (oracle)
+def o_tile_encode(kind, x):
+    """GPU-FOR ("for") / GPU-DFOR ("dfor") words and block starts of x."""
+    o = oracle()
+    x = np.ascontiguousarray(x, np.uint32)
+    n = x.size
+    out = np.zeros(int(o.orc_tile_max_words(n)), np.uint32)
+    starts = np.zeros((n + 127) // 128 + 8, np.uint32)
+    nw = getattr(o, f"orc_tile_{kind}_encode")(x, n, out, starts)
+    nb = (n + 127) // 128 if kind == "for" else 4 * ((n + 511) // 512)
+    return out[:nw].copy(), starts[:nb + 1].copy()
+
+
+def o_tile_decode(kind, words, starts, n):
+    out = np.zeros(max(1, n), np.uint32)
+    getattr(oracle(), f"orc_tile_{kind}_decode")(np.ascontiguousarray(words, np.uint32),
+                                                  np.ascontiguousarray(starts, np.uint32), n, out)
+    return out[:n]
+
+
+def o_rfor_encode(x):
+    """GPU-RFOR: (values, val_starts, run_lengths, rl_starts)."""
+    o = oracle()
+    x = np.ascontiguousarray(x, np.uint32)
+    n = x.size
+    cap = int(o.orc_tile_max_words(n))
+    vals, rls = np.zeros(cap, np.uint32), np.zeros(cap, np.uint32)
+    nt = (n + 511) // 512
+    vs, rs = np.zeros(nt + 1, np.uint32), np.zeros(nt + 1, np.uint32)
+    nv, nr = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    o.orc_tile_rfor_encode(x, n, vals, vs, rls, rs, ctypes.byref(nv), ctypes.byref(nr))
+    return vals[:nv.value].copy(), vs, rls[:nr.value].copy(), rs
+
+
+def o_rfor_decode(vals, vs, rls, rs, n):
+    out = np.zeros(max(1, n), np.uint32)
+    oracle().orc_tile_rfor_decode(vals, vs, rls, rs, n, out)
+    return out[:n]
