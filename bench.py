@@ -283,6 +283,63 @@ def bench_materialize(steps, warmup, seed):
     return res
 
 
+def bench_tile_vs_fastlanes(steps, warmup, seed):
+    """Decode to memory of the same 2^26-value columns in FastLanes and in the
+    tile-based GPU-FOR / GPU-DFOR / GPU-RFOR formats (the competitor baseline
+    the reference carries): bit-packing at widths 3 and 11 (uniform), delta
+    (the reference delta bench's column 7 i) and RLE (geometric runs, mean 8)."""
+    import numpy as np
+    import torch
+
+    import fastlanesgpu_amd as fls
+
+    n = 1 << 26
+    n_vec = n // 1024
+    rng = np.random.default_rng(seed)
+    out = torch.empty(n, dtype=torch.int32, device="cuda")
+
+    def timed(fn):
+        _, per = time_kernel(fn, out, steps, warmup)
+        return statistics.mean(per)
+
+    def entry(ms, enc_bytes):
+        return {"kernel_ms": ms, "values_per_s": n / (ms * 1e-3), "encoded_bytes": enc_bytes,
+                "bits_per_value": 8 * enc_bytes / n,
+                "GBs_read_plus_write": (enc_bytes + 4 * n) / (ms * 1e-3) / 1e9}
+
+    def tile(x, kind):
+        col = fls.tile_encode(x, kind)
+        dcol = col.to_device()
+        ms = timed(lambda: fls.tile_decode(dcol, out=out))
+        ok = bool(torch.equal(out, torch.from_numpy(x.view(np.int32)).cuda()))
+        return dict(entry(ms, col.nbytes()), decoded_equal=ok)
+
+    res = {"workload": "2^26 uint32 values per column, decode to memory (4 B per value written)"}
+    for bw in (3, 11):
+        x = rng.integers(0, 1 << bw, n).astype(np.uint32)
+        d_in = torch.from_numpy(fls.pack_column(x, bw).view(np.int32)).cuda()
+        ms = timed(lambda: fls.unpack_global(d_in, out, n_vec, bw))
+        ok = bool(torch.equal(out, torch.from_numpy(x.view(np.int32)).cuda()))
+        res[f"bitpack_bw{bw}"] = {"fastlanes": dict(entry(ms, d_in.numel() * 4), decoded_equal=ok),
+                                  "gpu_for": tile(x, "for")}
+    x = (7 * np.arange(n, dtype=np.uint64)).astype(np.uint32)
+    w, vm, bs = fls.delta_encode_device(torch.from_numpy(x.view(np.int32)).cuda(), n)
+    col = fls.Column(fls.ENC_DELTA, n, w, vmeta=vm, bases=bs)
+    ms = timed(lambda: fls.decode_column(col, n, fls.ORDER_ROWS, out=out))
+    ok = bool(torch.equal(out, torch.from_numpy(x.view(np.int32)).cuda()))
+    res["delta_7i"] = {"fastlanes_row_order": dict(entry(ms, col.nbytes()), decoded_equal=ok),
+                       "gpu_dfor": tile(x, "dfor")}
+    lens = rng.geometric(1 / 8, n)
+    x = np.repeat(rng.integers(0, 1 << 16, n), lens)[:n].astype(np.uint32)
+    w, vm, rm, bs = fls.rle_encode_device(torch.from_numpy(x.view(np.int32)).cuda(), n)
+    col = fls.Column(fls.ENC_RLE, n, w, vmeta=vm, bases=bs, aux=rm)
+    ms = timed(lambda: fls.decode_column(col, n, fls.ORDER_ROWS, out=out))
+    ok = bool(torch.equal(out, torch.from_numpy(x.view(np.int32)).cuda()))
+    res["rle_mean8"] = {"fastlanes_row_order": dict(entry(ms, col.nbytes()), decoded_equal=ok),
+                        "gpu_rfor": tile(x, "rfor")}
+    return res
+
+
 def bench_c4(steps, seed):
     """4096 uneven segments (bitpack/delta/RLE) decoded + range-filtered:
     one graph-replayed persistent launch vs one launch per segment."""
@@ -621,6 +678,8 @@ def main():
         line["secondary"]["star_joins"] = bench_star(max(5, args.steps // 2), 2, args.seed)
         log("[bench] secondary: materialize")
         line["secondary"]["materialize"] = bench_materialize(max(5, args.steps // 2), 2, args.seed)
+        log("[bench] secondary: tile-based formats vs FastLanes")
+        line["secondary"]["tile_vs_fastlanes"] = bench_tile_vs_fastlanes(max(5, args.steps // 2), 2, args.seed)
         log("[bench] secondary: batched segments")
         line["secondary"]["c4_batched"] = bench_c4(max(5, args.steps // 5), args.seed)
         if not args.no_c5:

@@ -1,11 +1,12 @@
 // MI355X decoders of the tile-based competitor formats GPU-FOR, GPU-DFOR and
 // GPU-RFOR (Shanbhag et al.; the reference's LoadBinPack / LoadDBinPack /
 // LoadRBinPack, tile_based/src/include/*.cuh, run 128-thread blocks with CUB
-// block scans).  Here one wave64 decodes one 512-value tile from the column
-// (no staging copy: the tile's words are read through the cache), the prefix
-// sums of DFOR and RFOR are wave scans, and the RLE expansion marks run starts
-// in a per-wave LDS slice.  Each decoder optionally writes the values and/or
-// adds their 64-bit sum (one atomic per wave at the end).
+// block scans).  Here one wave64 decodes one 512-value tile: the tile's words
+// are copied into the wave's LDS slice with coalesced loads (one memory round
+// trip per tile), elements are decoded from LDS, the prefix sums of DFOR and
+// RFOR are wave scans, and the RLE expansion marks run starts in LDS.  Each
+// decoder optionally writes the values and/or adds their 64-bit sum (one
+// atomic per wave at the end).
 #include "fls/device/wave.hpp"
 #include "fls_runtime.hpp"
 #include "fls_columns.hpp"
@@ -62,28 +63,60 @@ __device__ __forceinline__ void add_sum(uint64_t* sum, uint64_t s) {
     if (lane64() == 0 && s) atomicAdd(reinterpret_cast<unsigned long long*>(sum), (unsigned long long)s);
 }
 
+// Words of a tile: 4 FOR blocks of at most 2 + 4 x 32 words (DFOR: plus the
+// tile's first value); RFOR: a run-value and a run-length block of at most
+// 3 + 512 words each.
+constexpr uint32_t kForTileWords = 4u * (2u + 4u * 32u) + 1u;
+constexpr uint32_t kRforBlockWords = 3u + kTile;
+
+// Copy the first min(count, CAP) words of src into st, coalesced: all of a
+// lane's loads are issued before any is stored.
+template <uint32_t CAP>
+__device__ __forceinline__ void stage_words(uint32_t* st, const uint32_t* __restrict src, uint32_t count) {
+    constexpr int kSteps = (CAP + 63) / 64;
+    count = min(count, CAP);
+    const uint32_t l = lane64();
+    uint32_t w[kSteps];
+#pragma unroll
+    for (int k = 0; k < kSteps; ++k) {
+        const uint32_t i = l + 64u * k;
+        w[k] = i < count ? src[i] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kSteps; ++k) {
+        const uint32_t i = l + 64u * k;
+        if (i < count) st[i] = w[k];
+    }
+}
+
 // GPU-FOR: lane l decodes elements l + 64 k (k < 8) of its tile, so each
 // step's block (k / 2) is the same across the wave and the stores coalesce.
 template <bool OUT, bool SUM>
 __global__ __launch_bounds__(kTileBlock) void tile_for_kernel(const uint32_t* __restrict data,
                                                               const uint32_t* __restrict starts, uint64_t n,
                                                               uint32_t* __restrict out, uint64_t* sum) {
+    __shared__ uint32_t lds[kTileWaves][kForTileWords];
+    uint32_t* st = lds[threadIdx.x >> 6];
     const uint64_t n_tiles = (n + kTile - 1) / kTile, n_blocks = (n + 127) / 128;
     const uint32_t l = lane64();
     uint64_t acc = 0;
     for (uint64_t t = (uint64_t)blockIdx.x * kTileWaves + (threadIdx.x >> 6); t < n_tiles;
          t += (uint64_t)gridDim.x * kTileWaves) {
+        const uint64_t b0 = 4 * t, nb = min<uint64_t>(4, n_blocks - b0);
+        const uint32_t w0 = starts[b0];
+        stage_words<kForTileWords - 1u>(st, data + w0, starts[b0 + nb] - w0);
+        wave_fence();
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const uint64_t b = 4 * t + (k >> 1);
-            if (b >= n_blocks) break;  // wave-uniform
+            if ((uint64_t)(k >> 1) >= nb) break;  // wave-uniform
             const uint64_t e = kTile * t + l + 64u * k;
-            const uint32_t v = for_element(data + starts[b], l + 64u * (k & 1));
+            const uint32_t v = for_element(st + (starts[b0 + (k >> 1)] - w0), l + 64u * (k & 1));
             if (e < n) {
                 if constexpr (OUT) out[e] = v;
                 if constexpr (SUM) acc += v;
             }
         }
+        wave_fence();
     }
     if constexpr (SUM) add_sum(sum, acc);
 }
@@ -95,6 +128,8 @@ template <bool OUT, bool SUM>
 __global__ __launch_bounds__(kTileBlock) void tile_dfor_kernel(const uint32_t* __restrict data,
                                                                const uint32_t* __restrict starts, uint64_t n,
                                                                uint32_t* __restrict out, uint64_t* sum) {
+    __shared__ uint32_t lds[kTileWaves][kForTileWords];
+    uint32_t* st = lds[threadIdx.x >> 6];
     const uint64_t n_tiles = (n + kTile - 1) / kTile;
     const uint32_t l = lane64();
     uint64_t acc = 0;
@@ -102,16 +137,20 @@ __global__ __launch_bounds__(kTileBlock) void tile_dfor_kernel(const uint32_t* _
          t += (uint64_t)gridDim.x * kTileWaves) {
         const uint64_t base = kTile * t;
         const uint32_t live = (uint32_t)min<uint64_t>(kTile, n - base);
-        const uint32_t* blk = data + starts[4 * t + (l >> 4)];  // tiles always hold 4 blocks
+        const uint32_t w0 = starts[4 * t] - 1u;  // the tile's first value, then its 4 blocks
+        stage_words<kForTileWords>(st, data + w0, starts[4 * t + 4] - w0);
+        wave_fence();
+        const uint32_t* blk = st + (starts[4 * t + (l >> 4)] - w0);
         const uint32_t e0 = 8u * (l & 15u);
         uint32_t x[8], run = 0;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             x[i] = for_element(blk, e0 + i);
-            if (l == 0 && i == 0) x[0] = data[starts[4 * t] - 1u];
+            if (l == 0 && i == 0) x[0] = st[0];
             run += x[i];
             x[i] = run;
         }
+        wave_fence();
         const uint32_t before = wave_inclusive_scan(run) - run;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
@@ -137,9 +176,11 @@ __global__ __launch_bounds__(kTileBlock) void tile_rfor_kernel(const uint32_t* _
                                                                const uint32_t* __restrict run_lengths,
                                                                const uint32_t* __restrict rl_starts, uint64_t n,
                                                                uint32_t* __restrict out, uint64_t* sum) {
-    __shared__ uint32_t lds[kTileWaves][2 * kTile];
+    __shared__ uint32_t lds[kTileWaves][2 * kRforBlockWords];
     uint32_t* runval = lds[threadIdx.x >> 6];
     uint32_t* mark = runval + kTile;
+    uint32_t* vst = lds[threadIdx.x >> 6];
+    uint32_t* rst = vst + kRforBlockWords;
     const uint64_t n_tiles = (n + kTile - 1) / kTile;
     const uint32_t l = lane64();
     uint64_t acc = 0;
@@ -147,17 +188,20 @@ __global__ __launch_bounds__(kTileBlock) void tile_rfor_kernel(const uint32_t* _
          t += (uint64_t)gridDim.x * kTileWaves) {
         const uint64_t base = kTile * t;
         const uint32_t live = (uint32_t)min<uint64_t>(kTile, n - base);
-        const uint32_t* vb = values + val_starts[t];
-        const uint32_t* rb = run_lengths + rl_starts[t];
-        const uint32_t count = min(vb[2], kTile), vbw = vb[1] & 255u, rbw = rb[1] & 255u;
+        const uint32_t vs = val_starts[t], rs = rl_starts[t];
+        stage_words<kRforBlockWords>(vst, values + vs, val_starts[t + 1] - vs);
+        stage_words<kRforBlockWords>(rst, run_lengths + rs, rl_starts[t + 1] - rs);
+        wave_fence();
+        const uint32_t count = min(vst[2], kTile), vbw = vst[1] & 255u, rbw = rst[1] & 255u;
         uint32_t v[8], len[8], run = 0;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const uint32_t r = 8u * l + i;
-            v[i] = r < count ? vb[0] + run_value(vb + 3, r, vbw) : 0u;
-            len[i] = r < count ? rb[0] + run_value(rb + 3, r, rbw) : 0u;
+            v[i] = r < count ? vst[0] + run_value(vst + 3, r, vbw) : 0u;
+            len[i] = r < count ? rst[0] + run_value(rst + 3, r, rbw) : 0u;
             run += len[i];
         }
+        wave_fence();  // the slice is reused for the run values and marks below
         const uint32_t first = wave_inclusive_scan(run) - run;  // position of run 8l
         if constexpr (SUM) {
             uint32_t p = first;
