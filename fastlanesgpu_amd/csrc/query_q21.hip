@@ -152,15 +152,38 @@ __global__ __launch_bounds__(256) void q21_pht_kernel(const Q21Dev a) {
 
 // Compact-table probe.  1024 work-items = 32 vectors per step; LDS holds the
 // supplier and part bitmaps and the year table when they fit (USE_LDS).
+// Per-wave LDS queue of rows that passed every filter: part index, date
+// index and revenue, kQueue of each.  The flush runs with one lane per row:
+// brand from L2, year from the year table, 64-bit group atomics.
+constexpr uint32_t kQueue = 64;
+constexpr size_t kQueueBytes = 16ull * 3u * kQueue * 4u;  // 16 waves of a 1024-item block
+
+__device__ __forceinline__ void q21_flush(const Q21Dev& a, uint32_t* q, uint32_t qn, const uint16_t* year_tab) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t l = threadIdx.x & 63u;
+    if (l < qn) {
+        const uint32_t p = q[l], d = q[kQueue + l];
+        if (d < a.n_date) group_add(a, a.brand[p], year_tab[d], q[2 * kQueue + l]);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int DBW, int SBW, int PBW, int RBW, bool USE_LDS>
 __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
     extern __shared__ uint32_t lds[];
+    // the queues first (4-byte aligned at offset 0), then the staged tables
+    uint32_t* q = lds + (threadIdx.x >> 6) * 3u * kQueue;
+    uint32_t* stage = lds + kQueueBytes / 4u;
     const uint32_t* supp_bm = a.supp_bm;
     const uint32_t* part_bm = a.part_bm;
     const uint16_t* year_tab = a.year;
     if constexpr (USE_LDS) {
-        uint32_t* s = lds;
-        uint32_t* p = lds + a.supp_words;
+        uint32_t* s = stage;
+        uint32_t* p = stage + a.supp_words;
         uint16_t* y = reinterpret_cast<uint16_t*>(p + a.part_words);
         for (uint32_t i = threadIdx.x; i < a.supp_words; i += blockDim.x) s[i] = a.supp_bm[i];
         for (uint32_t i = threadIdx.x; i < a.part_words; i += blockDim.x) p[i] = a.part_bm[i];
@@ -175,20 +198,26 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
         live_row_mask((uint32_t)(a.n_rows - (a.n_vec - 1) * 1024u), a.rev.transposed != 0);
     for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
         const uint64_t v = g * 32u + (threadIdx.x >> 5);
-        if (v >= a.n_vec) continue;
+        // the whole wave stays in step (the queue flush needs every lane): a
+        // half-wave past the last vector reads the last vector again (in
+        // bounds, loads unconditional) and keeps no rows
+        const bool vlive = v < a.n_vec;
+        if (!__any(vlive)) continue;
+        const uint64_t vl = vlive ? v : a.n_vec - 1;
         LaneCol<SBW> sc;
         LaneCol<PBW> pc;
         LaneCol<DBW> dc;
         LaneCol<RBW> rc;
-        sc.load(a.supp, v);
-        pc.load(a.part, v);
-        dc.load(a.date, v);
-        rc.load(a.rev, v);
+        sc.load(a.supp, vl);
+        pc.load(a.part, vl);
+        dc.load(a.date, vl);
+        rc.load(a.rev, vl);
         // supplier semi-join on every row
         const uint32_t km = a.keys_min;
         uint32_t m = 0;
 #pragma unroll
         for (int r = 0; r < 32; ++r) m |= bit_at(supp_bm, sc.get(r), km, a.n_supp) << r;
+        if (!vlive) m = 0;
         if (v == a.n_vec - 1) m &= tail;
         pc.pin();
         dc.pin();
@@ -211,26 +240,30 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
             }
         }
         if (!__any(m != 0)) continue;
-        // group-by on the rows that passed both: brand (L2) and year lookups
-        // of 8 rows issued together, then the global atomics
+        // group-by on the rows that passed both (about 17 per 2,048 here):
+        // compacted into the wave's LDS queue, then looked up and added with
+        // one lane per row -- one brand round trip per queue, not per row slot
+        uint32_t qn = 0;  // wave-uniform
 #pragma unroll
-        for (int rb = 0; rb < 32; rb += 8) {
-            if (!__any(((m >> rb) & 0xFFu) != 0)) continue;
-            uint32_t bv[8], yv[8], ok = 0;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int r = rb + j;
-                const bool live = (m >> r) & 1u;  // passed the part bitmap: key in range
-                const uint32_t p = pc.get(r) - km, d = dc.get(r) - km;
-                const bool in = live && d < a.n_date;
-                ok |= (uint32_t)in << j;
-                bv[j] = a.brand[live ? p : 0u];
-                yv[j] = year_tab[in ? d : 0u];
+        for (int r = 0; r < 32; ++r) {
+            const bool live = (m >> r) & 1u;
+            const uint64_t bal = __ballot(live);
+            if (bal == 0) continue;
+            const uint32_t cnt = (uint32_t)__popcll(bal);
+            if (qn + cnt > kQueue) {
+                q21_flush(a, q, qn, year_tab);
+                qn = 0;
             }
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                if ((ok >> j) & 1u) group_add(a, bv[j], yv[j], rc.get(rb + j));
+            if (live) {
+                const uint32_t pos =
+                    qn + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                q[pos] = pc.get(r) - km;
+                q[kQueue + pos] = dc.get(r) - km;
+                q[2 * kQueue + pos] = rc.get(r);
+            }
+            qn += cnt;
         }
+        if (qn) q21_flush(a, q, qn, year_tab);
     }
 }
 
@@ -317,12 +350,10 @@ int launch_resident_q21(Kernel k, int block, size_t lds, const Q21Dev& a, hipStr
 
 template <int DBW, int SBW, int PBW, int RBW>
 int launch_compact(const Q21Dev& a, bool in_lds, size_t lds, hipStream_t s) {
-    if (in_lds) {
-        auto k = q21_compact_kernel<DBW, SBW, PBW, RBW, true>;
-        FLS_HIP_CHECK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        return launch_resident_q21(k, 1024, lds, a, s);
-    }
-    return launch_resident_q21(q21_compact_kernel<DBW, SBW, PBW, RBW, false>, 1024, 0, a, s);
+    auto k = in_lds ? q21_compact_kernel<DBW, SBW, PBW, RBW, true> : q21_compact_kernel<DBW, SBW, PBW, RBW, false>;
+    const size_t bytes = kQueueBytes + (in_lds ? lds : 0);
+    FLS_HIP_CHECK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return launch_resident_q21(k, 1024, bytes, a, s);
 }
 
 // allow_var: the compact probe decodes any encoding (decode_regs); the
@@ -456,7 +487,7 @@ int fls_query_q21(const fls_column* datekey, const fls_column* suppkey, const fl
     a.part_words = L.part_words;
     a.n_groups = ceil_div(a.n_vec, 32);
     const size_t lds = 4ull * (L.supp_words + L.part_words) + 2ull * dims->n_date + 16u;
-    const bool in_lds = lds <= 150u * 1024u;
+    const bool in_lds = lds + kQueueBytes <= 150u * 1024u;
     const bool fast = a.date.enc == FLS_ENC_BITPACK && a.supp.enc == FLS_ENC_BITPACK &&
                       a.part.enc == FLS_ENC_BITPACK && a.rev.enc == FLS_ENC_BITPACK && a.date.bw == 12 &&
                       a.supp.bw == 16 && a.part.bw == 20 && a.rev.bw == 24;

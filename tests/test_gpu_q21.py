@@ -92,6 +92,23 @@ def test_q21_any_encoding_columns(layout):
         fls.query_q21_pht(*cols, n, pht)
 
 
+@pytest.mark.parametrize("n", [40_000 + 3, 33 * 1024])
+def test_q21_every_row_qualifies(n):
+    # all suppliers in the region and all parts in the category: the per-wave
+    # queue of qualifying rows fills and flushes many times per vector pair,
+    # and a half-wave past the last vector must not drop queued rows
+    fact, dims = T.o_gen_c3(14, n, **SMALL)
+    fact[1] %= SMALL["n_supp"]
+    fact[2] %= SMALL["n_part"]
+    dims["s_region"][:] = 2
+    dims["p_category"][:] = 12
+    want = T.o_q21(fact, dims, n)
+    assert int(want[1].sum()) == n
+    out_c, out_p = run_both(fact, dims, SMALL, n)
+    check(out_c, want)
+    check(out_p, want)
+
+
 def test_q21_full_size_and_result_rows():
     n = 120_000_000
     fd = fls.gen_c3_fact(1, n)
