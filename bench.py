@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-secondary", action="store_true", help="skip the secondary workloads")
     ap.add_argument("--no-c5", action="store_true", help="skip the learned-selector workload")
+    ap.add_argument("--only", default="", help="comma list of secondary workloads to run "
+                    "(c1, c3, star, materialize, tile, c4, c5); default all")
+    ap.add_argument("--c5-train-vectors", type=int, default=1 << 22,
+                    help="training-set size of the learned selector (smaller for profiler runs)")
     return ap.parse_args()
 
 
@@ -670,21 +674,30 @@ def main():
         line["cpu_baseline"] = cpu_baseline(enc, n, result)
     del dev, enc
     if not args.no_secondary:
-        log("[bench] secondary: plain unpack")
-        line["secondary"] = {"c1_unpack": bench_c1(args.steps, args.warmup, args.seed)}
-        log("[bench] secondary: probe + group-by")
-        line["secondary"]["c3_probe_groupby"] = bench_c3(max(5, args.steps // 2), 2, args.seed)
-        log("[bench] secondary: star joins")
-        line["secondary"]["star_joins"] = bench_star(max(5, args.steps // 2), 2, args.seed)
-        log("[bench] secondary: materialize")
-        line["secondary"]["materialize"] = bench_materialize(max(5, args.steps // 2), 2, args.seed)
-        log("[bench] secondary: tile-based formats vs FastLanes")
-        line["secondary"]["tile_vs_fastlanes"] = bench_tile_vs_fastlanes(max(5, args.steps // 2), 2, args.seed)
-        log("[bench] secondary: batched segments")
-        line["secondary"]["c4_batched"] = bench_c4(max(5, args.steps // 5), args.seed)
-        if not args.no_c5:
+        only = {x.strip() for x in args.only.split(",") if x.strip()}
+        want = lambda name: not only or name in only  # noqa: E731
+        line["secondary"] = {}
+        if want("c1"):
+            log("[bench] secondary: plain unpack")
+            line["secondary"]["c1_unpack"] = bench_c1(args.steps, args.warmup, args.seed)
+        if want("c3"):
+            log("[bench] secondary: probe + group-by")
+            line["secondary"]["c3_probe_groupby"] = bench_c3(max(5, args.steps // 2), 2, args.seed)
+        if want("star"):
+            log("[bench] secondary: star joins")
+            line["secondary"]["star_joins"] = bench_star(max(5, args.steps // 2), 2, args.seed)
+        if want("materialize"):
+            log("[bench] secondary: materialize")
+            line["secondary"]["materialize"] = bench_materialize(max(5, args.steps // 2), 2, args.seed)
+        if want("tile"):
+            log("[bench] secondary: tile-based formats vs FastLanes")
+            line["secondary"]["tile_vs_fastlanes"] = bench_tile_vs_fastlanes(max(5, args.steps // 2), 2, args.seed)
+        if want("c4"):
+            log("[bench] secondary: batched segments")
+            line["secondary"]["c4_batched"] = bench_c4(max(5, args.steps // 5), args.seed)
+        if want("c5") and not args.no_c5:
             log("[bench] secondary: learned encoding selector")
-            line["secondary"]["c5_learned_selector"] = bench_c5(args.seed, n)
+            line["secondary"]["c5_learned_selector"] = bench_c5(args.seed, n, n_train=args.c5_train_vectors)
     print(json.dumps(line), flush=True)
 
 
