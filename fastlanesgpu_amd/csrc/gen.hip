@@ -85,11 +85,23 @@ __global__ void gen_c3_fact_kernel(uint64_t seed, uint64_t n, uint32_t* __restri
     }
 }
 
-__global__ void gen_dim2_kernel(uint64_t ka, uint32_t ra, uint64_t kb, uint32_t rb, uint32_t n, uint32_t* __restrict a,
-                                uint32_t* __restrict b) {
+// SSB's hierarchies.  Nation U{1..25}, its region (nation - 1) / 5 + 1.
+__global__ void gen_geo_kernel(uint64_t k_nation, uint32_t n, uint32_t* __restrict region,
+                               uint32_t* __restrict nation) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        a[i] = 1u + uniform(ka, i, ra);
-        b[i] = 1u + uniform(kb, i, rb);
+        const uint32_t na = 1u + uniform(k_nation, i, 25u);
+        nation[i] = na;
+        region[i] = (na - 1u) / 5u + 1u;
+    }
+}
+
+// Category U{1..25}, brand 40 (category - 1) + U{1..40}.
+__global__ void gen_part_kernel(uint64_t k_cat, uint64_t k_brand, uint32_t n, uint32_t* __restrict category,
+                                uint32_t* __restrict brand) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t c = 1u + uniform(k_cat, i, 25u);
+        category[i] = c;
+        brand[i] = (c - 1u) * 40u + 1u + uniform(k_brand, i, 40u);
     }
 }
 
@@ -171,18 +183,18 @@ int fls_gen_c3_dims(uint64_t seed, uint32_t n_date, uint32_t n_supp, uint32_t n_
     }
     if (n_supp) {
         FLS_RETURN_IF(!d_s_region || !d_s_nation, FLS_ERR_NULL);
-        hipLaunchKernelGGL(gen_dim2_kernel, dim3(grid_for(n_supp)), dim3(256), 0, s, stream_key(seed, ST_SREG), 5u,
-                           stream_key(seed, ST_SNAT), 25u, n_supp, d_s_region, d_s_nation);
+        hipLaunchKernelGGL(gen_geo_kernel, dim3(grid_for(n_supp)), dim3(256), 0, s, stream_key(seed, ST_SNAT), n_supp,
+                           d_s_region, d_s_nation);
     }
     if (n_part) {
         FLS_RETURN_IF(!d_p_category || !d_p_brand, FLS_ERR_NULL);
-        hipLaunchKernelGGL(gen_dim2_kernel, dim3(grid_for(n_part)), dim3(256), 0, s, stream_key(seed, ST_PCAT), 25u,
-                           stream_key(seed, ST_PBRAND), 1000u, n_part, d_p_category, d_p_brand);
+        hipLaunchKernelGGL(gen_part_kernel, dim3(grid_for(n_part)), dim3(256), 0, s, stream_key(seed, ST_PCAT),
+                           stream_key(seed, ST_PBRAND), n_part, d_p_category, d_p_brand);
     }
     if (n_cust) {
         FLS_RETURN_IF(!d_c_region || !d_c_nation, FLS_ERR_NULL);
-        hipLaunchKernelGGL(gen_dim2_kernel, dim3(grid_for(n_cust)), dim3(256), 0, s, stream_key(seed, ST_CREG), 5u,
-                           stream_key(seed, ST_CNAT), 25u, n_cust, d_c_region, d_c_nation);
+        hipLaunchKernelGGL(gen_geo_kernel, dim3(grid_for(n_cust)), dim3(256), 0, s, stream_key(seed, ST_CNAT), n_cust,
+                           d_c_region, d_c_nation);
     }
     return check_launch();
 }

@@ -152,20 +152,85 @@ __global__ __launch_bounds__(256) void q21_pht_kernel(const Q21Dev a) {
 
 // Compact-table probe.  1024 work-items = 32 vectors per step; LDS holds the
 // supplier and part bitmaps and the year table when they fit (USE_LDS).
+// Per-block LDS hash table of group slots.  A block meets far fewer groups
+// than the slot space holds (SSB Q2.1: 40 brands x 7 years of one category
+// among 7,000 slots), so rows are summed here and each block adds its groups
+// to the global table once.  Open addressing over kHash entries with linear
+// probing: key = slot + 1 (0 = empty), claimed with atomicCAS; a slot that
+// finds no entry within kProbe probes goes to the global table directly, so
+// any number of groups per block stays exact.
+constexpr uint32_t kHashBits = 10, kHash = 1u << kHashBits, kProbe = 32;
+constexpr size_t kHashBytes = kHash * (4u + 8u + 8u);
+
+struct GroupHash {
+    uint32_t* key;
+    unsigned long long *sum, *cnt;
+};
+
+__device__ __forceinline__ GroupHash hash_init(uint32_t* base) {
+    GroupHash h;
+    h.sum = reinterpret_cast<unsigned long long*>(base);  // 8-byte aligned
+    h.cnt = h.sum + kHash;
+    h.key = reinterpret_cast<uint32_t*>(h.cnt + kHash);
+    for (uint32_t i = threadIdx.x; i < kHash; i += blockDim.x) {
+        h.key[i] = 0u;
+        h.sum[i] = 0ull;
+        h.cnt[i] = 0ull;
+    }
+    return h;
+}
+
+__device__ __forceinline__ void hash_group_add(const GroupHash& h, const Q21Dev& a, uint32_t slot, uint32_t rev) {
+    const uint32_t key = slot + 1u;
+    uint32_t i = (slot * 0x9E3779B1u) >> (32u - kHashBits);
+#pragma unroll 1
+    for (uint32_t probe = 0; probe < kProbe; ++probe) {
+        uint32_t k = __atomic_load_n(&h.key[i], __ATOMIC_RELAXED);
+        if (k == 0u) {
+            const uint32_t old = atomicCAS(&h.key[i], 0u, key);
+            k = old == 0u ? key : old;
+        }
+        if (k == key) {
+            atomicAdd(&h.sum[i], (unsigned long long)rev);
+            atomicAdd(&h.cnt[i], 1ull);
+            return;
+        }
+        i = (i + 1u) & (kHash - 1u);
+    }
+    atomicAdd((unsigned long long*)&a.sums[slot], (unsigned long long)rev);
+    atomicAdd((unsigned long long*)&a.counts[slot], 1ull);
+}
+
+// Between the block's barriers: every claimed entry added to the global table.
+__device__ __forceinline__ void hash_flush(const GroupHash& h, const Q21Dev& a) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kHash; i += blockDim.x) {
+        const uint32_t k = h.key[i];
+        if (k) {
+            atomicAdd((unsigned long long*)&a.sums[k - 1u], h.sum[i]);
+            atomicAdd((unsigned long long*)&a.counts[k - 1u], h.cnt[i]);
+        }
+    }
+}
+
 // Per-wave LDS queue of rows that passed every filter: part index, date
 // index and revenue, kQueue of each.  The flush runs with one lane per row:
-// brand from L2, year from the year table, 64-bit group atomics.
+// brand from L2, year from the year table, then the block's hash table.
 constexpr uint32_t kQueue = 64;
 constexpr size_t kQueueBytes = 16ull * 3u * kQueue * 4u;  // 16 waves of a 1024-item block
 
-__device__ __forceinline__ void q21_flush(const Q21Dev& a, uint32_t* q, uint32_t qn, const uint16_t* year_tab) {
+__device__ __forceinline__ void q21_flush(const Q21Dev& a, uint32_t* q, uint32_t qn, const uint16_t* year_tab,
+                                          const GroupHash& h) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const uint32_t l = threadIdx.x & 63u;
     if (l < qn) {
         const uint32_t p = q[l], d = q[kQueue + l];
-        if (d < a.n_date) group_add(a, a.brand[p], year_tab[d], q[2 * kQueue + l]);
+        if (d < a.n_date) {
+            const uint32_t slot = fast_mod(a.brand[p] * a.n_years + (year_tab[d] - a.year_base), a.slot_div);
+            hash_group_add(h, a, slot, q[2 * kQueue + l]);
+        }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -174,10 +239,12 @@ __device__ __forceinline__ void q21_flush(const Q21Dev& a, uint32_t* q, uint32_t
 
 template <int DBW, int SBW, int PBW, int RBW, bool USE_LDS>
 __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
-    extern __shared__ uint32_t lds[];
-    // the queues first (4-byte aligned at offset 0), then the staged tables
-    uint32_t* q = lds + (threadIdx.x >> 6) * 3u * kQueue;
-    uint32_t* stage = lds + kQueueBytes / 4u;
+    extern __shared__ __align__(16) uint32_t lds[];
+    // the group hash table first (8-byte fields at offset 0), then the
+    // queues, then the staged tables
+    const GroupHash h = hash_init(lds);
+    uint32_t* q = lds + kHashBytes / 4u + (threadIdx.x >> 6) * 3u * kQueue;
+    uint32_t* stage = lds + (kHashBytes + kQueueBytes) / 4u;
     const uint32_t* supp_bm = a.supp_bm;
     const uint32_t* part_bm = a.part_bm;
     const uint16_t* year_tab = a.year;
@@ -188,11 +255,11 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
         for (uint32_t i = threadIdx.x; i < a.supp_words; i += blockDim.x) s[i] = a.supp_bm[i];
         for (uint32_t i = threadIdx.x; i < a.part_words; i += blockDim.x) p[i] = a.part_bm[i];
         for (uint32_t i = threadIdx.x; i < a.n_date; i += blockDim.x) y[i] = a.year[i];
-        __syncthreads();
         supp_bm = s;
         part_bm = p;
         year_tab = y;
     }
+    __syncthreads();
     // rows of the (possibly partial) last vector, computed once
     const uint32_t tail =
         live_row_mask((uint32_t)(a.n_rows - (a.n_vec - 1) * 1024u), a.rev.transposed != 0);
@@ -251,7 +318,7 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
             if (bal == 0) continue;
             const uint32_t cnt = (uint32_t)__popcll(bal);
             if (qn + cnt > kQueue) {
-                q21_flush(a, q, qn, year_tab);
+                q21_flush(a, q, qn, year_tab, h);
                 qn = 0;
             }
             if (live) {
@@ -263,8 +330,9 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
             }
             qn += cnt;
         }
-        if (qn) q21_flush(a, q, qn, year_tab);
+        if (qn) q21_flush(a, q, qn, year_tab, h);
     }
+    hash_flush(h, a);
 }
 
 // Two 16-bit values per register, row r in half r & 1 of register r >> 1.
@@ -351,7 +419,7 @@ int launch_resident_q21(Kernel k, int block, size_t lds, const Q21Dev& a, hipStr
 template <int DBW, int SBW, int PBW, int RBW>
 int launch_compact(const Q21Dev& a, bool in_lds, size_t lds, hipStream_t s) {
     auto k = in_lds ? q21_compact_kernel<DBW, SBW, PBW, RBW, true> : q21_compact_kernel<DBW, SBW, PBW, RBW, false>;
-    const size_t bytes = kQueueBytes + (in_lds ? lds : 0);
+    const size_t bytes = kHashBytes + kQueueBytes + (in_lds ? lds : 0);
     FLS_HIP_CHECK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     return launch_resident_q21(k, 1024, bytes, a, s);
 }
@@ -487,7 +555,7 @@ int fls_query_q21(const fls_column* datekey, const fls_column* suppkey, const fl
     a.part_words = L.part_words;
     a.n_groups = ceil_div(a.n_vec, 32);
     const size_t lds = 4ull * (L.supp_words + L.part_words) + 2ull * dims->n_date + 16u;
-    const bool in_lds = lds + kQueueBytes <= 150u * 1024u;
+    const bool in_lds = lds + kHashBytes + kQueueBytes <= 150u * 1024u;
     const bool fast = a.date.enc == FLS_ENC_BITPACK && a.supp.enc == FLS_ENC_BITPACK &&
                       a.part.enc == FLS_ENC_BITPACK && a.rev.enc == FLS_ENC_BITPACK && a.date.bw == 12 &&
                       a.supp.bw == 16 && a.part.bw == 20 && a.rev.bw == 24;

@@ -583,8 +583,9 @@ void fls_batch_destroy(fls_batch* b);
  *   c3: fact foreign keys as 0-based dense keys, date U[0,2556),
  *       supplier U[0,40000), part U[0,800000), customer U[0,600000) (may be
  *       NULL), revenue U[0,10^7); dimensions: year of each date row, supplier
- *       region U{1..5} / nation U{1..25}, part category U{1..25} / brand
- *       U{1..1000}, customer region / nation.
+ *       and customer nation U{1..25} with region (nation - 1) / 5 + 1, part
+ *       category U{1..25} with brand 40 (category - 1) + U{1..40} -- SSB's
+ *       nation -> region and category -> brand hierarchies.
  *   star extra (Q3.1 / Q4.1 columns the c3 table lacks): supplycost of each
  *       fact row U[60000, 120000), manufacturer of each part U{1..5}.       */
 int fls_gen_c1(uint64_t seed, uint64_t n, uint32_t bw, uint32_t* d_out, void* stream);

@@ -284,17 +284,19 @@ void orc_gen_c3_dims(uint64_t seed, uint32_t n_date, uint32_t n_supp, uint32_t n
                      uint32_t* d_year, uint32_t* s_region, uint32_t* s_nation, uint32_t* p_category,
                      uint32_t* p_brand, uint32_t* c_region, uint32_t* c_nation) {
     for (uint32_t d = 0; d < n_date; ++d) d_year[d] = orc_yyyymmdd(d) / 10000;
+    /* SSB's hierarchies: a nation lies in one region (5 nations each), a brand
+     * in one category (40 brands each, brand1 = category x 40 + 1..40) */
     for (uint32_t s = 0; s < n_supp; ++s) {
-        s_region[s] = 1 + orc_uniform(seed, ST_SREG, s, 5);
         s_nation[s] = 1 + orc_uniform(seed, ST_SNAT, s, 25);
+        s_region[s] = (s_nation[s] - 1) / 5 + 1;
     }
     for (uint32_t p = 0; p < n_part; ++p) {
         p_category[p] = 1 + orc_uniform(seed, ST_PCAT, p, 25);
-        p_brand[p] = 1 + orc_uniform(seed, ST_PBRAND, p, 1000);
+        p_brand[p] = (p_category[p] - 1) * 40 + 1 + orc_uniform(seed, ST_PBRAND, p, 40);
     }
     for (uint32_t c = 0; c < n_cust; ++c) {
-        c_region[c] = 1 + orc_uniform(seed, ST_CREG, c, 5);
         c_nation[c] = 1 + orc_uniform(seed, ST_CNAT, c, 25);
+        c_region[c] = (c_nation[c] - 1) / 5 + 1;
     }
 }
 

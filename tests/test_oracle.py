@@ -255,8 +255,9 @@ def test_q31_brute_force():
         key = ((v - cn) // 25, cn, y)
         got[key] = int(sums[slot])
         got_n[key] = int(counts[slot])
-    assert got == want and got_n == want_n and len(want) > 200
+    assert got == want and got_n == want_n and len(want) > 100
     assert max(k[2] for k in want) == 1997  # 1998 is filtered out
+    assert {k[0] for k in want} | {k[1] for k in want} <= set(range(11, 16))  # region 3's nations
 
 
 def test_q41_brute_force():
@@ -273,7 +274,8 @@ def test_q41_brute_force():
     for slot in np.nonzero(counts)[0]:
         cn = int(slot) // 7
         got[(cn if cn > 0 else 25, 1992 + int(slot) % 7)] = int(sums[slot])
-    assert got == want and len(want) > 50
+    assert got == want and len(want) > 25
+    assert {k[0] for k in want} <= set(range(6, 11))  # region 2's nations
     assert (f["revenue"] < f["supplycost"]).any()  # some rows add a negative profit
 
 
@@ -284,6 +286,19 @@ def test_star_extra_ranges():
     assert set(np.unique(d["p_mfgr"]).tolist()) == {1, 2, 3, 4, 5}
     # custfk is drawn by the same generator as the Q2.1 table's keys
     assert f["custfk"].max() < 600000 and len(np.unique(f["custfk"])) > 90_000
+
+
+def test_c3_dims_follow_ssb_hierarchies():
+    # a nation lies in one region, a brand in one category (SSB dbgen's
+    # nation -> region and p_category -> p_brand1)
+    _, d = T.o_gen_c3(4, 1, n_supp=20_000, n_part=50_000, n_cust=20_000)
+    for pre in ("s", "c"):
+        nat, reg = d[f"{pre}_nation"], d[f"{pre}_region"]
+        assert set(np.unique(nat).tolist()) == set(range(1, 26))
+        np.testing.assert_array_equal(reg, (nat - 1) // 5 + 1)
+    cat, brand = d["p_category"], d["p_brand"]
+    assert set(np.unique(cat).tolist()) == set(range(1, 26))
+    assert ((brand - 1) // 40 + 1 == cat).all() and set(np.unique(brand).tolist()) == set(range(1, 1001))
 
 
 def test_rle_roundtrip_and_run_count():
