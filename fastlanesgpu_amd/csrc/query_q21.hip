@@ -25,6 +25,7 @@
 #include "fls/device/wave.hpp"
 #include "fls_columns.hpp"
 #include "fls_fastdiv.hpp"
+#include "fls_group_hash.hpp"
 #include "fls_runtime.hpp"
 
 namespace fls {
@@ -152,66 +153,7 @@ __global__ __launch_bounds__(256) void q21_pht_kernel(const Q21Dev a) {
 
 // Compact-table probe.  1024 work-items = 32 vectors per step; LDS holds the
 // supplier and part bitmaps and the year table when they fit (USE_LDS).
-// Per-block LDS hash table of group slots.  A block meets far fewer groups
-// than the slot space holds (SSB Q2.1: 40 brands x 7 years of one category
-// among 7,000 slots), so rows are summed here and each block adds its groups
-// to the global table once.  Open addressing over kHash entries with linear
-// probing: key = slot + 1 (0 = empty), claimed with atomicCAS; a slot that
-// finds no entry within kProbe probes goes to the global table directly, so
-// any number of groups per block stays exact.
-constexpr uint32_t kHashBits = 10, kHash = 1u << kHashBits, kProbe = 32;
-constexpr size_t kHashBytes = kHash * (4u + 8u + 8u);
-
-struct GroupHash {
-    uint32_t* key;
-    unsigned long long *sum, *cnt;
-};
-
-__device__ __forceinline__ GroupHash hash_init(uint32_t* base) {
-    GroupHash h;
-    h.sum = reinterpret_cast<unsigned long long*>(base);  // 8-byte aligned
-    h.cnt = h.sum + kHash;
-    h.key = reinterpret_cast<uint32_t*>(h.cnt + kHash);
-    for (uint32_t i = threadIdx.x; i < kHash; i += blockDim.x) {
-        h.key[i] = 0u;
-        h.sum[i] = 0ull;
-        h.cnt[i] = 0ull;
-    }
-    return h;
-}
-
-__device__ __forceinline__ void hash_group_add(const GroupHash& h, const Q21Dev& a, uint32_t slot, uint32_t rev) {
-    const uint32_t key = slot + 1u;
-    uint32_t i = (slot * 0x9E3779B1u) >> (32u - kHashBits);
-#pragma unroll 1
-    for (uint32_t probe = 0; probe < kProbe; ++probe) {
-        uint32_t k = __atomic_load_n(&h.key[i], __ATOMIC_RELAXED);
-        if (k == 0u) {
-            const uint32_t old = atomicCAS(&h.key[i], 0u, key);
-            k = old == 0u ? key : old;
-        }
-        if (k == key) {
-            atomicAdd(&h.sum[i], (unsigned long long)rev);
-            atomicAdd(&h.cnt[i], 1ull);
-            return;
-        }
-        i = (i + 1u) & (kHash - 1u);
-    }
-    atomicAdd((unsigned long long*)&a.sums[slot], (unsigned long long)rev);
-    atomicAdd((unsigned long long*)&a.counts[slot], 1ull);
-}
-
-// Between the block's barriers: every claimed entry added to the global table.
-__device__ __forceinline__ void hash_flush(const GroupHash& h, const Q21Dev& a) {
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < kHash; i += blockDim.x) {
-        const uint32_t k = h.key[i];
-        if (k) {
-            atomicAdd((unsigned long long*)&a.sums[k - 1u], h.sum[i]);
-            atomicAdd((unsigned long long*)&a.counts[k - 1u], h.cnt[i]);
-        }
-    }
-}
+using Q21Hash = GroupHash<10>;  // 1,024 entries, 20 KB
 
 // Per-wave LDS queue of rows that passed every filter: part index, date
 // index and revenue, kQueue of each.  The flush runs with one lane per row:
@@ -220,7 +162,7 @@ constexpr uint32_t kQueue = 64;
 constexpr size_t kQueueBytes = 16ull * 3u * kQueue * 4u;  // 16 waves of a 1024-item block
 
 __device__ __forceinline__ void q21_flush(const Q21Dev& a, uint32_t* q, uint32_t qn, const uint16_t* year_tab,
-                                          const GroupHash& h) {
+                                          const Q21Hash& h) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -229,7 +171,7 @@ __device__ __forceinline__ void q21_flush(const Q21Dev& a, uint32_t* q, uint32_t
         const uint32_t p = q[l], d = q[kQueue + l];
         if (d < a.n_date) {
             const uint32_t slot = fast_mod(a.brand[p] * a.n_years + (year_tab[d] - a.year_base), a.slot_div);
-            hash_group_add(h, a, slot, q[2 * kQueue + l]);
+            h.add(slot, q[2 * kQueue + l], a.sums, a.counts);
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -242,9 +184,10 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
     extern __shared__ __align__(16) uint32_t lds[];
     // the group hash table first (8-byte fields at offset 0), then the
     // queues, then the staged tables
-    const GroupHash h = hash_init(lds);
-    uint32_t* q = lds + kHashBytes / 4u + (threadIdx.x >> 6) * 3u * kQueue;
-    uint32_t* stage = lds + (kHashBytes + kQueueBytes) / 4u;
+    Q21Hash h;
+    h.init(lds);
+    uint32_t* q = lds + Q21Hash::kBytes / 4u + (threadIdx.x >> 6) * 3u * kQueue;
+    uint32_t* stage = lds + (Q21Hash::kBytes + kQueueBytes) / 4u;
     const uint32_t* supp_bm = a.supp_bm;
     const uint32_t* part_bm = a.part_bm;
     const uint16_t* year_tab = a.year;
@@ -332,7 +275,7 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
         }
         if (qn) q21_flush(a, q, qn, year_tab, h);
     }
-    hash_flush(h, a);
+    h.flush(a.sums, a.counts);
 }
 
 // Two 16-bit values per register, row r in half r & 1 of register r >> 1.
@@ -419,7 +362,7 @@ int launch_resident_q21(Kernel k, int block, size_t lds, const Q21Dev& a, hipStr
 template <int DBW, int SBW, int PBW, int RBW>
 int launch_compact(const Q21Dev& a, bool in_lds, size_t lds, hipStream_t s) {
     auto k = in_lds ? q21_compact_kernel<DBW, SBW, PBW, RBW, true> : q21_compact_kernel<DBW, SBW, PBW, RBW, false>;
-    const size_t bytes = kHashBytes + kQueueBytes + (in_lds ? lds : 0);
+    const size_t bytes = Q21Hash::kBytes + kQueueBytes + (in_lds ? lds : 0);
     FLS_HIP_CHECK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     return launch_resident_q21(k, 1024, bytes, a, s);
 }
@@ -555,7 +498,7 @@ int fls_query_q21(const fls_column* datekey, const fls_column* suppkey, const fl
     a.part_words = L.part_words;
     a.n_groups = ceil_div(a.n_vec, 32);
     const size_t lds = 4ull * (L.supp_words + L.part_words) + 2ull * dims->n_date + 16u;
-    const bool in_lds = lds + kHashBytes + kQueueBytes <= 150u * 1024u;
+    const bool in_lds = lds + Q21Hash::kBytes + kQueueBytes <= 150u * 1024u;
     const bool fast = a.date.enc == FLS_ENC_BITPACK && a.supp.enc == FLS_ENC_BITPACK &&
                       a.part.enc == FLS_ENC_BITPACK && a.rev.enc == FLS_ENC_BITPACK && a.date.bw == 12 &&
                       a.supp.bw == 16 && a.part.bw == 20 && a.rev.bw == 24;

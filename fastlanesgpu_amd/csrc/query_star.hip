@@ -2,16 +2,17 @@
 // (reference: probe_v1 and build_hashtable_* of fastlanes/src/ssb/fls_q31.cu
 // and fls_q41.cu).  MI355X-native: the dimension tables are reduced once into
 // a compact workspace (one byte per supplier / customer / date: its nation or
-// year offset, 0xFF when the row is filtered out; one bit per part), the
-// probe kernel stages the small ones in LDS and keeps the customer codes
-// (600 KB) in L2, and the group-by runs in a per-block LDS table (4,375 or
-// 175 slots) flushed once per block: the ~4 M (Q3.1) / ~2 M (Q4.1) qualifying
-// rows of 120 M land in about 150 / 35 live slots, which global per-row
-// atomics would serialise on.
+// year offset, 0xFF when the row is filtered out; one bit per part and per
+// qualifying customer), the probe kernel stages the small ones in LDS and
+// keeps the customer codes (600 KB) in L2, and the group-by runs in a
+// per-block LDS table (4,375 or 175 slots, or a hash table) flushed once per
+// block: the ~4 M (Q3.1) / ~2 M (Q4.1) qualifying rows of 120 M land in about
+// 150 / 35 live slots, which global per-row atomics would serialise on.
 #include "fls/device/unpack.hpp"
 #include "fls/device/wave.hpp"
 #include "fls_columns.hpp"
 #include "fls_fastdiv.hpp"
+#include "fls_group_hash.hpp"
 #include "fls_runtime.hpp"
 
 namespace fls {
@@ -22,19 +23,21 @@ constexpr uint32_t kMaxSlots = 8192;
 
 // workspace layout (byte offsets, 256-aligned)
 struct StarLayout {
-    uint64_t s_code, c_code, p_bits, y_code, total;
-    uint32_t p_words;
+    uint64_t s_code, c_code, p_bits, y_code, c_bits, total;
+    uint32_t p_words, c_words;
 };
 
 StarLayout star_layout(const fls_star_dims& d) {
     auto up = [](uint64_t x) { return (x + 255u) & ~(uint64_t)255u; };
     StarLayout L{};
     L.p_words = (d.n_part + 31u) / 32u;
+    L.c_words = (d.n_cust + 31u) / 32u;
     L.s_code = 0;
     L.c_code = up(L.s_code + d.n_supp);
     L.p_bits = up(L.c_code + d.n_cust);
     L.y_code = up(L.p_bits + 4ull * L.p_words);
-    L.total = up(L.y_code + d.n_date);
+    L.c_bits = up(L.y_code + d.n_date);
+    L.total = up(L.c_bits + 4ull * L.c_words);
     return L;
 }
 
@@ -43,14 +46,19 @@ __global__ void star_zero_words(uint32_t* __restrict w, uint32_t n) {
 }
 
 // Q3.1 builds (build_hashtable_s/_c/_d, fls_q31.cu:300-372): nation of the
-// qualifying suppliers and customers, year offset of the dates in range
+// qualifying suppliers and customers (and a bit per qualifying customer;
+// c_bits zeroed before), year offset of the dates in range
 __global__ void q31_build_kernel(fls_star_dims d, fls_q31_params p, uint8_t* __restrict s_code,
-                                 uint8_t* __restrict c_code, uint8_t* __restrict y_code) {
+                                 uint8_t* __restrict c_code, uint8_t* __restrict y_code,
+                                 uint32_t* __restrict c_bits) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_supp; i += stride)
         s_code[i] = (d.s_region[i] == p.s_region && d.s_nation[i] < kNone) ? (uint8_t)d.s_nation[i] : kNone;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_cust; i += stride)
-        c_code[i] = (d.c_region[i] == p.c_region && d.c_nation[i] < kNone) ? (uint8_t)d.c_nation[i] : kNone;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_cust; i += stride) {
+        const bool hit = d.c_region[i] == p.c_region && d.c_nation[i] < kNone;
+        c_code[i] = hit ? (uint8_t)d.c_nation[i] : kNone;
+        if (hit) atomicOr(&c_bits[i >> 5], 1u << (i & 31u));
+    }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_date; i += stride) {
         const uint32_t y = d.d_year[i];
         y_code[i] = (y >= p.year_lo && y <= p.year_hi) ? (uint8_t)(y - p.year_base) : kNone;
@@ -79,8 +87,8 @@ struct StarDev {
     ColDev date, cust, supp, part, rev, cost;
     uint64_t n_rows, n_vec, n_groups;
     const uint8_t *s_code, *c_code, *y_code;
-    const uint32_t* p_bits;
-    uint32_t n_supp, n_cust, n_part, n_date, keys_min, p_words;
+    const uint32_t *p_bits, *c_bits;
+    uint32_t n_supp, n_cust, n_part, n_date, keys_min, p_words, c_words;
     uint32_t slot_mul_s, slot_mul_c, n_slots;  // slot = (s*slot_mul_s + c*slot_mul_c + y) % n_slots
     FastDiv slot_div;                          // n_slots
     int64_t* sums;
@@ -102,6 +110,7 @@ struct StarLds {
 
 template <bool USE_LDS, bool PARTS>
 __device__ __forceinline__ StarLds star_stage(const StarDev& a, uint8_t* lds) {
+    // 16-byte aligned (the caller's reserve, if any, is a multiple of 16)
     StarLds t;
     t.sum = reinterpret_cast<unsigned long long*>(lds);
     t.cnt = reinterpret_cast<uint32_t*>(lds + 8ull * a.n_slots);
@@ -207,56 +216,153 @@ __device__ __forceinline__ void group_add_lds(const StarLds& t, uint32_t slot, u
     atomicAdd(&t.cnt[slot], 1u);
 }
 
-// Q3.1 probe at fixed widths: all four columns' loads are issued before the
-// first probe (bytes in flight); supplier nation (LDS) -> customer nation
-// (L2) -> year offset (LDS) -> LDS group table.
-template <int DBW, int CBW, int SBW, int RBW, bool USE_LDS>
+// Q3.1 probe at fixed widths, three ways of holding the dimensions:
+//  kQ31Codes  group table in LDS, supplier / date codes in global memory;
+//  kQ31Staged group table, supplier and date codes in LDS;
+//  kQ31Bitmap supplier and date codes, a customer filter bitmap and a group
+//             hash table in LDS (SSB sizes: 40 KB + 2.5 KB + 75 KB + 10 KB).
+// All four columns' loads are issued before the first probe (bytes in
+// flight); the supplier nation (one byte per supplier) is looked up on every
+// row, and the survivors (about one in five) go to a per-wave LDS queue:
+// customer key, date key | supplier nation << 24, revenue.  The flush takes
+// one queued row per lane, two rows per lane in flight: year offset,
+// customer nation, group.  The customer lookups are L2 gathers of a 600 KB
+// table, each a distinct cache line: at 24 M of them the L2 request rate
+// bounds the kernel, so kQ31Bitmap first tests the customer's bit in LDS and
+// gathers the nation only for the rows that pass (one in five again).
+enum Q31Mode { kQ31Codes = 0, kQ31Staged = 1, kQ31Bitmap = 2 };
+constexpr uint32_t kQ31Queue = 128;
+constexpr size_t kQ31QueueBytes = (size_t)kStarVecs / 2u * 3u * kQ31Queue * 4u;  // 16 waves, 24 KB
+using Q31Hash = GroupHash<9>;  // 512 entries, 10 KB
+
+struct Q31Lds {
+    StarLds t;              // kQ31Codes / kQ31Staged: group table and code pointers
+    const uint32_t* c_bits;  // kQ31Bitmap
+    Q31Hash h;               // kQ31Bitmap
+};
+
+template <int MODE>
+__device__ __forceinline__ Q31Lds q31_stage(const StarDev& a, uint8_t* lds) {
+    Q31Lds s{};
+    if constexpr (MODE == kQ31Bitmap) {
+        s.h.init(lds);
+        uint32_t* cb = reinterpret_cast<uint32_t*>(lds + Q31Hash::kBytes);
+        uint8_t* sc = reinterpret_cast<uint8_t*>(cb + a.c_words);
+        uint8_t* yc = sc + a.n_supp;
+        for (uint32_t i = threadIdx.x; i < a.c_words; i += blockDim.x) cb[i] = a.c_bits[i];
+        for (uint32_t i = threadIdx.x; i < a.n_supp; i += blockDim.x) sc[i] = a.s_code[i];
+        for (uint32_t i = threadIdx.x; i < a.n_date; i += blockDim.x) yc[i] = a.y_code[i];
+        s.c_bits = cb;
+        s.t.s_code = sc;
+        s.t.y_code = yc;
+        __syncthreads();
+    } else {
+        s.t = star_stage<MODE == kQ31Staged, false>(a, lds);
+    }
+    return s;
+}
+
+template <int MODE>
+__device__ __forceinline__ void q31_flush(const StarDev& a, const Q31Lds& s, const uint32_t* q, uint32_t qn) {
+    constexpr int kPer = kQ31Queue / 64;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t l = threadIdx.x & 63u;
+    uint32_t ck[kPer], dk[kPer], cv[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const uint32_t i = l + 64u * j;
+        const uint32_t x = q[i];
+        ck[j] = i < qn ? x : 0xFFFFFFFFu;  // rows past the queue: out of range, dropped
+        dk[j] = q[kQ31Queue + i];
+    }
+    if constexpr (MODE == kQ31Bitmap) {
+#pragma unroll
+        for (int j = 0; j < kPer; ++j)
+            if (!bit_at(s.c_bits, ck[j], 0u, a.n_cust)) ck[j] = 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) cv[j] = code_at(a.c_code, ck[j], 0u, a.n_cust);
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const uint32_t i = l + 64u * j;
+        const uint32_t y = code_at(s.t.y_code, dk[j] & 0xFFFFFFu, 0u, a.n_date);
+        if (cv[j] != kNone && y != kNone) {
+            const uint32_t slot = fast_mod((dk[j] >> 24) * a.slot_mul_s + cv[j] * a.slot_mul_c + y, a.slot_div);
+            if constexpr (MODE == kQ31Bitmap)
+                s.h.add(slot, q[2u * kQ31Queue + i], a.sums, a.counts);
+            else
+                group_add_lds(s.t, slot, q[2u * kQ31Queue + i]);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Date keys are clamped to 24 bits in the queue (n_date < 2^24).
+template <int DBW, int CBW, int SBW, int RBW, int MODE>
 __global__ __launch_bounds__(kStarBlock) void q31_kernel(const StarDev a) {
     extern __shared__ __align__(16) uint8_t lds[];
-    const StarLds t = star_stage<USE_LDS, false>(a, lds);
+    uint32_t* q = reinterpret_cast<uint32_t*>(lds) + (threadIdx.x >> 6) * 3u * kQ31Queue;
+    const Q31Lds s = q31_stage<MODE>(a, lds + kQ31QueueBytes);
     const uint32_t tail = live_row_mask((uint32_t)(a.n_rows - (a.n_vec - 1) * 1024u), a.rev.transposed != 0);
     const uint32_t km = a.keys_min;
     for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
         const uint64_t v = g * kStarVecs + (threadIdx.x >> 5);
-        if (v >= a.n_vec) continue;
+        // the whole wave stays in step (the queue flush needs every lane): a
+        // half-wave past the last vector reads the last vector again (in
+        // bounds) and keeps no rows
+        const bool vlive = v < a.n_vec;
+        if (!__any(vlive)) continue;
+        const uint64_t vl = vlive ? v : a.n_vec - 1;
         LaneCol<SBW> sc;
         LaneCol<CBW> cc;
         LaneCol<DBW> dc;
         LaneCol<RBW> rc;
-        sc.load(a.supp, v);
-        cc.load(a.cust, v);
-        dc.load(a.date, v);
-        rc.load(a.rev, v);
+        sc.load(a.supp, vl);
+        cc.load(a.cust, vl);
+        dc.load(a.date, vl);
+        rc.load(a.rev, vl);
         uint32_t m = 0, s8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
-            const uint32_t s = code_at(t.s_code, sc.get(r), km, a.n_supp);
-            put8(s8, r, s);
-            m |= (uint32_t)(s != kNone) << r;
+            const uint32_t sn = code_at(s.t.s_code, sc.get(r), km, a.n_supp);
+            put8(s8, r, sn);
+            m |= (uint32_t)(sn != kNone) << r;
         }
+        if (!vlive) m = 0;
         if (v == a.n_vec - 1) m &= tail;
         cc.pin();
         dc.pin();
         rc.pin();
         if (!__any(m != 0)) continue;
-        uint32_t c8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        filter_codes<kB31>(cc, a.c_code, km, a.n_cust, m, c8);
-        if (!__any(m != 0)) continue;
+        uint32_t qn = 0;  // wave-uniform; at most 64 rows join per row slot
 #pragma unroll
-        for (int rb = 0; rb < 32; rb += kB31) {
-            if (!__any((m & batch_mask<kB31>(rb)) != 0)) continue;
-            uint32_t yv[kB31];
-            lookup_batch<kB31>(dc, t.y_code, km, a.n_date, m, rb, kNone, yv);
-#pragma unroll
-            for (int j = 0; j < kB31; ++j) {
-                const int r = rb + j;
-                if (yv[j] != kNone)
-                    group_add_lds(t, fast_mod(get8(s8, r) * a.slot_mul_s + get8(c8, r) * a.slot_mul_c + yv[j], a.slot_div),
-                                  rc.get(r));
+        for (int r = 0; r < 32; ++r) {
+            if (qn > kQ31Queue - 64u) {
+                q31_flush<MODE>(a, s, q, qn);
+                qn = 0;
             }
+            const bool live = (m >> r) & 1u;
+            const uint64_t bal = __ballot(live);
+            if (live) {
+                const uint32_t pos =
+                    qn + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                const uint32_t dk = dc.get(r) - km;
+                q[pos] = cc.get(r) - km;
+                q[kQ31Queue + pos] = min(dk, 0xFFFFFFu) | (get8(s8, r) << 24);
+                q[2u * kQ31Queue + pos] = rc.get(r);
+            }
+            qn += (uint32_t)__popcll(bal);
         }
+        if (qn) q31_flush<MODE>(a, s, q, qn);
     }
-    star_flush(a, t);
+    if constexpr (MODE == kQ31Bitmap)
+        s.h.flush(a.sums, a.counts);
+    else
+        star_flush(a, s.t);
 }
 
 // Q3.1 probe over columns of any encoding and width: one column decoded at a
@@ -436,12 +542,14 @@ int star_workspace(StarDev& a, const fls_star_dims* dims, const void* d_ws) {
     a.s_code = reinterpret_cast<const uint8_t*>(ws + L.s_code);
     a.c_code = reinterpret_cast<const uint8_t*>(ws + L.c_code);
     a.p_bits = reinterpret_cast<const uint32_t*>(ws + L.p_bits);
+    a.c_bits = reinterpret_cast<const uint32_t*>(ws + L.c_bits);
     a.y_code = reinterpret_cast<const uint8_t*>(ws + L.y_code);
     a.n_supp = dims->n_supp;
     a.n_cust = dims->n_cust;
     a.n_part = dims->n_part;
     a.n_date = dims->n_date;
     a.p_words = L.p_words;
+    a.c_words = L.c_words;
     a.keys_min = dims->keys_min;
     return FLS_OK;
 }
@@ -461,6 +569,7 @@ int star_outputs(StarDev& a, uint32_t n_slots, int64_t* d_sums, uint64_t* d_coun
 }
 
 constexpr size_t kLdsBudget = 150u * 1024u;
+constexpr size_t kLdsMax = 160u * 1024u;  // gfx950: LDS per CU, and per block
 
 }  // namespace
 }  // namespace fls
@@ -484,8 +593,11 @@ int fls_q31_build(const fls_star_dims* dims, const fls_q31_params* params, void*
     const StarLayout L = star_layout(*dims);
     FLS_RETURN_IF(ws_bytes < L.total, FLS_ERR_CAPACITY);
     uint8_t* ws = static_cast<uint8_t*>(d_workspace);
-    hipLaunchKernelGGL(q31_build_kernel, dim3(1024), dim3(256), 0, as_stream(stream), *dims, *params, ws + L.s_code,
-                       ws + L.c_code, ws + L.y_code);
+    hipStream_t s = as_stream(stream);
+    uint32_t* c_bits = reinterpret_cast<uint32_t*>(ws + L.c_bits);
+    hipLaunchKernelGGL(star_zero_words, dim3(256), dim3(256), 0, s, c_bits, L.c_words);
+    hipLaunchKernelGGL(q31_build_kernel, dim3(1024), dim3(256), 0, s, *dims, *params, ws + L.s_code, ws + L.c_code,
+                       ws + L.y_code, c_bits);
     return check_launch();
 }
 
@@ -507,13 +619,17 @@ int fls_query_q31(const fls_column* datekey, const fls_column* custkey, const fl
     a.slot_mul_c = params->n_years;
     const size_t table = 12ull * a.n_slots;
     const size_t staged = table + dims->n_supp + dims->n_date;
+    const bool fast = all_bitpack(outs, 4) && a.date.bw == 12 && a.cust.bw == 20 && a.supp.bw == 16 &&
+                      a.rev.bw == 24 && dims->n_date < (1u << 24);
+    if (fast) {
+        const size_t bitmap = kQ31QueueBytes + Q31Hash::kBytes + 4ull * a.c_words + dims->n_supp + dims->n_date;
+        if (bitmap <= kLdsMax) return launch_star(q31_kernel<12, 20, 16, 24, kQ31Bitmap>, bitmap, a, s);
+        if (kQ31QueueBytes + staged <= kLdsMax)
+            return launch_star(q31_kernel<12, 20, 16, 24, kQ31Staged>, kQ31QueueBytes + staged, a, s);
+        return launch_star(q31_kernel<12, 20, 16, 24, kQ31Codes>, kQ31QueueBytes + table, a, s);
+    }
     const bool in_lds = staged <= kLdsBudget;
     const size_t lds = in_lds ? staged : table;
-    const bool fast = all_bitpack(outs, 4) && a.date.bw == 12 && a.cust.bw == 20 && a.supp.bw == 16 &&
-                      a.rev.bw == 24;
-    if (fast)
-        return in_lds ? launch_star(q31_kernel<12, 20, 16, 24, true>, lds, a, s)
-                      : launch_star(q31_kernel<12, 20, 16, 24, false>, lds, a, s);
     return in_lds ? launch_star(q31_any_kernel<true>, lds, a, s) : launch_star(q31_any_kernel<false>, lds, a, s);
 }
 

@@ -121,6 +121,46 @@ def test_dimensions_beyond_lds(sizes, widths):
     check(run_q41(f, d, sizes, n, enc), T.o_q41(f, d, n))
 
 
+def fold_q31(out, n_slots):
+    """A Q3.1 result over n_slots >= 4375 folded onto the oracle's 4375 slots:
+    every group value V = s*175 + c*7 + y is below 4,556, so it sits in slot V
+    of the larger table and in slot V % 4375 of the oracle's."""
+    sums, counts = np.zeros(4375, np.int64), np.zeros(4375, np.uint64)
+    np.add.at(sums, np.arange(n_slots) % 4375, out[:n_slots])
+    np.add.at(counts, np.arange(n_slots) % 4375, out[n_slots:].view(np.uint64))
+    return np.concatenate([sums, counts.view(np.int64)])
+
+
+@pytest.mark.parametrize("sizes,n_slots", [
+    # SSB widths, customer bitmap too large to stage: group table, supplier
+    # and date codes in LDS, customer nations from L2
+    (dict(n_date=2556, n_supp=40_000, n_part=8000, n_cust=1_000_000), 4375),
+    # and with 8,192 group slots only the group table fits in LDS
+    (dict(n_date=2556, n_supp=65_000, n_part=8000, n_cust=1_000_000), 8192),
+])
+def test_q31_fixed_widths_without_customer_bitmap(sizes, n_slots):
+    n = 60_000 + 11
+    f, d = small_table(35, n, sizes)
+    dd = dev_dims(d, sizes)
+    cols = [c.to_device() for c in encode(f, sizes, FAST)]
+    ws = fls.q31_build(dd)
+    out = fls.query_q31(cols[0], cols[1], cols[2], cols[4], n, dd, ws, params=fls.q31_params(n_slots=n_slots))
+    check(fold_q31(out.cpu().numpy(), n_slots), T.o_q31(f, d, n))
+
+
+def test_q31_more_groups_than_the_block_hash():
+    # every supplier and customer in the filtered region: 25 x 25 nations x 6
+    # years = 3,750 live groups, beyond the probe's 512-entry LDS hash table,
+    # so most groups take the global-atomic path
+    n = 200_000 + 3
+    f, d = small_table(36, n)
+    d["s_region"][:] = 3
+    d["c_region"][:] = 3
+    sums, counts = T.o_q31(f, d, n)
+    assert (counts > 0).sum() > 3000
+    check(run_q31(f, d, SMALL, n, encode(f, SMALL, FAST)), (sums, counts))
+
+
 ANY_LAYOUTS = [
     # (date, cust, supp, part, revenue, supplycost); keys are encoded 1-based
     ("delta", "for", "rle", "exhaustive", "bitpack_t", "for"),
