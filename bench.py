@@ -526,35 +526,49 @@ def bench_c5(seed, n_rows, n_train=1 << 22, reps=3):
     }
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(enc, n_rows, gpu_result):
-    """Oracle decode + query on the host cores (all threads, then one thread)."""
+    """Oracle decode + query on the host cores: all threads (one warm-up, the
+    median of 7 runs over the full table), then one thread (median of 5 runs
+    over the full table) -- about 10 s of CPU work in all."""
     sys.path.insert(0, str(ROOT / "tests"))
     import fls_testlib as T
 
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     n_vec = (n_rows + 1023) // 1024
+
+    def median_time(runs, nthreads, first=0, count=None):
+        times, res = [], None
+        for _ in range(runs):
+            t = time.perf_counter()
+            res = T.o_q11_encoded(enc, n_rows, first, n_vec if count is None else count, threads=nthreads)
+            times.append(time.perf_counter() - t)
+        return statistics.median(times), res
+
     T.o_q11_encoded(enc, n_rows, 0, min(n_vec, 1024), threads=threads)  # warm-up
-    runs, res = [], None
-    for _ in range(3):
-        t = time.perf_counter()
-        res = T.o_q11_encoded(enc, n_rows, threads=threads)
-        runs.append(time.perf_counter() - t)
-    t_all = statistics.median(runs)
-    # one thread on a bounded sample (1/16 of the vectors)
-    sample_vec = max(1, n_vec // 16)
-    t = time.perf_counter()
-    T.o_q11_encoded(enc, n_rows, 0, sample_vec, threads=1)
-    t_one = time.perf_counter() - t
+    t_all, res = median_time(7, threads)
+    t_one, res_one = median_time(5, 1)
     return {
         "value": n_rows / t_all,
         "unit": "rows/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"full {n_rows}-row table, median of 3 runs on {threads} threads "
+        "sample": f"full {n_rows}-row table, median of 7 runs on {threads} threads after one warm-up "
                   f"(scalar per-lane decode + predicate/sum loop)",
-        "single_core_rows_per_s": sample_vec * 1024 / t_one,
-        "single_core_sample": f"first {sample_vec} vectors on 1 thread",
-        "result_matches_gpu": res == gpu_result,
+        "cpu_model": cpu_model(),
+        "single_core_rows_per_s": n_rows / t_one,
+        "single_core_sample": f"full {n_rows}-row table, median of 5 runs on 1 thread",
+        "result_matches_gpu": res == gpu_result and res_one == gpu_result,
     }
 
 
