@@ -399,9 +399,12 @@ __global__ __launch_bounds__(kStarBlock) void q31_any_kernel(const StarDev a) {
     star_flush(a, t);
 }
 
-// Q4.1 probe at fixed widths: supplier and part pass (LDS) while the last four
-// columns load, then customer nation (L2) and year offset (LDS) in batches,
-// revenue - supplycost into the LDS group table.
+// Q4.1 probe at fixed widths: supplier and part pass (LDS), then the last four
+// columns' loads, customer nation (L2) and year offset (LDS) in batches,
+// revenue - supplycost into the LDS group table.  Issuing all six columns'
+// loads before the filter measured no faster (the 16 waves of a CU already
+// cover the second round trip) and, kept there by a scheduling barrier,
+// spilled 4 VGPRs and ran 5 % slower.
 template <int DBW, int CBW, int SBW, int PBW, int RBW, int KBW, bool USE_LDS>
 __global__ __launch_bounds__(kStarBlock) void q41_kernel(const StarDev a) {
     extern __shared__ __align__(16) uint8_t lds[];
