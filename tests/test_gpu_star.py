@@ -148,6 +148,17 @@ def test_q31_fixed_widths_without_customer_bitmap(sizes, n_slots):
     check(fold_q31(out.cpu().numpy(), n_slots), T.o_q31(f, d, n))
 
 
+def test_q31_revenue_with_a_base():
+    # a frame-of-reference base on the 24-bit revenue column: values above 24
+    # bits once the base is added back, carried whole through the row queue
+    n = 30_000 + 7
+    f, d = small_table(37, n)
+    f["revenue"] = (f["revenue"].astype(np.uint64) + (1 << 24)).astype(np.uint32)
+    enc = encode(f, SMALL, FAST)
+    enc[4] = fls.bitpack_column(f["revenue"], 24, base=1 << 24)
+    check(run_q31(f, d, SMALL, n, enc), T.o_q31(f, d, n))
+
+
 def test_q31_more_groups_than_the_block_hash():
     # every supplier and customer in the filtered region: 25 x 25 nations x 6
     # years = 3,750 live groups, beyond the probe's 512-entry LDS hash table,
