@@ -338,15 +338,17 @@ __global__ __launch_bounds__(kStarBlock) void q31_kernel(const StarDev a) {
         dc.pin();
         rc.pin();
         if (!__any(m != 0)) continue;
-        uint32_t qn = 0;  // wave-uniform; at most 64 rows join per row slot
+        uint32_t qn = 0;  // wave-uniform
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
-            if (qn > kQ31Queue - 64u) {
+            // flush only when this row slot's rows would not fit: the flushes
+            // run nearly full (a chain of round trips each, whatever their size)
+            const bool live = (m >> r) & 1u;
+            const uint64_t bal = __ballot(live);
+            if (qn + (uint32_t)__popcll(bal) > kQ31Queue) {
                 q31_flush<MODE>(a, s, q, qn);
                 qn = 0;
             }
-            const bool live = (m >> r) & 1u;
-            const uint64_t bal = __ballot(live);
             if (live) {
                 const uint32_t pos =
                     qn + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
