@@ -206,6 +206,7 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
     // rows of the (possibly partial) last vector, computed once
     const uint32_t tail =
         live_row_mask((uint32_t)(a.n_rows - (a.n_vec - 1) * 1024u), a.rev.transposed != 0);
+    uint32_t qn = 0;  // wave-uniform; the queue carries over from step to step
     for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
         const uint64_t v = g * 32u + (threadIdx.x >> 5);
         // the whole wave stays in step (the queue flush needs every lane): a
@@ -253,7 +254,6 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
         // group-by on the rows that passed both (about 17 per 2,048 here):
         // compacted into the wave's LDS queue, then looked up and added with
         // one lane per row -- one brand round trip per queue, not per row slot
-        uint32_t qn = 0;  // wave-uniform
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
             const bool live = (m >> r) & 1u;
@@ -273,8 +273,8 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
             }
             qn += cnt;
         }
-        if (qn) q21_flush(a, q, qn, year_tab, h);
     }
+    if (qn) q21_flush(a, q, qn, year_tab, h);
     h.flush(a.sums, a.counts);
 }
 

@@ -309,6 +309,7 @@ __global__ __launch_bounds__(kStarBlock) void q31_kernel(const StarDev a) {
     const Q31Lds s = q31_stage<MODE>(a, lds + kQ31QueueBytes);
     const uint32_t tail = live_row_mask((uint32_t)(a.n_rows - (a.n_vec - 1) * 1024u), a.rev.transposed != 0);
     const uint32_t km = a.keys_min;
+    uint32_t qn = 0;  // wave-uniform; the queue carries over from step to step
     for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
         const uint64_t v = g * kStarVecs + (threadIdx.x >> 5);
         // the whole wave stays in step (the queue flush needs every lane): a
@@ -338,7 +339,6 @@ __global__ __launch_bounds__(kStarBlock) void q31_kernel(const StarDev a) {
         dc.pin();
         rc.pin();
         if (!__any(m != 0)) continue;
-        uint32_t qn = 0;  // wave-uniform
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
             // flush only when this row slot's rows would not fit: the flushes
@@ -359,8 +359,8 @@ __global__ __launch_bounds__(kStarBlock) void q31_kernel(const StarDev a) {
             }
             qn += (uint32_t)__popcll(bal);
         }
-        if (qn) q31_flush<MODE>(a, s, q, qn);
     }
+    if (qn) q31_flush<MODE>(a, s, q, qn);
     if constexpr (MODE == kQ31Bitmap)
         s.h.flush(a.sums, a.counts);
     else
