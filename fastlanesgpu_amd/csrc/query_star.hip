@@ -402,26 +402,56 @@ __global__ __launch_bounds__(kStarBlock) void q31_any_kernel(const StarDev a) {
 }
 
 // Q4.1 probe at fixed widths: supplier and part pass (LDS), then the last four
-// columns' loads, customer nation (L2) and year offset (LDS) in batches,
-// revenue - supplycost into the LDS group table.  Issuing all six columns'
-// loads before the filter measured no faster (the 16 waves of a CU already
-// cover the second round trip) and, kept there by a scheduling barrier,
-// spilled 4 VGPRs and ran 5 % slower.
+// columns' loads; the rows that pass (about 8 %) go to a per-wave LDS queue
+// (customer key, date key, revenue, supplycost) that carries over from step
+// to step and is flushed when a row slot's rows would not fit: one queued row
+// per lane, customer nation (L2), year offset (LDS), revenue - supplycost
+// into the LDS group table.  Issuing all six columns' loads before the filter
+// measured no faster (the 16 waves of a CU already cover the second round
+// trip) and, kept there by a scheduling barrier, spilled 4 VGPRs and ran 5 %
+// slower.
+constexpr uint32_t kQ41Queue = 64;
+constexpr size_t kQ41QueueBytes = (size_t)kStarVecs / 2u * 4u * kQ41Queue * 4u;  // 16 waves, 16 KB
+
+__device__ __forceinline__ void q41_flush(const StarDev& a, const StarLds& t, const uint32_t* q, uint32_t qn) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t l = threadIdx.x & 63u;
+    const uint32_t x = q[l];
+    const uint32_t ck = l < qn ? x : 0xFFFFFFFFu;  // rows past the queue: out of range, dropped
+    const uint32_t cv = code_at(a.c_code, ck, 0u, a.n_cust);
+    const uint32_t y = code_at(t.y_code, q[kQ41Queue + l], 0u, a.n_date);
+    if (cv != kNone && y != kNone)
+        group_add_lds(t, fast_mod(cv * a.slot_mul_c + y, a.slot_div),
+                      (uint64_t)((int64_t)q[2u * kQ41Queue + l] - (int64_t)q[3u * kQ41Queue + l]));
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int DBW, int CBW, int SBW, int PBW, int RBW, int KBW, bool USE_LDS>
 __global__ __launch_bounds__(kStarBlock) void q41_kernel(const StarDev a) {
     extern __shared__ __align__(16) uint8_t lds[];
-    const StarLds t = star_stage<USE_LDS, true>(a, lds);
+    uint32_t* q = reinterpret_cast<uint32_t*>(lds) + (threadIdx.x >> 6) * 4u * kQ41Queue;
+    const StarLds t = star_stage<USE_LDS, true>(a, lds + kQ41QueueBytes);
     const uint32_t tail = live_row_mask((uint32_t)(a.n_rows - (a.n_vec - 1) * 1024u), a.rev.transposed != 0);
     const uint32_t km = a.keys_min;
+    uint32_t qn = 0;  // wave-uniform; the queue carries over from step to step
     for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
         const uint64_t v = g * kStarVecs + (threadIdx.x >> 5);
-        if (v >= a.n_vec) continue;
+        // the whole wave stays in step (the queue flush needs every lane): a
+        // half-wave past the last vector reads the last vector again (in
+        // bounds) and keeps no rows
+        const bool vlive = v < a.n_vec;
+        if (!__any(vlive)) continue;
+        const uint64_t vl = vlive ? v : a.n_vec - 1;
         uint32_t m = 0;
         {
             LaneCol<SBW> sc;
             LaneCol<PBW> pc;
-            sc.load(a.supp, v);
-            pc.load(a.part, v);
+            sc.load(a.supp, vl);
+            pc.load(a.part, vl);
 #pragma unroll
             for (int r = 0; r < 32; ++r) {
                 const uint32_t hit = (code_at(t.s_code, sc.get(r), km, a.n_supp) != kNone) &
@@ -429,35 +459,41 @@ __global__ __launch_bounds__(kStarBlock) void q41_kernel(const StarDev a) {
                 m |= hit << r;
             }
         }
+        if (!vlive) m = 0;
         if (v == a.n_vec - 1) m &= tail;
         LaneCol<CBW> cc;
         LaneCol<DBW> dc;
         LaneCol<RBW> rc;
         LaneCol<KBW> kc;
-        cc.load(a.cust, v);
-        dc.load(a.date, v);
-        rc.load(a.rev, v);
-        kc.load(a.cost, v);
+        cc.load(a.cust, vl);
+        dc.load(a.date, vl);
+        rc.load(a.rev, vl);
+        kc.load(a.cost, vl);
         cc.pin();
         dc.pin();
         rc.pin();
         kc.pin();
         if (!__any(m != 0)) continue;
 #pragma unroll
-        for (int rb = 0; rb < 32; rb += kB41) {
-            if (!__any((m & batch_mask<kB41>(rb)) != 0)) continue;
-            uint32_t cv[kB41], yv[kB41];
-            lookup_batch<kB41>(cc, a.c_code, km, a.n_cust, m, rb, kNone, cv);
-            lookup_batch<kB41>(dc, t.y_code, km, a.n_date, m, rb, kNone, yv);
-#pragma unroll
-            for (int j = 0; j < kB41; ++j) {
-                const int r = rb + j;
-                if (cv[j] != kNone && yv[j] != kNone)
-                    group_add_lds(t, fast_mod(cv[j] * a.slot_mul_c + yv[j], a.slot_div),
-                                  (uint64_t)((int64_t)rc.get(r) - (int64_t)kc.get(r)));
+        for (int r = 0; r < 32; ++r) {
+            const bool live = (m >> r) & 1u;
+            const uint64_t bal = __ballot(live);
+            if (qn + (uint32_t)__popcll(bal) > kQ41Queue) {
+                q41_flush(a, t, q, qn);
+                qn = 0;
             }
+            if (live) {
+                const uint32_t pos =
+                    qn + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                q[pos] = cc.get(r) - km;
+                q[kQ41Queue + pos] = dc.get(r) - km;
+                q[2u * kQ41Queue + pos] = rc.get(r);
+                q[3u * kQ41Queue + pos] = kc.get(r);
+            }
+            qn += (uint32_t)__popcll(bal);
         }
     }
+    if (qn) q41_flush(a, t, q, qn);
     star_flush(a, t);
 }
 
@@ -675,13 +711,16 @@ int fls_query_q41(const fls_column* datekey, const fls_column* custkey, const fl
     a.slot_mul_c = params->n_years;
     const size_t table = 12ull * a.n_slots;
     const size_t staged = table + 4ull * a.p_words + dims->n_supp + dims->n_date;
-    const bool in_lds = staged <= kLdsBudget;
-    const size_t lds = in_lds ? staged : table;
     const bool fast = all_bitpack(outs, 6) && a.date.bw == 12 && a.cust.bw == 20 && a.supp.bw == 16 &&
                       a.part.bw == 20 && a.rev.bw == 24 && a.cost.bw == 16;
-    if (fast)
-        return in_lds ? launch_star(q41_kernel<12, 20, 16, 20, 24, 16, true>, lds, a, s)
-                      : launch_star(q41_kernel<12, 20, 16, 20, 24, 16, false>, lds, a, s);
+    if (fast) {
+        const bool fits = kQ41QueueBytes + staged <= kLdsMax;
+        const size_t lds = kQ41QueueBytes + (fits ? staged : table);
+        return fits ? launch_star(q41_kernel<12, 20, 16, 20, 24, 16, true>, lds, a, s)
+                    : launch_star(q41_kernel<12, 20, 16, 20, 24, 16, false>, lds, a, s);
+    }
+    const bool in_lds = staged <= kLdsBudget;
+    const size_t lds = in_lds ? staged : table;
     return in_lds ? launch_star(q41_any_kernel<true>, lds, a, s) : launch_star(q41_any_kernel<false>, lds, a, s);
 }
 
