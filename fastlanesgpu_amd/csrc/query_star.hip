@@ -223,13 +223,14 @@ __device__ __forceinline__ void group_add_lds(const StarLds& t, uint32_t slot, u
 //             hash table in LDS (SSB sizes: 40 KB + 2.5 KB + 75 KB + 10 KB).
 // All four columns' loads are issued before the first probe (bytes in
 // flight); the supplier nation (one byte per supplier) is looked up on every
-// row, and the survivors (about one in five) go to a per-wave LDS queue:
-// customer key, date key | supplier nation << 24, revenue.  The flush takes
-// one queued row per lane, two rows per lane in flight: year offset,
-// customer nation, group.  The customer lookups are L2 gathers of a 600 KB
-// table, each a distinct cache line: at 24 M of them the L2 request rate
-// bounds the kernel, so kQ31Bitmap first tests the customer's bit in LDS and
-// gathers the nation only for the rows that pass (one in five again).
+// row, and kQ31Bitmap also tests every row's customer bit in LDS.  The
+// survivors (about one in five, or one in 25 with the bitmap) go to a
+// per-wave LDS queue: customer key, date key | supplier nation << 24,
+// revenue.  The flush takes one queued row per lane, two rows per lane in
+// flight: year offset, customer nation, group.  The customer lookups are L2
+// gathers of a 600 KB table, each a distinct cache line: at 24 M of them the
+// L2 request rate bounds the kernel, which the bitmap cuts to the 4.8 M rows
+// that pass both filters (and the queue's flushes five-fold).
 enum Q31Mode { kQ31Codes = 0, kQ31Staged = 1, kQ31Bitmap = 2 };
 constexpr uint32_t kQ31Queue = 128;
 constexpr size_t kQ31QueueBytes = (size_t)kStarVecs / 2u * 3u * kQ31Queue * 4u;  // 16 waves, 24 KB
@@ -276,11 +277,6 @@ __device__ __forceinline__ void q31_flush(const StarDev& a, const Q31Lds& s, con
         const uint32_t x = q[i];
         ck[j] = i < qn ? x : 0xFFFFFFFFu;  // rows past the queue: out of range, dropped
         dk[j] = q[kQ31Queue + i];
-    }
-    if constexpr (MODE == kQ31Bitmap) {
-#pragma unroll
-        for (int j = 0; j < kPer; ++j)
-            if (!bit_at(s.c_bits, ck[j], 0u, a.n_cust)) ck[j] = 0xFFFFFFFFu;
     }
 #pragma unroll
     for (int j = 0; j < kPer; ++j) cv[j] = code_at(a.c_code, ck[j], 0u, a.n_cust);
@@ -339,6 +335,13 @@ __global__ __launch_bounds__(kStarBlock) void q31_kernel(const StarDev a) {
         dc.pin();
         rc.pin();
         if (!__any(m != 0)) continue;
+        if constexpr (MODE == kQ31Bitmap) {
+            // customer semi-join on every row from the LDS bitmap: only the
+            // rows that pass both filters (about 4 %) reach the queue
+#pragma unroll
+            for (int r = 0; r < 32; ++r) m &= ~((bit_at(s.c_bits, cc.get(r), km, a.n_cust) ^ 1u) << r);
+            if (!__any(m != 0)) continue;
+        }
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
             // flush only when this row slot's rows would not fit: the flushes
