@@ -56,6 +56,31 @@ def test_q21_small(n):
     check(out_p, want)
 
 
+@pytest.mark.parametrize("col", [0, 1, 2])
+@pytest.mark.parametrize("fixed", [True, False])
+def test_q21_keys_beyond_the_dimension_never_match(col, fixed):
+    # 5 % of the rows get a date, supplier or part key past the dimension's
+    # last row: the compact probe drops them (the reference's modulo hash
+    # tables leave such keys undefined, so the hash-table path is not run)
+    n = 40_000 + 9
+    fact, dims = T.o_gen_c3(13, n, **SMALL)
+    fact[1] %= SMALL["n_supp"]
+    fact[2] %= SMALL["n_part"]
+    rng = np.random.default_rng(6)
+    bad = rng.random(n) < 0.05
+    size = (SMALL["n_date"], SMALL["n_supp"], SMALL["n_part"])[col]
+    g = list(fact)
+    g[col] = np.where(bad, size + rng.integers(0, 50, n), fact[col]).astype(np.uint32)
+    widths = (12, 16, 20, 24) if fixed else (13, 17, 21, 25)
+    enc = [fls.HostColumn(fls.ENC_BITPACK, n, fls.pack_column(g[k], widths[k]), bw=widths[k], base=1)
+           for k in range(3)] + [fls.bitpack_column(g[3], widths[3])]
+    d = dev_dims(dims, SMALL)
+    cols = [c.to_device() for c in enc]
+    out = fls.query_q21(*cols, n, d, fls.q21_build(d)).cpu().numpy()
+    kept = [np.ascontiguousarray(x[~bad]) for x in fact]
+    check(out, T.o_q21(kept, dims, int((~bad).sum())))
+
+
 def test_q21_runtime_widths():
     # keys packed wider than needed take the run-time-width kernels
     n = 30_000

@@ -159,6 +159,25 @@ def test_q31_revenue_with_a_base():
     check(run_q31(f, d, SMALL, n, enc), T.o_q31(f, d, n))
 
 
+@pytest.mark.parametrize("col", ["custfk", "suppfk", "datefk", "partfk"])
+@pytest.mark.parametrize("widths", [FAST, None])
+def test_keys_beyond_the_dimension_never_match(col, widths):
+    # 5 % of the rows get a key past the dimension's last row: those rows
+    # match nothing, so the result is the oracle's over the other rows
+    n = 40_000 + 13
+    f, d = small_table(38, n)
+    rng = np.random.default_rng(5)
+    bad = rng.random(n) < 0.05
+    size = {"custfk": "n_cust", "suppfk": "n_supp", "datefk": "n_date", "partfk": "n_part"}[col]
+    g = dict(f)
+    g[col] = np.where(bad, SMALL[size] + rng.integers(0, 50, n), f[col]).astype(np.uint32)
+    enc = encode(g, SMALL, widths)
+    kept = {k: np.ascontiguousarray(v[~bad]) for k, v in f.items()}
+    want31 = T.o_q31(f, d, n) if col == "partfk" else T.o_q31(kept, d, int((~bad).sum()))
+    check(run_q31(g, d, SMALL, n, enc), want31)
+    check(run_q41(g, d, SMALL, n, enc), T.o_q41(kept, d, int((~bad).sum())))
+
+
 def test_q31_more_groups_than_the_block_hash():
     # every supplier and customer in the filtered region: 25 x 25 nations x 6
     # years = 3,750 live groups, beyond the probe's 512-entry LDS hash table,
