@@ -13,7 +13,7 @@
 // disk.
 //
 // No reference counterpart: every reference launch is one kernel per query
-// over one column set (SURVEY call stack E).
+// over one column set (fls_q11.cu:137-171, the query driver).
 #include <errno.h>
 #include <stdio.h>
 #include <stdlib.h>
