@@ -194,6 +194,28 @@ __device__ __forceinline__ uint32_t bit_at(const uint32_t* __restrict bits, uint
     return (bits[kk >> 5] >> (kk & 31u)) & (k < n ? 1u : 0u);
 }
 
+// m &= {rows r of the lane whose key's bit is set in `bits`} (bit_at per row),
+// B bitmap words read before any is used: written as one bit_at per row, the
+// compiler waits for each LDS read before issuing the next.
+template <int B, typename Col>
+__device__ __forceinline__ void bitmap_filter(const Col& col, const uint32_t* __restrict bits, uint32_t kmin,
+                                              uint32_t n, uint32_t& m) {
+#pragma unroll
+    for (int rb = 0; rb < 32; rb += B) {
+        uint32_t k[B], wv[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            k[j] = col.get(rb + j) - kmin;
+            wv[j] = bits[(k[j] < n ? k[j] : 0u) >> 5];
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const uint32_t hit = k[j] < n ? (wv[j] >> (k[j] & 31u)) & 1u : 0u;
+            m &= ~((hit ^ 1u) << (rb + j));
+        }
+    }
+}
+
 // Row mask (bit r) of the rows of this lane that exist in a vector holding
 // `live` <= 1024 rows, in normal or transposed row order.  Closed forms (no
 // per-row index table, which the compiler would hoist into 32 registers):

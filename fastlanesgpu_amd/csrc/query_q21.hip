@@ -237,8 +237,7 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
         // part semi-join: every row from the LDS bitmap; from L2 only the rows
         // that passed the supplier (rows no lane of the wave holds skipped)
         if constexpr (USE_LDS) {
-#pragma unroll
-            for (int r = 0; r < 32; ++r) m &= ~((bit_at(part_bm, pc.get(r), km, a.n_part) ^ 1u) << r);
+            bitmap_filter<16>(pc, part_bm, km, a.n_part, m);
         } else {
 #pragma unroll
             for (int r = 0; r < 32; ++r) {

@@ -338,6 +338,8 @@ __global__ __launch_bounds__(kStarBlock) void q31_kernel(const StarDev a) {
         if constexpr (MODE == kQ31Bitmap) {
             // customer semi-join on every row from the LDS bitmap: only the
             // rows that pass both filters (about 4 %) reach the queue
+            // one read per row: batching the reads (bitmap_filter) spills
+            // 28-40 VGPRs here, at any batch size from 4 to 16
 #pragma unroll
             for (int r = 0; r < 32; ++r) m &= ~((bit_at(s.c_bits, cc.get(r), km, a.n_cust) ^ 1u) << r);
             if (!__any(m != 0)) continue;
