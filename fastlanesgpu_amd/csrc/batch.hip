@@ -72,6 +72,7 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(const SegDev* __restrict s
         const TileDev tile = tiles[t];
         const SegDev sg = segs[tile.seg];
         const ColDev col{sg.enc, sg.bw, 0u, sg.transposed, sg.words, sg.vmeta, sg.bases, sg.aux};
+        const fls_rle_meta* rmeta = sg.rmeta;
         const uint32_t span = sg.hi - sg.lo;
         uint64_t cnt = 0;
         for (uint32_t vb = 0; vb < tile.nv; vb += VPB) {
@@ -84,12 +85,12 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(const SegDev* __restrict s
             if (first + 1024u <= sg.n_values) {
                 // a full vector: count in-range rows directly (compare + add with carry)
                 auto count = [&](int, uint32_t x) { c += (uint32_t)((x - lo) <= span); };
-                column_walk(col, sg.rmeta, v, stage[threadIdx.x >> 5], live, count);
+                column_walk(col, rmeta, v, stage[threadIdx.x >> 5], live, count);
             } else {
                 // the segment's partial last vector: row mask, then the rows that exist
                 uint32_t m = 0;
                 auto mark = [&](int r, uint32_t x) { m |= (uint32_t)((x - lo) <= span) << r; };
-                column_walk(col, sg.rmeta, v, stage[threadIdx.x >> 5], live, mark);
+                column_walk(col, rmeta, v, stage[threadIdx.x >> 5], live, mark);
                 if (live)
                     c = (uint32_t)__builtin_popcount(
                         m & live_row_mask((uint32_t)(sg.n_values - first), col.transposed != 0));

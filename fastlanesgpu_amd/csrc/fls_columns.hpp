@@ -7,6 +7,36 @@
 
 namespace fls {
 
+// Loads of column words and metadata, which always live in device memory,
+// through global address-space pointers.  A pointer the compiler cannot place
+// (one read from memory, e.g. a segment descriptor) is generic, and loads
+// through it compile to flat loads: those count on both the vector-memory and
+// the LDS counters, so every wait for one also waits for every LDS access in
+// flight and vice versa.  (A cast to the global space and back is folded
+// away; the load itself has to go through the global-space pointer.)
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p) {
+    if constexpr (__is_scalar(T)) {
+        return *(const __attribute__((address_space(1))) T*)p;
+    } else {  // a plain struct of whole words, read word by word
+        static_assert(sizeof(T) % 4 == 0, "struct of whole 32-bit words");
+        T t;
+        uint32_t* d = reinterpret_cast<uint32_t*>(&t);
+        const __attribute__((address_space(1))) uint32_t* g = (const __attribute__((address_space(1))) uint32_t*)p;
+#pragma unroll
+        for (unsigned i = 0; i < sizeof(T) / 4; ++i) d[i] = g[i];
+        return t;
+    }
+}
+__device__ __forceinline__ uint32_t ldg_nt(const uint32_t* p) {
+    return __builtin_nontemporal_load((const __attribute__((address_space(1))) uint32_t*)p);
+}
+template <int BW>
+__device__ __forceinline__ void load_lane_g(const uint32_t* lane_in, uint32_t (&w)[BW > 0 ? BW : 1]) {
+#pragma unroll
+    for (int k = 0; k < BW; ++k) w[k] = ldg_nt(lane_in + 32 * k);
+}
+
 struct ColDev {
     uint32_t enc, bw, base, transposed;
     const uint32_t* words;
@@ -73,8 +103,8 @@ __device__ __forceinline__ void decode_delta(const ColDev& c, uint64_t v, uint32
 __device__ __forceinline__ uint32_t linear_extract(const uint32_t* __restrict w, uint32_t k, uint32_t bw) {
     if (bw == 0) return 0u;
     const uint32_t bit = k * bw, i = bit >> 5, s = bit & 31u;
-    const uint32_t lo = w[i];
-    const uint32_t hi = (s + bw > 32u) ? w[i + 1] : 0u;
+    const uint32_t lo = ldg(w + i);
+    const uint32_t hi = (s + bw > 32u) ? ldg(w + i + 1) : 0u;
     const uint32_t mask = bw >= 32 ? 0xffffffffu : ((1u << bw) - 1u);
     return __builtin_amdgcn_alignbit(hi, lo, s) & mask;
 }
@@ -415,7 +445,7 @@ __device__ __forceinline__ void walk_fixed_lane(const uint32_t* __restrict lane_
         for (int r = 0; r < 32; ++r) f(r, base);
     } else {
         uint32_t w[BW];
-        dev::load_lane<BW>(lane_in, w);
+        load_lane_g<BW>(lane_in, w);
 #pragma unroll
         for (int r = 0; r < 32; ++r) f(r, dev::extract_at<BW>(w, r) + base);
     }
@@ -429,7 +459,7 @@ __device__ __forceinline__ void walk_delta_lane(const uint32_t* __restrict lane_
         for (int r = 0; r < 32; ++r) f(r, first);
     } else {
         uint32_t w[BW];
-        dev::load_lane<BW>(lane_in, w);
+        load_lane_g<BW>(lane_in, w);
         uint32_t acc = first;
 #pragma unroll
         for (int j = 0; j < 32; ++j) {
@@ -480,7 +510,7 @@ __device__ __forceinline__ void walk_rle(const uint32_t* __restrict idx_lane, ui
         for (int r = 0; r < 32; ++r) f(r, x);
         return;
     }
-    const uint32_t starts = __builtin_nontemporal_load(idx_lane);
+    const uint32_t starts = ldg_nt(idx_lane);
     uint32_t acc = first_idx;
 #pragma unroll
     for (int j = 0; j < 32; ++j) {
@@ -517,23 +547,23 @@ __device__ __forceinline__ void column_walk(const ColDev& c, const fls_rle_meta*
     }
     if (c.enc == FLS_ENC_DELTA) {
         if (!live) return;
-        const fls_vmeta m = c.vmeta[v];
-        walk_delta(c.words + FLS_VMETA_OFFSET(m) + l, FLS_VMETA_BW(m), c.bases[v * 32u + l], f);
+        const fls_vmeta m = ldg(c.vmeta + v);
+        walk_delta(c.words + FLS_VMETA_OFFSET(m) + l, FLS_VMETA_BW(m), ldg(c.bases + v * 32u + l), f);
         return;
     }
     if (c.enc == FLS_ENC_RLE) {
         fls_vmeta m = 0;
         fls_rle_meta rm{};
         if (live) {
-            m = c.vmeta[v];
-            rm = rmeta[v];
+            m = ldg(c.vmeta + v);
+            rm = ldg(rmeta + v);
         }
         const uint32_t* p = c.words + FLS_VMETA_OFFSET(m);
         const uint32_t ibw = FLS_VMETA_BW(m) ? 1u : 0u;
         stage_runs(p + 32u * ibw, rm.n_runs, rm.vbw, rm.vbase, stage, live);
         if (!live) return;
         const uint32_t last = rm.n_runs ? min((uint32_t)rm.n_runs, 1024u) - 1u : 0u;
-        walk_rle(p + l, ibw, c.bases[v * 32u + l], stage, last, f);
+        walk_rle(p + l, ibw, ldg(c.bases + v * 32u + l), stage, last, f);
         return;
     }
     // MIXED
@@ -541,10 +571,10 @@ __device__ __forceinline__ void column_walk(const ColDev& c, const fls_rle_meta*
     uint64_t ax = 0;
     const uint32_t* p = c.words;
     if (live) {
-        const fls_vmeta m = c.vmeta[v];
+        const fls_vmeta m = ldg(c.vmeta + v);
         enc = (uint32_t)(m >> 6) & 3u;
         w = (uint32_t)m & 63u;
-        ax = c.aux[v];
+        ax = ldg(c.aux + v);
         p = c.words + FLS_VMETA_OFFSET(m);
     }
     const uint32_t ibw = (uint32_t)(ax >> 48) & 1u, nr = (uint32_t)(ax >> 32) & 0xffffu;
@@ -553,10 +583,10 @@ __device__ __forceinline__ void column_walk(const ColDev& c, const fls_rle_meta*
     switch (enc) {
         case FLS_MIX_BITPACK: walk_fixed(p + l, w, 0u, f); break;
         case FLS_MIX_FOR: walk_fixed(p + l, w, (uint32_t)ax, f); break;
-        case FLS_MIX_DELTA: walk_delta(p + 32u + l, w, p[l], f); break;
+        case FLS_MIX_DELTA: walk_delta(p + 32u + l, w, ldg(p + l), f); break;
         default: {
             const uint32_t last = nr ? min(nr, 1024u) - 1u : 0u;
-            walk_rle(p + 32u + l, ibw, p[l], stage, last, f);
+            walk_rle(p + 32u + l, ibw, ldg(p + l), stage, last, f);
         }
     }
 }
