@@ -23,8 +23,8 @@ constexpr uint32_t kMaxSlots = 8192;
 
 // workspace layout (byte offsets, 256-aligned)
 struct StarLayout {
-    uint64_t s_code, c_code, p_bits, y_code, c_bits, total;
-    uint32_t p_words, c_words;
+    uint64_t s_code, c_code, p_bits, y_code, c_bits, s_bits, total;
+    uint32_t p_words, c_words, s_words;
 };
 
 StarLayout star_layout(const fls_star_dims& d) {
@@ -32,12 +32,14 @@ StarLayout star_layout(const fls_star_dims& d) {
     StarLayout L{};
     L.p_words = (d.n_part + 31u) / 32u;
     L.c_words = (d.n_cust + 31u) / 32u;
+    L.s_words = (d.n_supp + 31u) / 32u;
     L.s_code = 0;
     L.c_code = up(L.s_code + d.n_supp);
     L.p_bits = up(L.c_code + d.n_cust);
     L.y_code = up(L.p_bits + 4ull * L.p_words);
     L.c_bits = up(L.y_code + d.n_date);
-    L.total = up(L.c_bits + 4ull * L.c_words);
+    L.s_bits = up(L.c_bits + 4ull * L.c_words);
+    L.total = up(L.s_bits + 4ull * L.s_words);
     return L;
 }
 
@@ -65,14 +67,18 @@ __global__ void q31_build_kernel(fls_star_dims d, fls_q31_params p, uint8_t* __r
     }
 }
 
-// Q4.1 builds (fls_q41.cu:186-271): qualifying suppliers (code 0), parts (a
-// bit), customers (their nation), every date (its year offset)
+// Q4.1 builds (fls_q41.cu:186-271): qualifying suppliers (code 0 and a bit),
+// parts (a bit), customers (their nation), every date (its year offset);
+// p_bits and s_bits zeroed before
 __global__ void q41_build_kernel(fls_star_dims d, fls_q41_params p, uint8_t* __restrict s_code,
                                  uint8_t* __restrict c_code, uint32_t* __restrict p_bits,
-                                 uint8_t* __restrict y_code) {
+                                 uint8_t* __restrict y_code, uint32_t* __restrict s_bits) {
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_supp; i += stride)
-        s_code[i] = d.s_region[i] == p.s_region ? (uint8_t)0 : kNone;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_supp; i += stride) {
+        const bool hit = d.s_region[i] == p.s_region;
+        s_code[i] = hit ? (uint8_t)0 : kNone;
+        if (hit) atomicOr(&s_bits[i >> 5], 1u << (i & 31u));
+    }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_cust; i += stride)
         c_code[i] = (d.c_region[i] == p.c_region && d.c_nation[i] < kNone) ? (uint8_t)d.c_nation[i] : kNone;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_part; i += stride)
@@ -87,8 +93,8 @@ struct StarDev {
     ColDev date, cust, supp, part, rev, cost;
     uint64_t n_rows, n_vec, n_groups;
     const uint8_t *s_code, *c_code, *y_code;
-    const uint32_t *p_bits, *c_bits;
-    uint32_t n_supp, n_cust, n_part, n_date, keys_min, p_words, c_words;
+    const uint32_t *p_bits, *c_bits, *s_bits;
+    uint32_t n_supp, n_cust, n_part, n_date, keys_min, p_words, c_words, s_words;
     uint32_t slot_mul_s, slot_mul_c, n_slots;  // slot = (s*slot_mul_s + c*slot_mul_c + y) % n_slots
     FastDiv slot_div;                          // n_slots
     int64_t* sums;
@@ -415,31 +421,67 @@ __global__ __launch_bounds__(kStarBlock) void q31_any_kernel(const StarDev a) {
 // measured no faster (the 16 waves of a CU already cover the second round
 // trip) and, kept there by a scheduling barrier, spilled 4 VGPRs and ran 5 %
 // slower.
-constexpr uint32_t kQ41Queue = 64;
-constexpr size_t kQ41QueueBytes = (size_t)kStarVecs / 2u * 4u * kQ41Queue * 4u;  // 16 waves, 16 KB
+constexpr uint32_t kQ41Queue = 128;
+constexpr size_t kQ41QueueBytes = (size_t)kStarVecs / 2u * 4u * kQ41Queue * 4u;  // 16 waves, 32 KB
 
 __device__ __forceinline__ void q41_flush(const StarDev& a, const StarLds& t, const uint32_t* q, uint32_t qn) {
+    constexpr int kPer = kQ41Queue / 64;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const uint32_t l = threadIdx.x & 63u;
-    const uint32_t x = q[l];
-    const uint32_t ck = l < qn ? x : 0xFFFFFFFFu;  // rows past the queue: out of range, dropped
-    const uint32_t cv = code_at(a.c_code, ck, 0u, a.n_cust);
-    const uint32_t y = code_at(t.y_code, q[kQ41Queue + l], 0u, a.n_date);
-    if (cv != kNone && y != kNone)
-        group_add_lds(t, fast_mod(cv * a.slot_mul_c + y, a.slot_div),
-                      (uint64_t)((int64_t)q[2u * kQ41Queue + l] - (int64_t)q[3u * kQ41Queue + l]));
+    uint32_t ck[kPer], cv[kPer], y[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const uint32_t i = l + 64u * j;
+        const uint32_t x = q[i];
+        ck[j] = i < qn ? x : 0xFFFFFFFFu;  // rows past the queue: out of range, dropped
+        y[j] = code_at(t.y_code, q[kQ41Queue + i], 0u, a.n_date);
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) cv[j] = code_at(a.c_code, ck[j], 0u, a.n_cust);
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const uint32_t i = l + 64u * j;
+        if (cv[j] != kNone && y[j] != kNone)
+            group_add_lds(t, fast_mod(cv[j] * a.slot_mul_c + y[j], a.slot_div),
+                          (uint64_t)((int64_t)q[2u * kQ41Queue + i] - (int64_t)q[3u * kQ41Queue + i]));
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// USE_LDS: group table, part bits, supplier bits and year codes in LDS (SSB
+// sizes: 2.1 KB + 100 KB + 5 KB + 2.5 KB); otherwise the group table alone.
 template <int DBW, int CBW, int SBW, int PBW, int RBW, int KBW, bool USE_LDS>
 __global__ __launch_bounds__(kStarBlock) void q41_kernel(const StarDev a) {
     extern __shared__ __align__(16) uint8_t lds[];
     uint32_t* q = reinterpret_cast<uint32_t*>(lds) + (threadIdx.x >> 6) * 4u * kQ41Queue;
-    const StarLds t = star_stage<USE_LDS, true>(a, lds + kQ41QueueBytes);
+    StarLds t;
+    uint8_t* base = lds + kQ41QueueBytes;
+    t.sum = reinterpret_cast<unsigned long long*>(base);
+    t.cnt = reinterpret_cast<uint32_t*>(base + 8ull * a.n_slots);
+    for (uint32_t i = threadIdx.x; i < a.n_slots; i += blockDim.x) {
+        t.sum[i] = 0ull;
+        t.cnt[i] = 0u;
+    }
+    t.s_code = a.s_code;
+    t.y_code = a.y_code;
+    t.p_bits = a.p_bits;
+    const uint32_t* s_bits = a.s_bits;
+    if constexpr (USE_LDS) {
+        uint32_t* pb = reinterpret_cast<uint32_t*>(base + 12ull * a.n_slots);  // 4-byte aligned
+        uint32_t* sb = pb + a.p_words;
+        uint8_t* y = reinterpret_cast<uint8_t*>(sb + a.s_words);
+        for (uint32_t i = threadIdx.x; i < a.p_words; i += blockDim.x) pb[i] = a.p_bits[i];
+        for (uint32_t i = threadIdx.x; i < a.s_words; i += blockDim.x) sb[i] = a.s_bits[i];
+        for (uint32_t i = threadIdx.x; i < a.n_date; i += blockDim.x) y[i] = a.y_code[i];
+        t.p_bits = pb;
+        t.y_code = y;
+        s_bits = sb;
+    }
+    __syncthreads();
     const uint32_t tail = live_row_mask((uint32_t)(a.n_rows - (a.n_vec - 1) * 1024u), a.rev.transposed != 0);
     const uint32_t km = a.keys_min;
     uint32_t qn = 0;  // wave-uniform; the queue carries over from step to step
@@ -458,11 +500,8 @@ __global__ __launch_bounds__(kStarBlock) void q41_kernel(const StarDev a) {
             sc.load(a.supp, vl);
             pc.load(a.part, vl);
 #pragma unroll
-            for (int r = 0; r < 32; ++r) {
-                const uint32_t hit = (code_at(t.s_code, sc.get(r), km, a.n_supp) != kNone) &
-                                     bit_at(t.p_bits, pc.get(r), km, a.n_part);
-                m |= hit << r;
-            }
+            for (int r = 0; r < 32; ++r)
+                m |= (bit_at(s_bits, sc.get(r), km, a.n_supp) & bit_at(t.p_bits, pc.get(r), km, a.n_part)) << r;
         }
         if (!vlive) m = 0;
         if (v == a.n_vec - 1) m &= tail;
@@ -589,6 +628,7 @@ int star_workspace(StarDev& a, const fls_star_dims* dims, const void* d_ws) {
     a.c_code = reinterpret_cast<const uint8_t*>(ws + L.c_code);
     a.p_bits = reinterpret_cast<const uint32_t*>(ws + L.p_bits);
     a.c_bits = reinterpret_cast<const uint32_t*>(ws + L.c_bits);
+    a.s_bits = reinterpret_cast<const uint32_t*>(ws + L.s_bits);
     a.y_code = reinterpret_cast<const uint8_t*>(ws + L.y_code);
     a.n_supp = dims->n_supp;
     a.n_cust = dims->n_cust;
@@ -596,6 +636,7 @@ int star_workspace(StarDev& a, const fls_star_dims* dims, const void* d_ws) {
     a.n_date = dims->n_date;
     a.p_words = L.p_words;
     a.c_words = L.c_words;
+    a.s_words = L.s_words;
     a.keys_min = dims->keys_min;
     return FLS_OK;
 }
@@ -692,8 +733,11 @@ int fls_q41_build(const fls_star_dims* dims, const fls_q41_params* params, void*
     hipStream_t s = as_stream(stream);
     hipLaunchKernelGGL(star_zero_words, dim3(256), dim3(256), 0, s, reinterpret_cast<uint32_t*>(ws + L.p_bits),
                        L.p_words);
+    hipLaunchKernelGGL(star_zero_words, dim3(256), dim3(256), 0, s, reinterpret_cast<uint32_t*>(ws + L.s_bits),
+                       L.s_words);
     hipLaunchKernelGGL(q41_build_kernel, dim3(1024), dim3(256), 0, s, *dims, *params, ws + L.s_code, ws + L.c_code,
-                       reinterpret_cast<uint32_t*>(ws + L.p_bits), ws + L.y_code);
+                       reinterpret_cast<uint32_t*>(ws + L.p_bits), ws + L.y_code,
+                       reinterpret_cast<uint32_t*>(ws + L.s_bits));
     return check_launch();
 }
 
@@ -719,8 +763,9 @@ int fls_query_q41(const fls_column* datekey, const fls_column* custkey, const fl
     const bool fast = all_bitpack(outs, 6) && a.date.bw == 12 && a.cust.bw == 20 && a.supp.bw == 16 &&
                       a.part.bw == 20 && a.rev.bw == 24 && a.cost.bw == 16;
     if (fast) {
-        const bool fits = kQ41QueueBytes + staged <= kLdsMax;
-        const size_t lds = kQ41QueueBytes + (fits ? staged : table);
+        const size_t bits = table + 4ull * a.p_words + 4ull * a.s_words + dims->n_date;
+        const bool fits = kQ41QueueBytes + bits <= kLdsMax;
+        const size_t lds = kQ41QueueBytes + (fits ? bits : table);
         return fits ? launch_star(q41_kernel<12, 20, 16, 20, 24, 16, true>, lds, a, s)
                     : launch_star(q41_kernel<12, 20, 16, 20, 24, 16, false>, lds, a, s);
     }

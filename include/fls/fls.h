@@ -333,11 +333,12 @@ typedef struct fls_q41_params {
 } fls_q41_params;
 
 /* MI355X-native builds: one byte per supplier, customer and date (nation or
- * year offset, 0xFF for a filtered row), one bit per part (Q4.1) and one bit
- * per qualifying customer (Q3.1), in a caller-owned workspace of
- * fls_star_workspace_bytes() bytes.  The probes stage the supplier and date
- * codes (and the part or customer bits) in LDS, read the customer codes
- * through L2, and group in per-block LDS tables flushed once per block.
+ * year offset, 0xFF for a filtered row), one bit per part and per qualifying
+ * supplier (Q4.1) and one bit per qualifying customer (Q3.1), in a
+ * caller-owned workspace of fls_star_workspace_bytes() bytes.  The probes
+ * stage the small tables (supplier codes or bits, part or customer bits,
+ * date codes) in LDS, read the customer codes through L2, and group in
+ * per-block LDS tables flushed once per block.
  * d_sums / d_counts hold n_slots (<= 8192) entries each, overwritten unless
  * flags has FLS_Q_ACCUMULATE.  Keys outside [keys_min, keys_min + n) do not
  * match.  All fact columns are BITPACK with one row order.                   */
