@@ -103,7 +103,6 @@ struct StarDev {
 
 constexpr int kStarBlock = 1024;
 constexpr int kB31 = 16;  // rows per batch of dimension lookups
-constexpr int kB41 = 8;
 constexpr int kStarVecs = kStarBlock / 32;
 
 // per-block LDS: group table, then (USE_LDS) the staged dimension codes

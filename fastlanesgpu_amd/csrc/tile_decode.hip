@@ -255,7 +255,7 @@ int launch_tiles(Kernel k, uint64_t n, hipStream_t s, Args... args) {
     return check_launch();
 }
 
-int prepare(uint64_t n, uint32_t* d_out, uint64_t* d_sum, hipStream_t s) {
+int prepare(uint32_t* d_out, uint64_t* d_sum, hipStream_t s) {
     if (!d_out && !d_sum) return FLS_ERR_NULL;
     if (d_sum) return zero_u64(d_sum, 1, s);
     return FLS_OK;
@@ -270,7 +270,7 @@ int fls_tile_for_decode_u32(const uint32_t* d_data, const uint32_t* d_starts, ui
                             uint64_t* d_sum, void* stream) {
     using namespace fls;
     hipStream_t s = as_stream(stream);
-    if (const int st = prepare(n, d_out, d_sum, s)) return st;
+    if (const int st = prepare(d_out, d_sum, s)) return st;
     if (n == 0) return FLS_OK;
     FLS_RETURN_IF(!d_data || !d_starts, FLS_ERR_NULL);
     if (d_out && d_sum) return launch_tiles(tile_for_kernel<true, true>, n, s, d_data, d_starts, n, d_out, d_sum);
@@ -282,7 +282,7 @@ int fls_tile_dfor_decode_u32(const uint32_t* d_data, const uint32_t* d_starts, u
                              uint64_t* d_sum, void* stream) {
     using namespace fls;
     hipStream_t s = as_stream(stream);
-    if (const int st = prepare(n, d_out, d_sum, s)) return st;
+    if (const int st = prepare(d_out, d_sum, s)) return st;
     if (n == 0) return FLS_OK;
     FLS_RETURN_IF(!d_data || !d_starts, FLS_ERR_NULL);
     if (d_out && d_sum) return launch_tiles(tile_dfor_kernel<true, true>, n, s, d_data, d_starts, n, d_out, d_sum);
@@ -295,7 +295,7 @@ int fls_tile_rfor_decode_u32(const uint32_t* d_values, const uint32_t* d_val_sta
                              void* stream) {
     using namespace fls;
     hipStream_t s = as_stream(stream);
-    if (const int st = prepare(n, d_out, d_sum, s)) return st;
+    if (const int st = prepare(d_out, d_sum, s)) return st;
     if (n == 0) return FLS_OK;
     FLS_RETURN_IF(!d_values || !d_val_starts || !d_run_lengths || !d_rl_starts, FLS_ERR_NULL);
     if (d_out && d_sum)

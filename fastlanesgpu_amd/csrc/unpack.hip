@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void rle_decode_kernel(const uint32_t* __re
     __shared__ uint32_t stage[kVecPerBlock][1024];
     const uint64_t v = (uint64_t)blockIdx.x * kVecPerBlock + (threadIdx.x >> 5);
     const bool live = v < n_vec;
-    const ColDev c{FLS_ENC_RLE, 0, 0, 1, words, vmeta, bases};
+    const ColDev c{FLS_ENC_RLE, 0, 0, 1, words, vmeta, bases, nullptr};
     uint32_t x[32];
     decode_rle(c, rmeta, v, stage[threadIdx.x >> 5], live, x);
     if (!live) return;
