@@ -134,6 +134,26 @@ def test_q21_every_row_qualifies(n):
     check(out_p, want)
 
 
+def test_q21_more_groups_than_the_block_hash():
+    # every row qualifies at the SSB widths (the compact kernel): the brands of
+    # 8,000 parts span up to 1,000 x 7 year groups, beyond the probe's
+    # 1,024-entry LDS hash table, so most groups take the global-atomic path
+    n = 200_000 + 7
+    fact, dims = T.o_gen_c3(15, n, **SMALL)
+    fact[1] %= SMALL["n_supp"]
+    fact[2] %= SMALL["n_part"]
+    dims["s_region"][:] = 2
+    dims["p_category"][:] = 12
+    want = T.o_q21(fact, dims, n)
+    assert (want[1] > 0).sum() > 2048
+    widths = (12, 16, 20, 24)
+    enc = [fls.HostColumn(fls.ENC_BITPACK, n, fls.pack_column(fact[k], widths[k]), bw=widths[k], base=1)
+           for k in range(3)] + [fls.bitpack_column(fact[3], widths[3])]
+    out_c, out_p = run_both(fact, dims, SMALL, n, enc)
+    check(out_c, want)
+    check(out_p, want)
+
+
 def test_q21_full_size_and_result_rows():
     n = 120_000_000
     fd = fls.gen_c3_fact(1, n)
