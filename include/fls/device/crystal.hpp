@@ -130,16 +130,31 @@ FLS_CRYSTAL_PRED(BlockPredOrEQ, OrFlag, x == compare)
 // PHT_1 (crystal/join.cuh:8-72): slot = HASH(key); a non-zero slot is a hit.
 // A stored key of 0 is indistinguishable from an empty slot, so callers keep
 // keys >= 1 (the FOR base of a dense 0-based key column is 1).
+// The probes of a thread are issued kProbeBatch at a time before any result is
+// used (an item that is not probed reads slot 0 and ignores it): written as
+// one conditional load per item, every probe waits for the one before it.
+constexpr int kProbeBatch = 8;
+
 template <typename K, int BLOCK_THREADS, int ITEMS_PER_THREAD>
 __device__ __forceinline__ void BlockProbeAndPHT_1(const K (&items)[ITEMS_PER_THREAD],
                                                    int (&selection_flags)[ITEMS_PER_THREAD], const K* ht, int ht_len,
                                                    K keys_min, int num_items) {
     const int tid = fls::dev::tile_tid<BLOCK_THREADS>();
 #pragma unroll
-    for (int i = 0; i < ITEMS_PER_THREAD; ++i) {
-        if (tid + i * BLOCK_THREADS < num_items && selection_flags[i]) {
-            const K slot = ht[fls::dev::pht_hash(items[i], ht_len, keys_min)];
-            selection_flags[i] = slot != 0;
+    for (int i0 = 0; i0 < ITEMS_PER_THREAD; i0 += kProbeBatch) {
+        constexpr int B = ITEMS_PER_THREAD < kProbeBatch ? ITEMS_PER_THREAD : kProbeBatch;
+        K slot[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const int i = i0 + j;
+            const bool live = i < ITEMS_PER_THREAD && tid + i * BLOCK_THREADS < num_items && selection_flags[i];
+            slot[j] = ht[live ? fls::dev::pht_hash(items[i], ht_len, keys_min) : 0];
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const int i = i0 + j;
+            if (i < ITEMS_PER_THREAD && tid + i * BLOCK_THREADS < num_items && selection_flags[i])
+                selection_flags[i] = slot[j] != 0;
         }
     }
 }
@@ -159,12 +174,23 @@ __device__ __forceinline__ void BlockProbeAndPHT_2(const K (&items)[ITEMS_PER_TH
                                                    K keys_min, int num_items) {
     const int tid = fls::dev::tile_tid<BLOCK_THREADS>();
 #pragma unroll
-    for (int i = 0; i < ITEMS_PER_THREAD; ++i) {
-        if (tid + i * BLOCK_THREADS < num_items && selection_flags[i]) {
-            const int h = fls::dev::pht_hash(items[i], ht_len, keys_min);
-            const uint64_t slot = *reinterpret_cast<const uint64_t*>(&ht[h << 1]);
-            if (slot != 0) res[i] = (V)(slot >> 32);
-            else selection_flags[i] = 0;
+    for (int i0 = 0; i0 < ITEMS_PER_THREAD; i0 += kProbeBatch) {
+        constexpr int B = ITEMS_PER_THREAD < kProbeBatch ? ITEMS_PER_THREAD : kProbeBatch;
+        uint64_t slot[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const int i = i0 + j;
+            const bool live = i < ITEMS_PER_THREAD && tid + i * BLOCK_THREADS < num_items && selection_flags[i];
+            const int h = live ? fls::dev::pht_hash(items[i], ht_len, keys_min) : 0;
+            slot[j] = *reinterpret_cast<const uint64_t*>(&ht[h << 1]);
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const int i = i0 + j;
+            if (i < ITEMS_PER_THREAD && tid + i * BLOCK_THREADS < num_items && selection_flags[i]) {
+                if (slot[j] != 0) res[i] = (V)(slot[j] >> 32);
+                else selection_flags[i] = 0;
+            }
         }
     }
 }
