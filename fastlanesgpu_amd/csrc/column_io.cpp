@@ -67,6 +67,47 @@ bool words_ok(const fls_column& c, uint64_t n_rows) {
     return true;
 }
 
+// Every vector's metadata describes a payload inside the column's words, at
+// widths the decoders take (0..32), with an RLE run count of 1..1024: the
+// kernels read exactly the words this describes, so an image that passes
+// never makes a decode or query kernel read outside the words array.
+bool vectors_ok(const fls_column& c, uint64_t n_rows) {
+    if (c.encoding == FLS_ENC_BITPACK) return true;
+    const uint64_t n_vec = (n_rows + 1023u) / 1024u;
+    for (uint64_t v = 0; v < n_vec; ++v) {
+        const fls_vmeta m = c.vmeta[v];
+        const uint64_t off = FLS_VMETA_OFFSET(m);
+        uint64_t need = 0;
+        if (c.encoding == FLS_ENC_DELTA) {
+            const uint32_t bw = FLS_VMETA_BW(m);
+            if (bw > 32) return false;
+            need = 32ull * bw;
+        } else if (c.encoding == FLS_ENC_RLE) {
+            const uint32_t ibw = FLS_VMETA_BW(m);
+            fls_rle_meta rm;
+            memcpy(&rm, &c.aux[v], sizeof rm);
+            if (ibw > 32 || rm.vbw > 32 || rm.n_runs == 0 || rm.n_runs > 1024) return false;
+            need = 32ull * ibw + ((uint64_t)rm.n_runs * rm.vbw + 31u) / 32u;
+        } else {  // MIXED: vmeta = offset << 8 | enc << 6 | width
+            const uint32_t enc = (uint32_t)(m >> 6) & 3u, w = (uint32_t)m & 63u;
+            if (w > 32) return false;
+            const uint64_t ax = c.aux[v];
+            switch (enc) {
+                case FLS_MIX_BITPACK:
+                case FLS_MIX_FOR: need = 32ull * w; break;
+                case FLS_MIX_DELTA: need = 32ull + 32ull * w; break;
+                default: {
+                    const uint64_t nr = (ax >> 32) & 0xffffu, ibw = ax >> 48;
+                    if (ibw > 1 || nr == 0 || nr > 1024) return false;
+                    need = 32ull + 32ull * ibw + (nr * w + 31u) / 32u;
+                }
+            }
+        }
+        if (off > c.n_words || need > c.n_words - off) return false;
+    }
+    return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -88,6 +129,7 @@ int fls_column_serialize(const fls_column* col, uint64_t n_rows, void* out, uint
     if ((col->n_words && !col->words) || (c.vmeta && !col->vmeta) || (c.bases && !col->bases) ||
         (c.aux && !col->aux))
         return FLS_ERR_NULL;
+    if (!vectors_ok(*col, n_rows)) return FLS_ERR_FORMAT;
     fls_column_image h{};
     memcpy(h.magic, kMagic, 8);
     h.version = kVersion;
@@ -141,7 +183,7 @@ int fls_column_deserialize(const void* image, uint64_t image_bytes, fls_column* 
     out.vmeta = h.n_vmeta ? reinterpret_cast<const fls_vmeta*>(p + h.off_vmeta) : nullptr;
     out.bases = h.n_bases ? reinterpret_cast<const uint32_t*>(p + h.off_bases) : nullptr;
     out.aux = h.n_aux ? reinterpret_cast<const uint64_t*>(p + h.off_aux) : nullptr;
-    if (!words_ok(out, h.n_rows)) return FLS_ERR_FORMAT;
+    if (!words_ok(out, h.n_rows) || !vectors_ok(out, h.n_rows)) return FLS_ERR_FORMAT;
     *col = out;
     *n_rows = h.n_rows;
     return FLS_OK;

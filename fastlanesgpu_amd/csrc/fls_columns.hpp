@@ -292,7 +292,7 @@ __device__ __forceinline__ void decode_rle(const ColDev& c, const fls_rle_meta* 
     if (!live) return;
     dev::unpack_device(c.words + FLS_VMETA_OFFSET(m), x, FLS_VMETA_BW(m));
     dev::rsum_lane(x, c.bases[v * 32u + dev::lane32()]);
-    const uint32_t last = rm.n_runs ? rm.n_runs - 1u : 0u;
+    const uint32_t last = rm.n_runs ? min((uint32_t)rm.n_runs, 1024u) - 1u : 0u;  // inside this vector's stage
 #pragma unroll
     for (int r = 0; r < 32; ++r) x[r] = stage[min(x[r], last)];
 }

@@ -127,16 +127,25 @@ __device__ __forceinline__ void group_add(const Q21Dev& a, uint32_t brand, uint3
 }
 
 // Reference-layout probe (probe_v1, fls_q21.cu:21-78) through the Crystal
-// primitives, one 32-lane tile per FastLanes vector.
+// primitives, one 32-lane tile per FastLanes vector.  The rows of a partial
+// last vector are the first `num` striped items in row order; in transposed
+// order they are scattered over the lane's registers, so the flags start from
+// the live-row mask and the primitives see a whole tile.
 template <int DBW, int SBW, int PBW, int RBW>
 __global__ __launch_bounds__(256) void q21_pht_kernel(const Q21Dev a) {
     for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
         const uint64_t v = g * 8u + (threadIdx.x >> 5);
         if (v >= a.n_vec) continue;
-        const int num = (int)min<uint64_t>(1024u, a.n_rows - v * 1024u);
+        int num = (int)min<uint64_t>(1024u, a.n_rows - v * 1024u);
         uint32_t items[32], brand[32], year[32];
         int flags[32];
         InitFlags<32, 32>(flags);
+        if (num < 1024 && a.rev.transposed) {
+            const uint32_t live = live_row_mask((uint32_t)num, true);
+#pragma unroll
+            for (int i = 0; i < 32; ++i) flags[i] = (int)((live >> i) & 1u);
+            num = 1024;
+        }
         decode_fixed<SBW>(a.supp, v, items);
         BlockProbeAndPHT_1<uint32_t, 32, 32>(items, flags, a.ht_s, (int)a.s_len, a.s_min, num);
         decode_fixed<PBW>(a.part, v, items);

@@ -120,7 +120,7 @@ const char* fls_status_string(int status) {
         case FLS_ERR_ARG: return "invalid argument";
         case FLS_ERR_CAPACITY: return "output buffer too small";
         case FLS_ERR_NO_DEVICE: return "no usable gfx950 device";
-        case FLS_ERR_FORMAT: return "serialized column is malformed (magic, version, sizes or checksum)";
+        case FLS_ERR_FORMAT: return "serialized column is malformed (magic, version, sizes, checksum or vector metadata)";
         default: return "unknown status";
     }
 }

@@ -159,3 +159,197 @@ __host__ __device__ constexpr uint32_t untranspose_index(uint32_t p) {
 
 }  // namespace dev
 }  // namespace fls
+
+// ======================================================================
+// The reference's decode entry points under their own names
+// (fastlanes/src/include/fls_gen/unpack/{unpack,unpack_fused,hardcoded_16}.cuh,
+// fastlanes/src/include/fls_gen/rsum/rsum.cuh), so a reference kernel body
+// compiles against this header as written.  Every one works on the vector (or
+// quarter vector) whose first packed word `in` points at, for the lane
+// threadIdx.x % 32: launched the reference's way (32 work-items per vector)
+// the lane is threadIdx.x itself; a 64-wide wave may serve two vectors, each
+// half-wave passing its own `in`.
+// ======================================================================
+
+namespace fls {
+namespace dev {
+
+// Rows 0..7 of one lane of a quarter vector.  Rows 8q..8q+7 of a vector packed
+// at a width that is a multiple of 4 occupy lane words [q*bw/4, (q+1)*bw/4),
+// so the quarter is itself a 256-value FastLanes block of 32 lanes x 8 rows
+// starting at word 8*bw*q of the vector (hardcoded_16.cuh:436-786).
+template <int BW>
+__device__ __forceinline__ void unpack_quarter_lane(const uint32_t* __restrict q_in, uint32_t (&out)[8]) {
+    static_assert(BW % 4 == 0 && BW >= 0 && BW <= 32, "a quarter vector needs a width that is a multiple of 4");
+    if constexpr (BW == 0) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) out[r] = 0u;
+    } else {
+        uint32_t w[BW] = {};
+        const uint32_t* lane_in = q_in + lane32();
+#pragma unroll
+        for (int k = 0; k < BW / 4; ++k) w[k] = lane_in[32 * k];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) out[r] = extract_at<BW>(w, r);
+    }
+}
+
+// Decode one lane into memory in the reference's "normal" order:
+// out[32*r + lane] = row r.
+template <int BW>
+__device__ __forceinline__ void unpack_lane_to(const uint32_t* __restrict vec_in, uint32_t* __restrict out) {
+    uint32_t v[32];
+    unpack_lane<BW>(vec_in, v);
+    const uint32_t l = lane32();
+#pragma unroll
+    for (int r = 0; r < 32; ++r) out[32 * r + l] = v[r];
+}
+
+}  // namespace dev
+}  // namespace fls
+
+#define FLS_FOR_EACH_BW(X)                                                                                       \
+    X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) \
+        X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31) X(32)
+#define FLS_FOR_EACH_BW4(X) X(0) X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32)
+
+namespace generated {
+namespace unpack {
+namespace cuda {
+
+// unpack_fused.cuh:7-3433: one lane's 32 rows into registers, out[r] = row r.
+namespace fused {
+#define FLS_FUSED_NAME(B)                                                                                     \
+    __device__ __forceinline__ void unpack_##B##bw_32ow_32crw_1uf(const uint32_t* __restrict in,              \
+                                                                  uint32_t* __restrict out) {                 \
+        fls::dev::unpack_lane<B>(in, *reinterpret_cast<uint32_t(*)[32]>(out));                                \
+    }
+FLS_FOR_EACH_BW(FLS_FUSED_NAME)
+#undef FLS_FUSED_NAME
+
+// A width outside 0..32 (which the reference has no case for) decodes as zeros,
+// here and in normal::unpack.
+__device__ __forceinline__ void unpack(const uint32_t* __restrict in, uint32_t* __restrict out, uint8_t bw) {
+    fls::dev::unpack_device(in, *reinterpret_cast<uint32_t(*)[32]>(out), bw);
+}
+}  // namespace fused
+
+// unpack.cuh:7-3434: one lane's 32 rows into memory, out[32*r + lane] = row r.
+namespace normal {
+#define FLS_NORMAL_NAME(B)                                                                                    \
+    __device__ __forceinline__ void unpack_##B##bw_32ow_32crw_1uf(const uint32_t* __restrict in,              \
+                                                                  uint32_t* __restrict out) {                 \
+        fls::dev::unpack_lane_to<B>(in, out);                                                                 \
+    }
+FLS_FOR_EACH_BW(FLS_NORMAL_NAME)
+#undef FLS_NORMAL_NAME
+
+__device__ __forceinline__ void unpack(const uint32_t* __restrict in, uint32_t* __restrict out, uint8_t bw) {
+    switch (bw) {
+#define FLS_NORMAL_CASE(B) \
+    case B: unpack_##B##bw_32ow_32crw_1uf(in, out); break;
+        FLS_FOR_EACH_BW(FLS_NORMAL_CASE)
+#undef FLS_NORMAL_CASE
+    default:
+#pragma unroll
+        for (int r = 0; r < 32; ++r) out[32 * r + fls::dev::lane32()] = 0u;
+        break;
+    }
+}
+}  // namespace normal
+
+}  // namespace cuda
+}  // namespace unpack
+}  // namespace generated
+
+// unpack_fused.cuh:3455-3462 / unpack.cuh:3444-3451.  The reference picks the
+// register ("fused") or the memory ("normal") decode by which header it
+// includes; here the destination's type picks it: a 32-element array is one
+// lane's registers (out[r] = row r), a 1024-element array is the whole vector
+// in memory, e.g. in LDS (out[32*r + lane] = row r).
+template <typename T>
+__device__ __forceinline__ void unpack_device(const T* __restrict in, T (&out)[32], uint32_t bw) {
+    fls::dev::unpack_device(in, out, bw);
+}
+
+template <typename T>
+__device__ __forceinline__ void unpack_device(const T* __restrict in, T (&out)[1024], uint32_t bw) {
+    generated::unpack::cuda::normal::unpack(reinterpret_cast<const uint32_t*>(in), reinterpret_cast<uint32_t*>(out),
+                                            (uint8_t)bw);
+}
+
+// unpack_fused.cuh:3436-3442: out[j*BLOCK_THREADS + i] = registers[j].
+template <typename T, int BLOCK_THREADS, int ITEMS_PER_THREAD>
+__device__ __forceinline__ void load_registers(int i, T* out, const T* registers) {
+#pragma unroll
+    for (int j = 0; j < ITEMS_PER_THREAD; ++j) out[j * BLOCK_THREADS + i] = registers[j];
+}
+
+// unpack_fused.cuh:3444-3453: decode whole vectors of a fixed-width column,
+// vector v from in + 32*bw*v to out + 1024*v.  The reference launches one
+// 32-work-item block per vector; any block size that is a multiple of 32
+// works here (each half-wave takes the next vector), so a grid of
+// n_vec * 32 / blockDim.x blocks covers n_vec vectors.  A template with a
+// defaulted parameter so the header can be included by several translation
+// units; `unpack_global<<<grid, block>>>(in, out, bw)` launches it.
+template <int = 0>
+__global__ void unpack_global(const uint32_t* __restrict in, uint32_t* __restrict out, uint8_t bw) {
+    const uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / 32u) + threadIdx.x / 32u;
+    uint32_t registers[32];
+    fls::dev::unpack_device(in + v * 32u * bw, registers, bw);
+    load_registers<uint32_t, 32, 32>((int)fls::dev::lane32(), out + (v << 10), registers);
+}
+
+// hardcoded_16.cuh:436-786: quarter-vector decode, 8 rows per lane
+// (out[r] = row r of the quarter, i.e. row 8q + r of the vector).  The
+// reference defines widths 0, 4, ..., 24 and traps on others; this defines
+// every multiple of 4 up to 32 and leaves out zero for any other width (a
+// quarter of such a vector does not start on a word boundary).
+namespace unpack_8_at_a_time {
+#define FLS_QUARTER_NAME(B)                                                                                   \
+    __device__ __forceinline__ void unpack_##B##bw_32ow_32crw_1uf(const uint32_t* __restrict in,              \
+                                                                  uint32_t* __restrict out) {                 \
+        fls::dev::unpack_quarter_lane<B>(in, *reinterpret_cast<uint32_t(*)[8]>(out));                         \
+    }
+FLS_FOR_EACH_BW4(FLS_QUARTER_NAME)
+#undef FLS_QUARTER_NAME
+
+__device__ __forceinline__ void unpack(const uint32_t* __restrict in, uint32_t* __restrict out, uint8_t bw) {
+    switch (bw) {
+#define FLS_QUARTER_CASE(B) \
+    case B: unpack_##B##bw_32ow_32crw_1uf(in, out); break;
+        FLS_FOR_EACH_BW4(FLS_QUARTER_CASE)
+#undef FLS_QUARTER_CASE
+    default:
+#pragma unroll
+        for (int r = 0; r < 8; ++r) out[r] = 0u;
+        break;
+    }
+}
+
+__device__ __forceinline__ void unpack_device(const uint32_t* __restrict in, uint32_t* __restrict out, uint8_t bw) {
+    unpack(in, out, bw);
+}
+
+__device__ __forceinline__ void unpack_device(const int32_t* __restrict in, int32_t* __restrict out, uint8_t bw) {
+    unpack(reinterpret_cast<const uint32_t*>(in), reinterpret_cast<uint32_t*>(out), bw);
+}
+}  // namespace unpack_8_at_a_time
+
+// rsum.cuh:3-107: FastLanes delta decode of one vector in memory.  Lane l
+// walks its 32 positions l + O_j (O_j = 128*(j%8) + {0,64,32,96}[j/8]) in
+// chain order, out[p] = base[l] + in[first] + ... + in[p]; in and out hold the
+// vector's 1024 values in transposed order (in LDS or global memory).
+__device__ __forceinline__ void d_rsum_32(const uint32_t* in, uint32_t* out, const uint32_t* base) {
+    const uint32_t l = fls::dev::lane32();
+    uint32_t acc = base[l];
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
+        const uint32_t p = l + 32u * (uint32_t)fls::dev::rsum_row(j);
+        acc += in[p];
+        out[p] = acc;
+    }
+}
+
+#undef FLS_FOR_EACH_BW
+#undef FLS_FOR_EACH_BW4

@@ -39,7 +39,7 @@ enum {
     FLS_ERR_ARG = -6,       /* other invalid argument */
     FLS_ERR_CAPACITY = -7,  /* output buffer too small */
     FLS_ERR_NO_DEVICE = -8, /* no usable gfx950 device */
-    FLS_ERR_FORMAT = -9     /* serialized column: bad magic, version, sizes or checksum */
+    FLS_ERR_FORMAT = -9     /* column image or metadata: bad magic, version, sizes, checksum or vmeta */
 };
 
 #define FLS_VECTOR_SIZE 1024u
@@ -143,7 +143,9 @@ int fls_rle_pack_device(const uint32_t* d_in, uint64_t n, const fls_vmeta* d_vme
 int fls_unpack_u32(const uint32_t* d_in, uint32_t* d_out, uint64_t n_vec, uint32_t bw, void* stream);
 /* Same decode that also produces the order-sensitive checksum
  * sum_i (i+1)*value_i mod 2^64 over the first n_values values into *d_checksum
- * (overwritten).  d_out may be NULL to decode without materialising.          */
+ * (overwritten).  d_out may be NULL to decode without materialising; when
+ * given, it receives whole vectors and must hold ceil(n_values/1024)*1024
+ * words (the padding rows of the last vector are written too).              */
 int fls_unpack_u32_checksum(const uint32_t* d_in, uint32_t* d_out, uint64_t n_values, uint32_t bw,
                             uint64_t* d_checksum, void* stream);
 /* FastLanes delta decode: unpack at each vector's bw + d_rsum_32 with the
@@ -420,8 +422,11 @@ uint64_t fls_column_bytes(const fls_column* col, uint64_t n_rows);
  *   BITPACK 0 / 0 / 0, DELTA n_vec / 32 n_vec / 0, RLE n_vec / 32 n_vec /
  *   n_vec (fls_rle_meta), MIXED n_vec / 0 / n_vec.
  * Serialization reads host memory (copy device columns to the host first);
- * deserialization validates the header, every size and the checksum, and
- * returns a column whose pointers are views into the image (no copy).      */
+ * deserialization validates the header, every size, the checksum and every
+ * vector's metadata (widths 0..32, RLE run counts 1..1024, each vector's
+ * payload inside the words), and returns a column whose pointers are views
+ * into the image (no copy).  Serialization rejects a column whose metadata
+ * fails the same checks (FLS_ERR_FORMAT).                                   */
 typedef struct fls_column_image {
     char magic[8];
     uint32_t version, encoding, bw, base, transposed, reserved;

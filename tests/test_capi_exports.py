@@ -31,3 +31,15 @@ def test_every_declared_symbol_has_a_python_signature():
 
 def test_version():
     assert fls.load().fls_version().startswith(b"fls-mi355x")
+
+
+def test_reference_shaped_test_library_loads():
+    # the reference-shaped kernels compiled against include/fls/device (run by
+    # tests/test_gpu_dropin.py) load without a GPU and export every launcher
+    import ctypes
+
+    lib = ctypes.CDLL(str(T.ROOT / "tests" / "dropin" / "librefshape.so"))
+    for name in ("rs_q11", "rs_q21", "rs_q31", "rs_delta", "rs_unpack_named", "rs_unpack_quarters",
+                 "rs_unpack_global", "rs_prims", "rs_simd16", "rs_n_pred_out"):
+        assert hasattr(lib, name), name
+    assert lib.rs_n_pred_out() == 22

@@ -117,6 +117,27 @@ def test_q21_any_encoding_columns(layout):
         fls.query_q21_pht(*cols, n, pht)
 
 
+def test_q21_transposed_columns_partial_vector():
+    # transposed BITPACK columns with a partial last vector: the live rows of
+    # that vector are scattered over the lanes' registers, and the padding rows
+    # decode to key 1 (dimension row 0), which is made to qualify so that a
+    # padding row counted by mistake changes the result
+    n = 50_017
+    fact, dims = T.o_gen_c3(16, n, **SMALL)
+    fact[1] %= SMALL["n_supp"]
+    fact[2] %= SMALL["n_part"]
+    dims["s_region"][0] = 2
+    dims["p_category"][0] = 12
+    widths = (12, 16, 20, 24)
+    enc = [fls.HostColumn(fls.ENC_BITPACK, n, fls.pack_column(fact[k], widths[k], transpose=True), bw=widths[k],
+                          base=1, transposed=True) for k in range(3)]
+    enc.append(fls.bitpack_column(fact[3], widths[3], transposed=True))
+    want = T.o_q21(fact, dims, n)
+    out_c, out_p = run_both(fact, dims, SMALL, n, enc)
+    check(out_c, want)
+    check(out_p, want)
+
+
 @pytest.mark.parametrize("n", [40_000 + 3, 33 * 1024])
 def test_q21_every_row_qualifies(n):
     # all suppliers in the region and all parts in the category: the per-wave
