@@ -160,8 +160,9 @@ def read_bandwidth(n_bytes=1 << 30, reps=20):
     return n_bytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9, ok
 
 
-def bench_c1(steps, warmup, seed):
+def bench_c1(steps, warmup, seed, cpu=True):
     """Plain bit-unpack: 2^24 values at bw 11, decode + write + checksum."""
+    import numpy as np
     import torch
 
     import fastlanesgpu_amd as fls
@@ -177,13 +178,24 @@ def bench_c1(steps, warmup, seed):
     ms = statistics.mean(per)
     ok = torch.equal(d_out, vals)
     bytes_moved = words.size * 4 + n * 4
-    return {"workload": "2^24 u32 uniform [0,2^11), bw 11: decode + write + checksum", "call_ms": ms,
-            "values_per_s": n / (ms * 1e-3), "GBs_read_plus_write": bytes_moved / (ms * 1e-3) / 1e9,
-            "decoded_equal": bool(ok), "note": "call = zero-fill kernel + decode kernel"}
+    res = {"workload": "2^24 u32 uniform [0,2^11), bw 11: decode + write + checksum", "call_ms": ms,
+           "values_per_s": n / (ms * 1e-3), "GBs_read_plus_write": bytes_moved / (ms * 1e-3) / 1e9,
+           "decoded_equal": bool(ok), "note": "call = zero-fill kernel + decode kernel"}
+    if cpu:
+        T = testlib()
+        gpu_sum = int(d_sum.cpu().numpy().view(np.uint64)[0])
+        plain = vals.cpu().numpy().view(np.uint32)
+        res["cpu_baseline"] = cpu_leg(
+            "values/s", n, lambda t: T.o_c1_encoded(words, n, bw, threads=t), 7,
+            f"all {n} values (decode + write + checksum)", n, lambda: T.o_c1_encoded(words, n, bw, threads=1), 5,
+            f"all {n} values",
+            lambda a, b: a[1] == gpu_sum and b[1] == gpu_sum and np.array_equal(a[0], plain))
+    return res
 
 
-def bench_c3(steps, warmup, seed):
+def bench_c3(steps, warmup, seed, cpu=True):
     """Fused decode + semi-joins + group-by (Q2.1 shape), 120M rows, 4 dimension tables."""
+    import numpy as np
     import torch
 
     import fastlanesgpu_amd as fls
@@ -206,15 +218,32 @@ def bench_c3(steps, warmup, seed):
     fn2 = lambda: fls.query_q21_pht(*cols, n, pht, accumulate=True, d_out=out)  # noqa: E731
     _, per2 = time_kernel(fn2, out, max(3, steps // 5), 2)
     ms, ms2 = statistics.mean(per), statistics.mean(per2)
-    return {"workload": "fused decode+probe+group-by (Q2.1 shape), 120M rows; fks at 12/16/20 bits, revenue 24",
-            "encoded_bytes": nbytes, "kernel_ms_compact": ms, "rows_per_s_compact": n / (ms * 1e-3),
-            "GBs_compact": nbytes / (ms * 1e-3) / 1e9, "frac_hbm_compact": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "kernel_ms_pht": ms2, "rows_per_s_pht": n / (ms2 * 1e-3),
-            "groups": int((res_c[7000:] > 0).sum().item()), "paths_agree": bool(torch.equal(res_c, out))}
+    res = {"workload": "fused decode+probe+group-by (Q2.1 shape), 120M rows; fks at 12/16/20 bits, revenue 24",
+           "encoded_bytes": nbytes, "kernel_ms_compact": ms, "rows_per_s_compact": n / (ms * 1e-3),
+           "GBs_compact": nbytes / (ms * 1e-3) / 1e9, "frac_hbm_compact": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "kernel_ms_pht": ms2, "rows_per_s_pht": n / (ms2 * 1e-3),
+           "groups": int((res_c[7000:] > 0).sum().item()), "paths_agree": bool(torch.equal(res_c, out))}
+    if cpu:
+        T = testlib()
+        dims_h = {k: v.cpu().numpy().view(np.uint32) for k, v in dims.items() if not k.startswith("n_")}
+        sizes = {k: dims[k] for k in ("n_date", "n_supp", "n_part")}
+        gpu = res_c.cpu().numpy()
+        n_vec = (n + 1023) // 1024
+        one = n_vec // 16
+        full = lambda t: T.o_q21_encoded(enc, n, dims_h, sizes, threads=t)  # noqa: E731
+        part = lambda t: T.o_q21_encoded(enc, n, dims_h, sizes, threads=t, v_end=one)  # noqa: E731
+        ref_part = part(host_threads())
+        res["cpu_baseline"] = cpu_leg(
+            "rows/s", n, full, 5, f"full {n}-row table (decode 4 columns + dimension lookups + group-by)",
+            one * 1024, lambda: part(1), 3, f"first {one} vectors ({one * 1024} rows, 1/16 of the table)",
+            lambda a, b: np.array_equal(a[0], gpu[:7000]) and np.array_equal(a[1], gpu[7000:].view(np.uint64))
+            and np.array_equal(b[0], ref_part[0]) and np.array_equal(b[1], ref_part[1]))
+    return res
 
 
-def bench_star(steps, warmup, seed):
+def bench_star(steps, warmup, seed, cpu=True):
     """Fused decode + star joins + group-by (Q3.1 and Q4.1 shapes), 120M rows."""
+    import numpy as np
     import torch
 
     import fastlanesgpu_amd as fls
@@ -252,6 +281,24 @@ def bench_star(steps, warmup, seed):
                      "GBs": nbytes / (ms * 1e-3) / 1e9, "frac_hbm": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "groups": int((one[n_slots:] > 0).sum().item()), "rows_qualifying": int(one[n_slots:].sum()),
                      "repeat_consistent": bool(torch.equal(total, 3 * one))}
+        if cpu:
+            T = testlib()
+            dims_h = {k: v.cpu().numpy().view(np.uint32) for k, v in dims.items() if not k.startswith("n_")}
+            sizes = dict(fls.C3_DIMS)
+            gpu = one.cpu().numpy()
+            n_vec = (n + 1023) // 1024
+            part_v = n_vec // 16
+            if name == "q31":
+                q31c = [enc[i] for i in (0, 1, 2, 4)]
+                run = lambda t, ve=None: T.o_q31_encoded(q31c, n, dims_h, sizes, threads=t, v_end=ve)  # noqa: E731
+            else:
+                run = lambda t, ve=None: T.o_q41_encoded(enc, n, dims_h, sizes, threads=t, v_end=ve)  # noqa: E731
+            ref_part = run(host_threads(), part_v)
+            res[name]["cpu_baseline"] = cpu_leg(
+                "rows/s", n, run, 5, f"full {n}-row table", part_v * 1024, lambda: run(1, part_v), 3,
+                f"first {part_v} vectors ({part_v * 1024} rows, 1/16 of the table)",
+                lambda a, b: np.array_equal(a[0], gpu[:n_slots]) and np.array_equal(
+                    a[1], gpu[n_slots:].view(np.uint64)) and np.array_equal(b[0], ref_part[0]))
     return res
 
 
@@ -344,9 +391,10 @@ def bench_tile_vs_fastlanes(steps, warmup, seed):
     return res
 
 
-def bench_c4(steps, seed):
+def bench_c4(steps, seed, cpu=True):
     """4096 uneven segments (bitpack/delta/RLE) decoded + range-filtered:
     one graph-replayed persistent launch vs one launch per segment."""
+    import numpy as np
     import torch
 
     import fastlanesgpu_amd as fls
@@ -379,8 +427,25 @@ def bench_c4(steps, seed):
     run_ms, same_run = timed(lambda: b.run(best, d), max(5, steps))
     seg_ms, same_seg = timed(lambda: b.run_per_segment(best, d), 3)
     nbytes = info["encoded_bytes"]
+    cpu_res = None
+    if cpu:
+        T = testlib()
+        gpu_counts = ref.cpu().numpy().view(np.uint64)
+        all_ids = list(range(0, 4096, 8))
+        one_ids = list(range(0, 4096, 64))
+        segs = {s: b.segment_host(s) for s in all_ids}
+        work = lambda ids: sum(int(segs[s][1]) for s in ids)  # noqa: E731
+        run = lambda ids, t: T.o_c4_encoded([segs[s] for s in ids], threads=t)  # noqa: E731
+        cpu_res = cpu_leg(
+            "values/s", work(all_ids), lambda t: run(all_ids, t), 5,
+            f"every 8th segment ({len(all_ids)} segments, {work(all_ids)} values, decode + range count)",
+            work(one_ids), lambda: run(one_ids, 1), 3,
+            f"every 64th segment ({len(one_ids)} segments, {work(one_ids)} values)",
+            lambda a, c: np.array_equal(a, gpu_counts[all_ids]) and np.array_equal(c, gpu_counts[one_ids]))
+        del segs
     b.close()
-    return {"workload": "4096 segments, lengths log-uniform by octave in [2^10,2^24), bw U{1..32}, "
+    return {"cpu_baseline": cpu_res,
+            "workload": "4096 segments, lengths log-uniform by octave in [2^10,2^24), bw U{1..32}, "
                         "40/30/30 bitpack/delta/RLE, middle-half range filter, count per segment",
             "n_values": info["n_values"], "encoded_bytes": nbytes,
             "segments_by_encoding": [info["n_bitpack"], info["n_delta"], info["n_rle"]],
@@ -395,7 +460,7 @@ def bench_c4(steps, seed):
             "counts_agree": bool(same_run and same_seg)}
 
 
-def bench_c5(seed, n_rows, n_train=1 << 22, reps=3):
+def bench_c5(seed, n_rows, n_train=1 << 22, reps=3, cpu=True):
     """Learned encoding selector: train on 2^22 synthetic vectors (four
     families, exhaustive labels) with the library's own MFMA kernels, then
     encode a held-out 120M-row Q1.1 column set three ways -- learned (with the
@@ -504,7 +569,26 @@ def bench_c5(seed, n_rows, n_train=1 << 22, reps=3):
     del od, qt, dc, pr, m
     q_out = torch.zeros(2, dtype=torch.int64, device="cuda")
     q_ms = run(lambda: fls.query_q11(*mixed, n, d_out=q_out, sync=False))
-    return {
+    cpu_res = None
+    if cpu:
+        import numpy as np
+
+        T = testlib()
+        k_all, k_one = 2048, 256  # vectors per column
+        sample = [c[:k_all * 1024].cpu().numpy().view(np.uint32) for c in cols]
+        gpu_cls = [w.cls[:k_all].cpu().numpy() for w in ws_x]
+        vec_all = np.concatenate(sample)
+        vec_one = np.concatenate([x[:k_one * 1024] for x in sample])
+        cpu_res = cpu_leg(
+            "GB/s", 4 * vec_all.size / 1e9, lambda t: T.o_exhaustive_classes(vec_all, threads=t), 3,
+            f"exhaustive search (all four encoders per vector) over the first {k_all} vectors of each of the "
+            f"{len(cols)} held-out columns ({vec_all.size // 1024} vectors)",
+            4 * vec_one.size / 1e9, lambda: T.o_exhaustive_classes(vec_one, threads=1), 3,
+            f"the first {k_one} vectors of each column ({vec_one.size // 1024} vectors)",
+            lambda a, b: np.array_equal(a, np.concatenate(gpu_cls))
+            and np.array_equal(b, np.concatenate([g[:k_one] for g in gpu_cls])))
+    return {"cpu_baseline_exhaustive": cpu_res,
+            
         "workload": f"train on {n_train} vectors (uniform / Zipf 1.1 / sorted+N(0,16) / runs mean 32), "
                     f"encode held-out Q1.1 column set {len(cols)} x {n} u32",
         "train": {"gen_features_labels_ms": gen_ms, "epochs": epochs, "batch": batch, "adam_steps": steps,
@@ -535,6 +619,43 @@ def cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+def host_threads():
+    return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+
+
+def testlib():
+    """The oracle's ctypes wrappers (tests/fls_testlib.py): the CPU baselines
+    are the oracle's scalar decode + query."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import fls_testlib as T
+
+    return T
+
+
+def median_run(fn, runs):
+    times, res = [], None
+    for _ in range(runs):
+        t = time.perf_counter()
+        res = fn()
+        times.append(time.perf_counter() - t)
+    return statistics.median(times), res
+
+
+def cpu_leg(unit, work_all, run_all, runs_all, sample_all, work_one, run_one, runs_one, sample_one, matches):
+    """Oracle baseline of one workload on all host threads and on one thread.
+    run_all(threads) / run_one() return results; matches(res_all, res_one)
+    says whether they equal the GPU's."""
+    threads = host_threads()
+    run_all(threads)  # warm-up
+    t_all, r_all = median_run(lambda: run_all(threads), runs_all)
+    t_one, r_one = median_run(run_one, runs_one)
+    return {"value": work_all / t_all, "unit": unit, "cores": threads, "kind": "port",
+            "sample": f"{sample_all}, median of {runs_all} runs on {threads} threads after one warm-up",
+            "single_core_value": work_one / t_one,
+            "single_core_sample": f"{sample_one}, median of {runs_one} runs on 1 thread",
+            "result_matches_gpu": bool(matches(r_all, r_one))}
 
 
 def cpu_baseline(enc, n_rows, gpu_result):
@@ -693,13 +814,14 @@ def main():
         line["secondary"] = {}
         if want("c1"):
             log("[bench] secondary: plain unpack")
-            line["secondary"]["c1_unpack"] = bench_c1(args.steps, args.warmup, args.seed)
+            line["secondary"]["c1_unpack"] = bench_c1(args.steps, args.warmup, args.seed, cpu=not args.no_cpu)
         if want("c3"):
             log("[bench] secondary: probe + group-by")
-            line["secondary"]["c3_probe_groupby"] = bench_c3(max(5, args.steps // 2), 2, args.seed)
+            line["secondary"]["c3_probe_groupby"] = bench_c3(max(5, args.steps // 2), 2, args.seed,
+                                                             cpu=not args.no_cpu)
         if want("star"):
             log("[bench] secondary: star joins")
-            line["secondary"]["star_joins"] = bench_star(max(5, args.steps // 2), 2, args.seed)
+            line["secondary"]["star_joins"] = bench_star(max(5, args.steps // 2), 2, args.seed, cpu=not args.no_cpu)
         if want("materialize"):
             log("[bench] secondary: materialize")
             line["secondary"]["materialize"] = bench_materialize(max(5, args.steps // 2), 2, args.seed)
@@ -708,10 +830,11 @@ def main():
             line["secondary"]["tile_vs_fastlanes"] = bench_tile_vs_fastlanes(max(5, args.steps // 2), 2, args.seed)
         if want("c4"):
             log("[bench] secondary: batched segments")
-            line["secondary"]["c4_batched"] = bench_c4(max(5, args.steps // 5), args.seed)
+            line["secondary"]["c4_batched"] = bench_c4(max(5, args.steps // 5), args.seed, cpu=not args.no_cpu)
         if want("c5") and not args.no_c5:
             log("[bench] secondary: learned encoding selector")
-            line["secondary"]["c5_learned_selector"] = bench_c5(args.seed, n, n_train=args.c5_train_vectors)
+            line["secondary"]["c5_learned_selector"] = bench_c5(args.seed, n, n_train=args.c5_train_vectors,
+                                                                cpu=not args.no_cpu)
     print(json.dumps(line), flush=True)
 
 

@@ -662,6 +662,24 @@ class Batch:
         call("fls_batch_segment", self._h, s, ctypes.byref(i))
         return {k: getattr(i, k) for k, _ in SegmentInfo._fields_}
 
+    def segment_host(self, s):
+        """Segment s copied to host memory: a HostColumn-like object (numpy
+        words, vmeta, bases and RLE metadata as uint64 aux) plus n_values, lo,
+        hi (fls_batch_segment_column, then device-to-host copies)."""
+        import numpy as np
+
+        seg = Segment()
+        call("fls_batch_segment_column", self._h, s, ctypes.byref(seg))
+        c = seg.col
+        nv = (seg.n_values + 1023) // 1024
+        fetch = lambda ptr, n, dt: _device_to_host(ptr, n, dt) if ptr and n else None  # noqa: E731
+        hc = HostColumn(c.encoding, seg.n_values, fetch(c.words, c.n_words, np.uint32), bw=c.bw, base=c.base,
+                        transposed=bool(c.transposed),
+                        vmeta=fetch(c.vmeta, nv, np.uint64) if c.encoding != ENC_BITPACK else None,
+                        bases=fetch(c.bases, 32 * nv, np.uint32) if c.encoding in (ENC_DELTA, ENC_RLE) else None,
+                        aux=fetch(c.aux, nv, np.uint64) if c.encoding in (ENC_RLE, ENC_MIXED) else None)
+        return hc, seg.n_values, seg.lo, seg.hi
+
     def run(self, tune, d_counts, stream=None):
         call("fls_batch_run", self._h, ctypes.byref(tune), dptr(d_counts), _stream(stream))
 
@@ -692,6 +710,20 @@ class Batch:
 
 def tune(tile_vectors=64, block_threads=256, schedule=SCHED_STATIC):
     return Tune(tile_vectors, block_threads, schedule, 0)
+
+
+def _device_to_host(ptr, n, dtype):
+    """n elements of dtype at device address ptr, copied into a new numpy
+    array (hipMemcpy on the HIP runtime this process already uses)."""
+    import numpy as np
+
+    out = np.empty(n, dtype)
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    st = hip.hipMemcpy(out.ctypes.data, ptr, out.nbytes, 2)  # hipMemcpyDeviceToHost
+    if st != 0:
+        raise RuntimeError(f"hipMemcpy device-to-host failed: {st}")
+    return out
 
 
 def batch_c4(seed, n_segments=4096, log2_min_len=10, log2_max_len=24, stream=None):

@@ -251,6 +251,7 @@ SIGNATURES.update({
     "fls_gen_c4_segment": (c_int, [c_u64, c_u32, c_u32, c_u32, c_vp, c_vp]),
     "fls_batch_get_info": (c_int, [c_vp, ctypes.POINTER(BatchInfo)]),
     "fls_batch_segment": (c_int, [c_vp, c_u32, ctypes.POINTER(SegmentInfo)]),
+    "fls_batch_segment_column": (c_int, [c_vp, c_u32, ctypes.POINTER(Segment)]),
     "fls_batch_run": (c_int, [c_vp, _TP, c_vp, c_vp]),
     "fls_batch_run_per_segment": (c_int, [c_vp, _TP, c_vp, c_vp]),
     "fls_batch_graph_launch": (c_int, [c_vp, _TP, c_vp, c_vp]),

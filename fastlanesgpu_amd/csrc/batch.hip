@@ -558,6 +558,28 @@ int fls_batch_segment(const fls_batch* b, uint32_t s, fls_segment_info* info) {
     return FLS_OK;
 }
 
+int fls_batch_segment_column(const fls_batch* b, uint32_t s, fls_segment* seg) {
+    if (!b || !seg) return FLS_ERR_NULL;
+    if (s >= b->segs.size()) return FLS_ERR_ARG;
+    const fls::SegDev& d = b->segs[s];
+    fls_segment g{};
+    g.col.encoding = d.enc;
+    g.col.bw = d.bw;
+    g.col.base = 0;
+    g.col.transposed = d.transposed;
+    g.col.n_words = b->info[s].n_words;
+    g.col.words = d.words;
+    g.col.vmeta = d.vmeta;
+    g.col.bases = d.bases;
+    g.col.aux = d.enc == FLS_ENC_RLE ? reinterpret_cast<const uint64_t*>(d.rmeta) : d.aux;
+    g.rmeta = d.rmeta;
+    g.n_values = d.n_values;
+    g.lo = d.lo;
+    g.hi = d.hi;
+    *seg = g;
+    return FLS_OK;
+}
+
 int fls_batch_run(fls_batch* b, const fls_tune* tune, uint64_t* d_counts, void* stream) {
     if (!b || !d_counts) return FLS_ERR_NULL;
     return fls::run_batch(b, tune, d_counts, fls::as_stream(stream));

@@ -285,6 +285,7 @@ int fls_query_q21(const fls_column* datekey, const fls_column* suppkey, const fl
                   const fls_column* revenue, uint64_t n_rows, const fls_q21_dims* dims, const void* d_workspace,
                   const fls_q21_params* params, int64_t* d_sums, uint64_t* d_counts, uint32_t flags, void* stream);
 
+
 /* ============================================ star joins (Q3.1, Q4.1)
  * Dense-key dimension columns for the three- and four-way star joins: row i of
  * each dimension holds key keys_min + i, and each used dimension holds at
@@ -564,6 +565,9 @@ int fls_gen_c4_segment(uint64_t seed, uint32_t s, uint32_t log2_min_len, uint32_
                        void* stream);
 int fls_batch_get_info(const fls_batch* b, fls_batch_info* info);
 int fls_batch_segment(const fls_batch* b, uint32_t s, fls_segment_info* info);
+/* Segment s as fls_batch_create takes it: device pointers borrowed from the
+ * batch (valid until fls_batch_destroy), e.g. to copy a segment out.        */
+int fls_batch_segment_column(const fls_batch* b, uint32_t s, fls_segment* seg);
 /* one persistent launch (after two tiny zero-fill kernels) */
 int fls_batch_run(fls_batch* b, const fls_tune* tune, uint64_t* d_counts, void* stream);
 /* baseline: one launch per segment */

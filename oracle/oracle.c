@@ -407,6 +407,24 @@ void orc_decode_vector(const orc_col* c, uint64_t v, uint32_t* out) {
         uint32_t d[1024];
         orc_unpack_lanewise_u32(c->words + (m >> 8), d, (unsigned)(m & 0xff));
         orc_rsum_u32(d, out, c->bases + 32 * v);
+    } else if (c->encoding == 2) {
+        /* run index: delta-coded like a delta vector; run values after it */
+        const uint64_t m = c->vmeta[v], a = c->aux[v];
+        const unsigned ibw = (unsigned)(m & 0xff), vbw = (unsigned)((a >> 48) & 0xff);
+        const uint32_t vbase = (uint32_t)a, n_runs = (uint32_t)((a >> 32) & 0xffff);
+        const uint32_t* vals = c->words + (m >> 8) + 32u * ibw;
+        const uint32_t mask = vbw >= 32 ? 0xffffffffu : ((1u << vbw) - 1u);
+        uint32_t d[1024], idx[1024], run[1024];
+        for (uint32_t k = 0; k < n_runs && k < 1024; ++k) { /* word-window extraction */
+            const uint64_t bit = (uint64_t)k * vbw;
+            const uint32_t w = (uint32_t)(bit / 32), sh = (uint32_t)(bit % 32);
+            uint32_t x = vbw ? vals[w] >> sh : 0;
+            if (vbw && sh + vbw > 32) x |= vals[w + 1] << (32 - sh);
+            run[k] = vbase + (x & mask);
+        }
+        orc_unpack_lanewise_u32(c->words + (m >> 8), d, ibw);
+        orc_rsum_u32(d, idx, c->bases + 32 * v);
+        for (unsigned i = 0; i < 1024; ++i) out[i] = run[idx[i] < n_runs ? idx[i] : n_runs - 1u];
     } else {
         orc_unpack_lanewise_u32(c->words + v * 32 * c->bw, out, c->bw);
         if (c->base)

@@ -163,13 +163,14 @@ void orc_c4_counts(uint64_t seed, uint32_t n_seg, uint32_t lmin, uint32_t lmax, 
  * for delta, scalar predicate/aggregate loop; OpenMP over vector ranges.
  * Pinned against the bit-by-bit functions above.                             */
 typedef struct orc_col {
-    uint32_t encoding;   /* 0 bitpack(+FOR base), 1 delta */
+    uint32_t encoding;   /* 0 bitpack(+FOR base), 1 delta, 2 RLE */
     uint32_t bw;
     uint32_t base;
     uint32_t transposed;
     const uint32_t* words;
-    const uint64_t* vmeta; /* (offset << 8) | bw */
+    const uint64_t* vmeta; /* (offset << 8) | bw (RLE: the run-index delta width) */
     const uint32_t* bases;
+    const uint64_t* aux;   /* RLE: vbase | n_runs << 32 | vbw << 48 | ibw << 56 per vector */
 } orc_col;
 
 void orc_unpack_lanewise_u32(const uint32_t* in, uint32_t* out, unsigned bw);
@@ -177,6 +178,39 @@ void orc_decode_vector(const orc_col* c, uint64_t v, uint32_t* out);
 int64_t orc_q11_encoded(const orc_col* od, const orc_col* q, const orc_col* d, const orc_col* p, uint64_t n_rows,
                         uint64_t v_begin, uint64_t v_end, uint32_t date_gt, uint32_t date_lt, uint32_t qty_lt,
                         uint32_t disc_lo, uint32_t disc_hi, int n_threads, uint64_t* count);
+
+/* Same-run CPU baselines of the other workloads (baseline.c), each the
+ * oracle's scalar decode (orc_decode_vector) followed by the scalar query of
+ * the matching plain function above, OpenMP over vector (or segment) ranges:
+ *   orc_c1_encoded    decode fixed-width vectors to out[] + orc_checksum_u32;
+ *   orc_q21_encoded   orc_q21 over four encoded columns (dense keys
+ *                     keys_min + i; keys outside the dimension do not match);
+ *   orc_q31_encoded   orc_q31, orc_q41_encoded orc_q41 likewise;
+ *   orc_c4_encoded    per-segment count of lo <= value <= hi over encoded
+ *                     segments (the batched-segments query);
+ *   orc_exhaustive_classes  orc_exhaustive_class per vector (the
+ *                     try-all-encodings search the learned selector replaces). */
+uint64_t orc_c1_encoded(const uint32_t* words, uint64_t n_values, unsigned bw, uint32_t* out, int n_threads);
+void orc_q21_encoded(const orc_col* date, const orc_col* supp, const orc_col* part, const orc_col* rev,
+                     uint64_t n_rows, uint64_t v_begin, uint64_t v_end, const uint32_t* d_year,
+                     const uint32_t* s_region, const uint32_t* p_category, const uint32_t* p_brand, uint32_t n_date,
+                     uint32_t n_supp, uint32_t n_part, uint32_t keys_min, uint32_t cat, uint32_t reg, int n_threads,
+                     int64_t* sums, uint64_t* counts);
+void orc_q31_encoded(const orc_col* date, const orc_col* cust, const orc_col* supp, const orc_col* rev,
+                     uint64_t n_rows, uint64_t v_begin, uint64_t v_end, const uint32_t* d_year,
+                     const uint32_t* s_region, const uint32_t* s_nation, const uint32_t* c_region,
+                     const uint32_t* c_nation, uint32_t n_date, uint32_t n_supp, uint32_t n_cust, uint32_t keys_min,
+                     uint32_t reg_c, uint32_t reg_s, uint32_t year_lo, uint32_t year_hi, int n_threads,
+                     int64_t* sums, uint64_t* counts);
+void orc_q41_encoded(const orc_col* date, const orc_col* cust, const orc_col* supp, const orc_col* part,
+                     const orc_col* rev, const orc_col* cost, uint64_t n_rows, uint64_t v_begin, uint64_t v_end,
+                     const uint32_t* d_year, const uint32_t* s_region, const uint32_t* c_region,
+                     const uint32_t* c_nation, const uint32_t* p_mfgr, uint32_t n_date, uint32_t n_supp,
+                     uint32_t n_cust, uint32_t n_part, uint32_t keys_min, uint32_t reg_c, uint32_t reg_s,
+                     uint32_t mfgr_lo, uint32_t mfgr_hi, int n_threads, int64_t* sums, uint64_t* counts);
+void orc_c4_encoded(const orc_col* segs, const uint64_t* n_values, const uint32_t* lo, const uint32_t* hi,
+                    uint32_t n_seg, int n_threads, uint64_t* counts);
+void orc_exhaustive_classes(const uint32_t* in, uint64_t n_vec, uint8_t* cls, int n_threads);
 
 /* ---------------------------------------------------------------- selector
  * (oracle/selector.c; no reference counterpart.)  Mixed-vector encodings:

@@ -196,15 +196,106 @@ def from_dev(t, dtype=np.uint32):
 class OrcCol(ctypes.Structure):
     _fields_ = [("encoding", ctypes.c_uint32), ("bw", ctypes.c_uint32), ("base", ctypes.c_uint32),
                 ("transposed", ctypes.c_uint32), ("words", ctypes.c_void_p), ("vmeta", ctypes.c_void_p),
-                ("bases", ctypes.c_void_p)]
+                ("bases", ctypes.c_void_p), ("aux", ctypes.c_void_p)]
 
 
 def orc_col(hc):
-    """oracle view (orc_col) of a fastlanesgpu_amd.HostColumn; keeps arrays alive via the returned tuple."""
-    keep = [a for a in (hc.words, hc.vmeta, hc.bases) if a is not None]
+    """oracle view (orc_col) of a fastlanesgpu_amd.HostColumn (or any object
+    with encoding, bw, base, transposed, words, vmeta, bases, aux numpy
+    attributes); keeps arrays alive via the returned tuple."""
+    aux = getattr(hc, "aux", None)
+    keep = [a for a in (hc.words, hc.vmeta, hc.bases, aux) if a is not None]
     ptr = lambda a: None if a is None else a.ctypes.data  # noqa: E731
-    c = OrcCol(hc.encoding, hc.bw, hc.base, int(hc.transposed), ptr(hc.words), ptr(hc.vmeta), ptr(hc.bases))
+    c = OrcCol(hc.encoding, hc.bw, hc.base, int(hc.transposed), ptr(hc.words), ptr(hc.vmeta), ptr(hc.bases),
+               ptr(aux))
     return c, keep
+
+
+def _baseline_sigs():
+    o = oracle()
+    if hasattr(o, "_baseline_set"):
+        return o
+    P = ctypes.POINTER(OrcCol)
+    u32, u64, i = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+    o.orc_c1_encoded.argtypes = [U32P, u64, ctypes.c_uint, U32P, i]
+    o.orc_c1_encoded.restype = u64
+    o.orc_q21_encoded.argtypes = [P] * 4 + [u64, u64, u64] + [U32P] * 4 + [u32] * 6 + [i, I64P, U64P]
+    o.orc_q31_encoded.argtypes = [P] * 4 + [u64, u64, u64] + [U32P] * 5 + [u32] * 8 + [i, I64P, U64P]
+    o.orc_q41_encoded.argtypes = [P] * 6 + [u64, u64, u64] + [U32P] * 5 + [u32] * 9 + [i, I64P, U64P]
+    o.orc_c4_encoded.argtypes = [P, U64P, U32P, U32P, u32, i, U64P]
+    o.orc_exhaustive_classes.argtypes = [U32P, u64, np.ctypeslib.ndpointer(dtype=np.uint8, flags="C"), i]
+    o._baseline_set = True
+    return o
+
+
+def o_c1_encoded(words, n, bw, threads=8):
+    out = np.zeros(max(1, n), np.uint32)
+    s = _baseline_sigs().orc_c1_encoded(np.ascontiguousarray(words, np.uint32), n, bw, out, threads)
+    return out[:n], int(s)
+
+
+def _vrange(n_rows, v_begin, v_end):
+    return v_begin, (n_rows + 1023) // 1024 if v_end is None else v_end
+
+
+def o_q21_encoded(cols, n_rows, dims, sizes, keys_min=1, cat=12, reg=2, threads=8, v_begin=0, v_end=None):
+    cs = [orc_col(c) for c in cols]
+    sums, counts = np.zeros(7000, np.int64), np.zeros(7000, np.uint64)
+    d = [np.ascontiguousarray(dims[k], np.uint32) for k in ("d_year", "s_region", "p_category", "p_brand")]
+    _baseline_sigs().orc_q21_encoded(*(ctypes.byref(c[0]) for c in cs), n_rows, *_vrange(n_rows, v_begin, v_end),
+                                     *d, sizes["n_date"], sizes["n_supp"], sizes["n_part"], keys_min, cat, reg,
+                                     threads, sums, counts)
+    return sums, counts
+
+
+def o_q31_encoded(cols, n_rows, dims, sizes, keys_min=1, reg_c=3, reg_s=3, year_lo=1992, year_hi=1997, threads=8,
+                  v_begin=0, v_end=None):
+    """cols: date, customer, supplier, revenue."""
+    cs = [orc_col(c) for c in cols]
+    sums, counts = np.zeros(4375, np.int64), np.zeros(4375, np.uint64)
+    d = [np.ascontiguousarray(dims[k], np.uint32) for k in ("d_year", "s_region", "s_nation", "c_region",
+                                                            "c_nation")]
+    _baseline_sigs().orc_q31_encoded(*(ctypes.byref(c[0]) for c in cs), n_rows, *_vrange(n_rows, v_begin, v_end),
+                                     *d, sizes["n_date"], sizes["n_supp"], sizes["n_cust"], keys_min, reg_c, reg_s,
+                                     year_lo, year_hi, threads, sums, counts)
+    return sums, counts
+
+
+def o_q41_encoded(cols, n_rows, dims, sizes, keys_min=1, reg_c=2, reg_s=2, mfgr_lo=1, mfgr_hi=2, threads=8,
+                  v_begin=0, v_end=None):
+    """cols: date, customer, supplier, part, revenue, supplycost."""
+    cs = [orc_col(c) for c in cols]
+    sums, counts = np.zeros(175, np.int64), np.zeros(175, np.uint64)
+    d = [np.ascontiguousarray(dims[k], np.uint32) for k in ("d_year", "s_region", "c_region", "c_nation", "p_mfgr")]
+    _baseline_sigs().orc_q41_encoded(*(ctypes.byref(c[0]) for c in cs), n_rows, *_vrange(n_rows, v_begin, v_end),
+                                     *d, sizes["n_date"], sizes["n_supp"], sizes["n_cust"], sizes["n_part"],
+                                     keys_min, reg_c, reg_s, mfgr_lo, mfgr_hi, threads, sums, counts)
+    return sums, counts
+
+
+def o_c4_encoded(segs, threads=8):
+    """segs: list of (column-like with numpy arrays, n_values, lo, hi)."""
+    n = len(segs)
+    arr = (OrcCol * max(1, n))()
+    keep = []
+    for k, (c, _, _, _) in enumerate(segs):
+        oc, kp = orc_col(c)
+        arr[k] = oc
+        keep.append(kp)
+    nv = np.array([s[1] for s in segs], np.uint64)
+    lo = np.array([s[2] for s in segs], np.uint32)
+    hi = np.array([s[3] for s in segs], np.uint32)
+    counts = np.zeros(max(1, n), np.uint64)
+    _baseline_sigs().orc_c4_encoded(arr, nv, lo, hi, n, threads, counts)
+    return counts[:n]
+
+
+def o_exhaustive_classes(vectors, threads=8):
+    v = np.ascontiguousarray(vectors, np.uint32)
+    n_vec = v.size // 1024
+    cls = np.zeros(max(1, n_vec), np.uint8)
+    _baseline_sigs().orc_exhaustive_classes(v, n_vec, cls, threads)
+    return cls[:n_vec]
 
 
 def o_q11_encoded(cols, n_rows, v_begin=0, v_end=None, threads=8, params=(19930000, 19940000, 25, 1, 3)):
