@@ -413,6 +413,19 @@ def query_q21(date, supp, part, rev, n_rows, dims, ws, params=None, accumulate=F
     return d_out
 
 
+def query_q21_noalloc(date, supp, part, rev, n_rows, dims, params=None, accumulate=False, d_out=None, stream=None):
+    """query_q21 without a workspace: the library builds the compact tables
+    into its static per-GPU scratch array (fls_query_q21_noalloc; calls that
+    overlap on one GPU race).  Allocates nothing itself when d_out is given."""
+    params = params or q21_params()
+    d_out, ps, pc = _q21_out(params, d_out)
+    cols = [c.struct() for c in (date, supp, part, rev)]
+    ds = q21_dims_struct(dims)
+    call("fls_query_q21_noalloc", *(ctypes.byref(c) for c in cols), n_rows, ctypes.byref(ds), ctypes.byref(params),
+         ps, pc, Q_ACCUMULATE if accumulate else 0, _stream(stream))
+    return d_out
+
+
 def build_pht(d_filter, filter_eq, d_keys, d_vals, n, d_ht, ht_len, keys_min, stream=None):
     call("fls_build_pht_u32", dptr(d_filter), filter_eq, dptr(d_keys), dptr(d_vals), n, dptr(d_ht), ht_len,
          keys_min, _stream(stream))

@@ -142,6 +142,8 @@ SIGNATURES.update({
     "fls_q21_build": (c_int, [ctypes.POINTER(Q21Dims), _Q21P, c_vp, c_u64, c_vp]),
     "fls_query_q21": (c_int, [COLP, COLP, COLP, COLP, c_u64, ctypes.POINTER(Q21Dims), c_vp, _Q21P, c_vp, c_vp,
                               c_u32, c_vp]),
+    "fls_query_q21_noalloc": (c_int, [COLP, COLP, COLP, COLP, c_u64, ctypes.POINTER(Q21Dims), _Q21P, c_vp, c_vp,
+                                      c_u32, c_vp]),
 })
 
 class StarDims(ctypes.Structure):

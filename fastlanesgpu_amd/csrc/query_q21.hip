@@ -20,6 +20,8 @@
 // (fls_q21.cu:71).
 #include <string.h>
 
+#include <mutex>
+
 #include "fls/device/crystal.hpp"
 #include "fls/device/unpack.hpp"
 #include "fls/device/wave.hpp"
@@ -412,6 +414,57 @@ int fill_groups(Q21Dev& a, const fls_q21_params* p, int64_t* d_sums, uint64_t* d
     return FLS_OK;
 }
 
+// Shared by fls_q21_build and the allocation-free entry point: clear both
+// bitmaps with a kernel, then reduce the dimensions into the workspace.
+int q21_build_into(const fls_q21_dims* dims, const fls_q21_params* params, char* ws, const Q21Layout& L,
+                   hipStream_t s) {
+    const uint32_t words = (uint32_t)(L.brand / 4u);  // both bitmaps
+    hipLaunchKernelGGL(q21_zero_words, dim3(256), dim3(256), 0, s, reinterpret_cast<uint32_t*>(ws), words);
+    hipLaunchKernelGGL(q21_build_compact, dim3(1024), dim3(256), 0, s, dims->s_region, dims->n_supp,
+                       dims->p_category, dims->p_brand, dims->n_part, dims->d_year, dims->n_date, params->region,
+                       params->category, reinterpret_cast<uint32_t*>(ws + L.supp_bm),
+                       reinterpret_cast<uint32_t*>(ws + L.part_bm), reinterpret_cast<uint16_t*>(ws + L.brand),
+                       reinterpret_cast<uint16_t*>(ws + L.year));
+    return check_launch();
+}
+
+int q21_check_dims(const fls_q21_dims* dims) {
+    FLS_RETURN_IF(!dims->s_region || !dims->p_category || !dims->p_brand || !dims->d_year, FLS_ERR_NULL);
+    FLS_RETURN_IF(!dims->n_supp || !dims->n_part || !dims->n_date, FLS_ERR_ARG);
+    return FLS_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------- static scratch
+// Device memory of the allocation-free Q2.1 entry point, which takes no
+// workspace: its compact dimension tables are built here.  One copy per GPU,
+// held until the process ends.  In host code `scratch` names a host-side
+// stand-in, not device memory: kernels get the address static_scratch()
+// looks up for the current device.
+__device__ __align__(256) unsigned char scratch[FLS_Q21_STATIC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];
+
+// Device address of the current GPU's copy of `scratch`, looked up on the
+// first call on each device (with that device current) and kept.
+int static_scratch(void** ptr) {
+    int dev = 0;
+    FLS_HIP_CHECK(hipGetDevice(&dev));
+    FLS_RETURN_IF(dev < 0 || dev >= kMaxDevices, FLS_ERR_NO_DEVICE);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        FLS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        FLS_RETURN_IF(!p, FLS_ERR_HIP);
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return FLS_OK;
+}
 }  // namespace
 }  // namespace fls
 
@@ -464,20 +517,10 @@ int fls_q21_build(const fls_q21_dims* dims, const fls_q21_params* params, void* 
                   void* stream) {
     using namespace fls;
     FLS_RETURN_IF(!dims || !params || !d_workspace, FLS_ERR_NULL);
-    FLS_RETURN_IF(!dims->s_region || !dims->p_category || !dims->p_brand || !dims->d_year, FLS_ERR_NULL);
-    FLS_RETURN_IF(!dims->n_supp || !dims->n_part || !dims->n_date, FLS_ERR_ARG);
+    if (const int st = q21_check_dims(dims)) return st;
     const Q21Layout L = q21_layout(dims->n_supp, dims->n_part, dims->n_date);
     FLS_RETURN_IF(ws_bytes < L.total, FLS_ERR_CAPACITY);
-    char* ws = static_cast<char*>(d_workspace);
-    hipStream_t s = as_stream(stream);
-    const uint32_t words = (uint32_t)(L.brand / 4u);  // both bitmaps
-    hipLaunchKernelGGL(q21_zero_words, dim3(256), dim3(256), 0, s, reinterpret_cast<uint32_t*>(ws), words);
-    hipLaunchKernelGGL(q21_build_compact, dim3(1024), dim3(256), 0, s, dims->s_region, dims->n_supp,
-                       dims->p_category, dims->p_brand, dims->n_part, dims->d_year, dims->n_date, params->region,
-                       params->category, reinterpret_cast<uint32_t*>(ws + L.supp_bm),
-                       reinterpret_cast<uint32_t*>(ws + L.part_bm), reinterpret_cast<uint16_t*>(ws + L.brand),
-                       reinterpret_cast<uint16_t*>(ws + L.year));
-    return check_launch();
+    return q21_build_into(dims, params, static_cast<char*>(d_workspace), L, as_stream(stream));
 }
 
 int fls_query_q21(const fls_column* datekey, const fls_column* suppkey, const fls_column* partkey,
@@ -517,6 +560,26 @@ int fls_query_q21(const fls_column* datekey, const fls_column* suppkey, const fl
         return launch_resident_q21(k, 1024, lds, a, s);
     }
     return launch_resident_q21(q21_any_kernel<false>, 1024, 0, a, s);
+}
+
+int fls_query_q21_noalloc(const fls_column* datekey, const fls_column* suppkey, const fls_column* partkey,
+                          const fls_column* revenue, uint64_t n_rows, const fls_q21_dims* dims,
+                          const fls_q21_params* params, int64_t* d_sums, uint64_t* d_counts, uint32_t flags,
+                          void* stream) {
+    using namespace fls;
+    FLS_RETURN_IF(!dims || !params, FLS_ERR_NULL);
+    if (const int st = q21_check_dims(dims)) return st;
+    const Q21Layout L = q21_layout(dims->n_supp, dims->n_part, dims->n_date);
+    FLS_RETURN_IF(L.total > FLS_Q21_STATIC_SCRATCH_BYTES, FLS_ERR_SCRATCH);
+    Q21Dev probe{};  // every argument is checked before the first launch
+    if (const int st = fill_fact(probe, datekey, suppkey, partkey, revenue, n_rows, true)) return st;
+    FLS_RETURN_IF(!d_sums || !d_counts, FLS_ERR_NULL);
+    FLS_RETURN_IF(params->n_slots == 0 || params->n_years == 0, FLS_ERR_ARG);
+    void* ws = nullptr;
+    if (const int st = static_scratch(&ws)) return st;
+    if (const int st = q21_build_into(dims, params, static_cast<char*>(ws), L, as_stream(stream))) return st;
+    return fls_query_q21(datekey, suppkey, partkey, revenue, n_rows, dims, ws, params, d_sums, d_counts, flags,
+                         stream);
 }
 
 }  // extern "C"

@@ -121,6 +121,7 @@ const char* fls_status_string(int status) {
         case FLS_ERR_CAPACITY: return "output buffer too small";
         case FLS_ERR_NO_DEVICE: return "no usable gfx950 device";
         case FLS_ERR_FORMAT: return "serialized column is malformed (magic, version, sizes, checksum or vector metadata)";
+        case FLS_ERR_SCRATCH: return "the work exceeds the library's static scratch array (allocation-free entry point)";
         default: return "unknown status";
     }
 }

@@ -39,7 +39,8 @@ enum {
     FLS_ERR_ARG = -6,       /* other invalid argument */
     FLS_ERR_CAPACITY = -7,  /* output buffer too small */
     FLS_ERR_NO_DEVICE = -8, /* no usable gfx950 device */
-    FLS_ERR_FORMAT = -9     /* column image or metadata: bad magic, version, sizes, checksum or vmeta */
+    FLS_ERR_FORMAT = -9,    /* column image or metadata: bad magic, version, sizes, checksum or vmeta */
+    FLS_ERR_SCRATCH = -10   /* the work needs more than the library's static scratch array holds */
 };
 
 #define FLS_VECTOR_SIZE 1024u
@@ -285,6 +286,22 @@ int fls_query_q21(const fls_column* datekey, const fls_column* suppkey, const fl
                   const fls_column* revenue, uint64_t n_rows, const fls_q21_dims* dims, const void* d_workspace,
                   const fls_q21_params* params, int64_t* d_sums, uint64_t* d_counts, uint32_t flags, void* stream);
 
+/* Allocation-free fls_q21_build + fls_query_q21 in one call, with the same
+ * arguments minus the workspace and the same results.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The compact dimension tables are built into a static device array of
+ * FLS_Q21_STATIC_SCRATCH_BYTES that each GPU the library runs on holds until
+ * the process ends; dimensions whose fls_q21_workspace_bytes() exceeds it
+ * return FLS_ERR_SCRATCH before any work is queued.  The call allocates
+ * nothing and clears what must start at zero itself, but every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams, other
+ * host threads) race.                                                        */
+#define FLS_Q21_STATIC_SCRATCH_BYTES (4u << 20)
+int fls_query_q21_noalloc(const fls_column* datekey, const fls_column* suppkey, const fls_column* partkey,
+                          const fls_column* revenue, uint64_t n_rows, const fls_q21_dims* dims,
+                          const fls_q21_params* params, int64_t* d_sums, uint64_t* d_counts, uint32_t flags,
+                          void* stream);
 
 /* ============================================ star joins (Q3.1, Q4.1)
  * Dense-key dimension columns for the three- and four-way star joins: row i of
