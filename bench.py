@@ -43,6 +43,11 @@ def parse():
     ap.add_argument("--no-c5", action="store_true", help="skip the learned-selector workload")
     ap.add_argument("--only", default="", help="comma list of secondary workloads to run "
                     "(c1, c3, star, materialize, tile, c4, c5); default all")
+    ap.add_argument("--c4-no-per-segment", action="store_true",
+                    help="skip the batched section's one-launch-per-segment baseline (thousands of small "
+                         "launches; leave it out of counter-collection runs)")
+    ap.add_argument("--c4-segments", type=int, default=4096,
+                    help="segments of the batched workload (fewer for counter-collection runs)")
     ap.add_argument("--c5-train-vectors", type=int, default=1 << 22,
                     help="training-set size of the learned selector (smaller for profiler runs)")
     return ap.parse_args()
@@ -391,7 +396,7 @@ def bench_tile_vs_fastlanes(steps, warmup, seed):
     return res
 
 
-def bench_c4(steps, seed, cpu=True):
+def bench_c4(steps, seed, cpu=True, per_segment=True, n_seg=4096):
     """4096 uneven segments (bitpack/delta/RLE) decoded + range-filtered:
     one graph-replayed persistent launch vs one launch per segment."""
     import numpy as np
@@ -400,12 +405,16 @@ def bench_c4(steps, seed, cpu=True):
     import fastlanesgpu_amd as fls
 
     t = time.perf_counter()
-    b = fls.batch_c4(seed, 4096, 10, 24)
+    b = fls.batch_c4(seed, n_seg, 10, 24)
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t
     info = b.info()
-    d = torch.zeros(4096, dtype=torch.int64, device="cuda")
-    best, tune_ms = b.autotune(d)
+    d = torch.zeros(n_seg, dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    best, tune_best_ms = b.autotune(d, force=True)  # every candidate timed (no cache)
+    torch.cuda.synchronize()
+    tune_cost_ms = (time.perf_counter() - t) * 1e3
     ref = None
 
     def timed(fn, reps):
@@ -425,14 +434,14 @@ def bench_c4(steps, seed, cpu=True):
 
     graph_ms, _ = timed(lambda: b.graph_launch(best, d), max(5, steps))
     run_ms, same_run = timed(lambda: b.run(best, d), max(5, steps))
-    seg_ms, same_seg = timed(lambda: b.run_per_segment(best, d), 3)
+    seg_ms, same_seg = timed(lambda: b.run_per_segment(best, d), 3) if per_segment else (None, True)
     nbytes = info["encoded_bytes"]
     cpu_res = None
     if cpu:
         T = testlib()
         gpu_counts = ref.cpu().numpy().view(np.uint64)
-        all_ids = list(range(0, 4096, 8))
-        one_ids = list(range(0, 4096, 64))
+        all_ids = list(range(0, n_seg, 8))
+        one_ids = list(range(0, n_seg, 64))
         segs = {s: b.segment_host(s) for s in all_ids}
         work = lambda ids: sum(int(segs[s][1]) for s in ids)  # noqa: E731
         run = lambda ids, t: T.o_c4_encoded([segs[s] for s in ids], threads=t)  # noqa: E731
@@ -445,18 +454,22 @@ def bench_c4(steps, seed, cpu=True):
         del segs
     b.close()
     return {"cpu_baseline": cpu_res,
-            "workload": "4096 segments, lengths log-uniform by octave in [2^10,2^24), bw U{1..32}, "
+            "workload": f"{n_seg} segments, lengths log-uniform by octave in [2^10,2^24), bw U{{1..32}}, "
                         "40/30/30 bitpack/delta/RLE, middle-half range filter, count per segment",
             "n_values": info["n_values"], "encoded_bytes": nbytes,
             "segments_by_encoding": [info["n_bitpack"], info["n_delta"], info["n_rle"]],
             "build_s": build_s,
             "tune": {"tile_vectors": best.tile_vectors, "block_threads": best.block_threads,
-                     "schedule": ["static", "dynamic"][best.schedule], "tuning_ms": tune_ms},
+                     "schedule": ["static", "dynamic"][best.schedule],
+                     "class_tiles": {f"{['bitpack', 'delta', 'rle', 'mixed'][c // 2]}_{['narrow', 'wide'][c % 2]}":
+                                     1 << ((best.class_tiles >> (4 * c)) & 15)
+                                     for c in range(8) if (best.class_tiles >> (4 * c)) & 15},
+                     "tuning_cost_ms": tune_cost_ms, "best_candidate_ms": tune_best_ms},
             "graph_ms": graph_ms, "launch_ms": run_ms, "per_segment_ms": seg_ms,
             "values_per_s": info["n_values"] / (graph_ms * 1e-3),
             "encoded_GBs": nbytes / (graph_ms * 1e-3) / 1e9,
             "frac_hbm": nbytes / (graph_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "speedup_vs_per_segment": seg_ms / graph_ms,
+            "speedup_vs_per_segment": seg_ms / graph_ms if seg_ms else None,
             "counts_agree": bool(same_run and same_seg)}
 
 
@@ -830,7 +843,9 @@ def main():
             line["secondary"]["tile_vs_fastlanes"] = bench_tile_vs_fastlanes(max(5, args.steps // 2), 2, args.seed)
         if want("c4"):
             log("[bench] secondary: batched segments")
-            line["secondary"]["c4_batched"] = bench_c4(max(5, args.steps // 5), args.seed, cpu=not args.no_cpu)
+            line["secondary"]["c4_batched"] = bench_c4(max(5, args.steps // 5), args.seed, cpu=not args.no_cpu,
+                                                       per_segment=not args.c4_no_per_segment,
+                                                       n_seg=args.c4_segments)
         if want("c5") and not args.no_c5:
             log("[bench] secondary: learned encoding selector")
             line["secondary"]["c5_learned_selector"] = bench_c5(args.seed, n, n_train=args.c5_train_vectors,

@@ -721,8 +721,14 @@ class Batch:
             pass
 
 
-def tune(tile_vectors=64, block_threads=256, schedule=SCHED_STATIC):
-    return Tune(tile_vectors, block_threads, schedule, 0)
+def tune(tile_vectors=64, block_threads=256, schedule=SCHED_STATIC, class_tiles=None):
+    """Launch shape of a batch; class_tiles: optional {class: tile size} with
+    class = 2*encoding + (more than 8 encoded bits per value)."""
+    packed = 0
+    for c, tv in (class_tiles or {}).items():
+        assert tv & (tv - 1) == 0 and 2 <= tv <= 32768, tv
+        packed |= (tv.bit_length() - 1) << (4 * c)
+    return Tune(tile_vectors, block_threads, schedule, packed)
 
 
 def _device_to_host(ptr, n, dtype):
@@ -731,9 +737,11 @@ def _device_to_host(ptr, n, dtype):
     import numpy as np
 
     out = np.empty(n, dtype)
-    hip = ctypes.CDLL("libamdhip64.so")
-    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-    st = hip.hipMemcpy(out.ctypes.data, ptr, out.nbytes, 2)  # hipMemcpyDeviceToHost
+    # through libfls.so's own handle: dlsym finds the HIP runtime libfls.so is
+    # bound to (the one torch loaded), never a second copy
+    memcpy = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int)(
+        ("hipMemcpy", load()))
+    st = memcpy(out.ctypes.data, ptr, out.nbytes, 2)  # hipMemcpyDeviceToHost
     if st != 0:
         raise RuntimeError(f"hipMemcpy device-to-host failed: {st}")
     return out

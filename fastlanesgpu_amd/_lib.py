@@ -225,7 +225,7 @@ SIGNATURES.update({
 
 class Tune(ctypes.Structure):
     """struct fls_tune."""
-    _fields_ = [("tile_vectors", c_u32), ("block_threads", c_u32), ("schedule", c_u32), ("reserved", c_u32)]
+    _fields_ = [("tile_vectors", c_u32), ("block_threads", c_u32), ("schedule", c_u32), ("class_tiles", c_u32)]
 
 
 class BatchInfo(ctypes.Structure):

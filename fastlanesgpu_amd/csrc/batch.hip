@@ -1,8 +1,8 @@
 // Batched decode + range filter over many compressed segments of uneven
 // length, bit width and encoding, in one persistent launch.
 //
-// Segments are cut into tiles of `tile_vectors` FastLanes vectors; a tile
-// never spans two segments, so every block-wide branch on encoding and width
+// Segments are cut into tiles of `tile_vectors` FastLanes vectors (or a tile
+// size per encoding and width class); a tile never spans two segments, so every block-wide branch on encoding and width
 // is uniform.  A persistent grid (every block resident) walks the tile list
 // either round-robin (static) or through an atomic work queue (dynamic).  Per
 // tile the block counts the rows with lo <= value <= hi and adds the count to
@@ -34,7 +34,7 @@ namespace fls {
 namespace {
 
 struct SegDev {
-    uint32_t enc, bw, transposed, pad;  // transposed: rows in FastLanes transposed order
+    uint32_t enc, bw, transposed, base;  // transposed: rows in FastLanes transposed order; base: BITPACK FOR
     uint64_t n_values;
     const uint32_t* words;
     const fls_vmeta* vmeta;
@@ -48,15 +48,268 @@ struct TileDev {
     uint32_t seg, v0, nv, pad;
 };
 
+// ------------------------------------------------------------ range counts
+// Every path counts the rows of a vector with lo <= value <= hi as
+// (value - lo) <= span (unsigned), so one compare decides each row.
+//
+// Fast paths (full vectors of BITPACK / DELTA / RLE segments): each compare
+// adds its carry to a per-vector counter in the work-item; a half-wave whose
+// slot is past the tile's vectors discards its counter once per vector, not
+// once per row.  (A wave ballot per row with a scalar popcount measured
+// slower: the compiler turns the predicate into a 0/1 value and back, and the
+// CU's one scalar unit serialises the counts of all four SIMDs.)  At narrow
+// widths a half-wave takes K vectors per step and issues all their loads
+// before any extraction (the packed words of K vectors fit the registers of
+// one 32-bit vector), so a narrow segment keeps as many bytes in flight as a
+// wide one.
+
+// Vectors per half-wave per step at width BW: K * BW <= 32 words per lane.
+template <int BW>
+constexpr int batch_k() {
+    return BW <= 4 ? 8 : BW <= 8 ? 4 : BW <= 16 ? 2 : 1;
+}
+
+// BITPACK (+ FOR): vectors [0, nfull) of the segment's tile range starting at
+// v0; `lo_adj` = lo - base.  Returns this work-item's count.
+template <int BW, int VPB>
+__device__ __forceinline__ uint32_t count_fixed_full(const uint32_t* __restrict words, uint64_t v0, uint32_t nfull,
+                                                     uint32_t lo_adj, uint32_t span) {
+    constexpr int K = batch_k<BW>();
+    const uint32_t hw = threadIdx.x >> 5, l = dev::lane32();
+    if constexpr (BW == 0) {
+        return 0u;  // every value is the base: the caller counts the whole range at once
+    } else {
+        uint32_t cnt = 0;
+        for (uint32_t vb = 0; vb < nfull; vb += VPB * K) {
+            uint32_t w[K][BW];
+            bool live[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const uint32_t idx = vb + hw * K + k;
+                live[k] = idx < nfull;
+                const uint64_t v = v0 + (live[k] ? idx : nfull - 1u);  // a dead slot re-reads a live vector
+                load_lane_g<BW>(words + v * (32u * BW) + l, w[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                uint32_t ck = 0;
+#pragma unroll
+                for (int r = 0; r < 32; ++r) ck += (uint32_t)(dev::extract_at<BW>(w[k], r) - lo_adj <= span);
+                cnt += live[k] ? ck : 0u;
+            }
+        }
+        return cnt;
+    }
+}
+
+// DELTA with one width for every vector of the tile range: the chain walk of
+// rsum.cuh with the running sum kept as value - lo.  Returns this work-item's
+// count.
+template <int BW, int VPB>
+__device__ __forceinline__ uint32_t count_delta_full(const uint32_t* __restrict words,
+                                                     const fls_vmeta* __restrict vmeta,
+                                                     const uint32_t* __restrict bases, uint64_t v0, uint32_t nfull,
+                                                     uint32_t lo, uint32_t span) {
+    constexpr int K = batch_k<BW>();
+    const uint32_t hw = threadIdx.x >> 5, l = dev::lane32();
+    uint32_t cnt = 0;
+    for (uint32_t vb = 0; vb < nfull; vb += VPB * K) {
+        uint32_t w[K][BW > 0 ? BW : 1];
+        uint32_t acc[K];
+        bool live[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t idx = vb + hw * K + k;
+            live[k] = idx < nfull;
+            const uint64_t v = v0 + (live[k] ? idx : nfull - 1u);
+            acc[k] = ldg(bases + v * 32u + l) - lo;
+            if constexpr (BW > 0) load_lane_g<BW>(words + FLS_VMETA_OFFSET(ldg(vmeta + v)) + l, w[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            uint32_t ck = 0;
+            if constexpr (BW == 0) {
+                ck = 32u * (uint32_t)(acc[k] <= span);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 32; ++j) {
+                    acc[k] += dev::extract_at<BW>(w[k], dev::rsum_row(j));
+                    ck += (uint32_t)(acc[k] <= span);
+                }
+            }
+            cnt += live[k] ? ck : 0u;
+        }
+    }
+    return cnt;
+}
+
+// Flag words of a vector slot: bit k of word k/32 = run k's value in range.
+constexpr uint32_t kFlagWords = 33;  // 1024 runs + one word the window may read past the last
+constexpr int kRleK = 2;             // RLE vectors per half-wave per step
+
+// RLE: each run value is tested once, by the 32 work-items of the vector's
+// half-wave together (a ballot per 32 runs), into 32 LDS flag words; each
+// work-item then reads the 32 flags from its first run on (two LDS words and
+// a funnel shift) and walks its 32 rows in chain order: the run index advances
+// at each run start (bit r of the lane's index word), and the row counts when
+// the flag at that run offset is set.  No per-row LDS access or gather.  A
+// half-wave takes K vectors per step; every load that does not depend on a
+// run value (metadata, the lane's first run and index word) is issued before
+// the run values are tested, so a step waits for two rounds of loads.
+template <int VPB, int K>
+__device__ __forceinline__ uint32_t count_rle_full(const SegDev& sg, uint64_t v0, uint32_t nfull, uint32_t lo,
+                                                   uint32_t span, uint32_t* __restrict flags_lds) {
+    const uint32_t hw = threadIdx.x >> 5, l = dev::lane32(), half = (threadIdx.x >> 5) & 1u;
+    uint32_t c = 0;
+    for (uint32_t vb = 0; vb < nfull; vb += VPB * K) {
+        bool live[K];
+        const uint32_t* vals[K];
+        uint32_t nr[K], vlo[K], vbw[K], first[K], starts[K], ibw[K];
+        uint32_t steps_here = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t idx = vb + hw * K + k;
+            live[k] = idx < nfull;
+            const uint64_t v = v0 + (live[k] ? idx : nfull - 1u);
+            const fls_vmeta m = ldg(sg.vmeta + v);
+            const fls_rle_meta rm = ldg(sg.rmeta + v);
+            first[k] = ldg(sg.bases + v * 32u + l);
+            ibw[k] = FLS_VMETA_BW(m) ? 1u : 0u;
+            const uint32_t* p = sg.words + FLS_VMETA_OFFSET(m);
+            starts[k] = ibw[k] ? ldg_nt(p + l) : 0u;
+            vals[k] = p + 32u * ibw[k];
+            nr[k] = min((uint32_t)rm.n_runs, 1024u);
+            vlo[k] = lo - rm.vbase;  // run value - vbase compared against lo - vbase
+            vbw[k] = rm.vbw;
+            steps_here = max(steps_here, (nr[k] + 31u) >> 5);
+        }
+        // the wave's half-waves may need different numbers of ballots: run to the larger
+        const uint32_t steps = max(__builtin_amdgcn_readlane(steps_here, 0), __builtin_amdgcn_readlane(steps_here, 32));
+        for (uint32_t i = 0; i < steps; ++i) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const uint32_t r = 32u * i + l;
+                const bool in = r < nr[k] && linear_extract(vals[k], r < nr[k] ? r : 0u, vbw[k]) - vlo[k] <= span;
+                const uint64_t b = __builtin_amdgcn_ballot_w64(in);
+                if (l == 0u) flags_lds[(hw * K + k) * kFlagWords + i] = (uint32_t)(b >> (32u * half));
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (!live[k]) continue;
+            const uint32_t* flags = flags_lds + (hw * K + k) * kFlagWords;
+            const uint32_t F = __builtin_amdgcn_alignbit(flags[(first[k] >> 5) + 1u], flags[first[k] >> 5],
+                                                         first[k] & 31u);
+            if (ibw[k] == 0) {
+                c += 32u * (F & 1u);
+            } else {
+                uint32_t off = 0;
+#pragma unroll
+                for (int j = 0; j < 32; ++j) {
+                    off += (starts[k] >> dev::rsum_row(j)) & 1u;
+                    c += (F >> off) & 1u;
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();  // the flag words are rewritten next step
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    return c;
+}
+
+// Slow path, any encoding, for a segment's partial last vector and for MIXED
+// segments: one lane, rows walked with run-time width (two word loads per
+// row, cache hits after the first) in a rolled loop, counting only the rows
+// set in `rows` (register rows of this lane).  Few registers, so the fast
+// paths keep the kernel's occupancy.
+__device__ __forceinline__ uint32_t lane_extract(const uint32_t* __restrict lane_in, uint32_t bw, uint32_t r) {
+    if (bw == 0) return 0u;
+    const uint32_t bit = r * bw, k = bit >> 5, s = bit & 31u;
+    const uint32_t a = ldg(lane_in + 32u * k);
+    const uint32_t b = (s + bw > 32u) ? ldg(lane_in + 32u * (k + 1u)) : 0u;
+    const uint32_t mask = bw >= 32 ? 0xffffffffu : ((1u << bw) - 1u);
+    return __builtin_amdgcn_alignbit(b, a, s) & mask;
+}
+
+__device__ __forceinline__ uint32_t count_vector_slow(const SegDev& sg, uint64_t v, uint32_t rows, uint32_t lo,
+                                                   uint32_t span) {
+    const uint32_t l = dev::lane32();
+    const uint32_t* p = sg.words;
+    const uint32_t* vals = nullptr;
+    uint32_t w = sg.bw, add = sg.base, seed = 0, gbase = 0, gbw = 0, nr = 1;
+    bool chain = false;
+    if (sg.enc == FLS_ENC_BITPACK) {
+        p = sg.words + v * (32u * sg.bw);
+    } else if (sg.enc == FLS_ENC_MIXED) {
+        const fls_vmeta m = ldg(sg.vmeta + v);
+        const uint32_t enc = (uint32_t)(m >> 6) & 3u;
+        const uint64_t ax = ldg(sg.aux + v);
+        p = sg.words + FLS_VMETA_OFFSET(m);
+        w = (uint32_t)m & 63u;
+        add = enc == FLS_MIX_FOR ? (uint32_t)ax : 0u;
+        if (enc == FLS_MIX_DELTA || enc == FLS_MIX_RLE) {
+            chain = true;
+            seed = ldg(p + l);
+            p += 32u;
+            if (enc == FLS_MIX_RLE) {
+                const uint32_t ibw = (uint32_t)(ax >> 48) & 1u;
+                vals = p + 32u * ibw;
+                gbase = (uint32_t)ax;
+                gbw = w;
+                w = ibw;
+                nr = min((uint32_t)(ax >> 32) & 0xffffu, 1024u);
+            }
+        }
+    } else {  // DELTA, RLE
+        const fls_vmeta m = ldg(sg.vmeta + v);
+        p = sg.words + FLS_VMETA_OFFSET(m);
+        w = FLS_VMETA_BW(m);
+        add = 0u;
+        chain = true;
+        seed = ldg(sg.bases + v * 32u + l);
+        if (sg.enc == FLS_ENC_RLE) {
+            const fls_rle_meta rm = ldg(sg.rmeta + v);
+            vals = p + 32u * w;
+            gbase = rm.vbase;
+            gbw = rm.vbw;
+            nr = min((uint32_t)rm.n_runs, 1024u);
+        }
+    }
+    const uint32_t last = nr ? nr - 1u : 0u;
+    uint32_t acc = seed, c = 0;
+#pragma unroll 1
+    for (uint32_t j = 0; j < 32; ++j) {
+        const uint32_t r = chain ? (uint32_t)dev::rsum_row((int)j) : j;
+        uint32_t x = lane_extract(p + l, w, r);
+        if (chain) {
+            acc += x;
+            x = acc;
+        }
+        x = vals ? gbase + linear_extract(vals, min(x, last), gbw) : x + add;
+        c += ((rows >> r) & 1u) & (uint32_t)(x - lo <= span);
+    }
+    return c;
+}
+
 // ---------------------------------------------------------------- kernel
+// Eight waves per SIMD (64 VGPRs): the fast paths are latency-bound, and
+// measured 2.32 ms at five waves against 1.99 ms at eight over the
+// 4096-segment batch (a few loop-invariant values spill; no spill in a row loop).
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void batch_kernel(const SegDev* __restrict segs, const TileDev* __restrict tiles,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void batch_kernel(const SegDev* __restrict segs, const TileDev* __restrict tiles,
                                                       uint32_t n_tiles, uint64_t* __restrict counts,
                                                       uint32_t* __restrict queue, uint32_t dynamic) {
     constexpr int VPB = BLOCK / 32;
-    __shared__ uint32_t stage[VPB][1024];
-    __shared__ uint64_t s_part[BLOCK / 64];
+    constexpr int WAVES = BLOCK / 64;
+    __shared__ uint32_t flags_lds[VPB * kRleK * kFlagWords];
+    __shared__ uint32_t s_part[WAVES];
+    __shared__ uint32_t s_mm[2 * WAVES];
     __shared__ uint32_t s_tile;
+    const uint32_t hw = threadIdx.x >> 5, wave = threadIdx.x >> 6;
     uint32_t iter = 0;
     for (;;) {
         uint32_t t;
@@ -71,42 +324,88 @@ __global__ __launch_bounds__(BLOCK) void batch_kernel(const SegDev* __restrict s
         if (t >= n_tiles) break;  // uniform over the block
         const TileDev tile = tiles[t];
         const SegDev sg = segs[tile.seg];
-        const ColDev col{sg.enc, sg.bw, 0u, sg.transposed, sg.words, sg.vmeta, sg.bases, sg.aux};
-        const fls_rle_meta* rmeta = sg.rmeta;
-        const uint32_t span = sg.hi - sg.lo;
-        uint64_t cnt = 0;
-        for (uint32_t vb = 0; vb < tile.nv; vb += VPB) {
-            const uint32_t local = vb + (threadIdx.x >> 5);
-            const bool live = local < tile.nv;
-            const uint64_t v = (uint64_t)tile.v0 + local;
-            const uint32_t lo = sg.lo;
-            const uint64_t first = v * 1024u;
-            uint32_t c = 0;
-            if (first + 1024u <= sg.n_values) {
-                // a full vector: count in-range rows directly (compare + add with carry)
-                auto count = [&](int, uint32_t x) { c += (uint32_t)((x - lo) <= span); };
-                column_walk(col, rmeta, v, stage[threadIdx.x >> 5], live, count);
-            } else {
-                // the segment's partial last vector: row mask, then the rows that exist
-                uint32_t m = 0;
-                auto mark = [&](int r, uint32_t x) { m |= (uint32_t)((x - lo) <= span) << r; };
-                column_walk(col, rmeta, v, stage[threadIdx.x >> 5], live, mark);
-                if (live)
-                    c = (uint32_t)__builtin_popcount(
-                        m & live_row_mask((uint32_t)(sg.n_values - first), col.transposed != 0));
+        const uint32_t lo = sg.lo, span = sg.hi - sg.lo;
+        const uint64_t seg_vecs = ceil_div(sg.n_values, 1024);
+        const bool has_partial = (sg.n_values & 1023u) != 0 && tile.v0 + tile.nv == seg_vecs;
+        const uint32_t nfull = tile.nv - (has_partial ? 1u : 0u);
+        uint32_t lane_cnt = 0;  // per work-item
+        if (sg.enc == FLS_ENC_BITPACK) {
+            const uint32_t lo_adj = lo - sg.base;
+            switch (sg.bw) {
+#define FLS_BK_CASE(B) \
+    case B: lane_cnt = count_fixed_full<B, VPB>(sg.words, tile.v0, nfull, lo_adj, span); break;
+                FLS_WIDTH_CASES(FLS_BK_CASE)
+#undef FLS_BK_CASE
+                default: break;
             }
-            cnt += c;
+            if (sg.bw == 0 && (0u - lo_adj) <= span && threadIdx.x == 0) lane_cnt = 1024u * nfull;
+        } else if (sg.enc == FLS_ENC_DELTA) {
+            // one width for the whole tile range takes the batched walk
+            uint32_t mn = 0xffffffffu, mx = 0u;
+            for (uint32_t i = threadIdx.x; i < nfull; i += BLOCK) {
+                const uint32_t b = FLS_VMETA_BW(ldg(sg.vmeta + tile.v0 + i));
+                mn = min(mn, b);
+                mx = max(mx, b);
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                mn = min(mn, (uint32_t)__shfl_xor(mn, off, 64));
+                mx = max(mx, (uint32_t)__shfl_xor(mx, off, 64));
+            }
+            if (dev::is_wave_leader()) {
+                s_mm[2 * wave] = mn;
+                s_mm[2 * wave + 1] = mx;
+            }
+            __syncthreads();
+            mn = s_mm[0];
+            mx = s_mm[1];
+#pragma unroll
+            for (int w = 1; w < WAVES; ++w) {
+                mn = min(mn, s_mm[2 * w]);
+                mx = max(mx, s_mm[2 * w + 1]);
+            }
+            if (nfull && mn == mx) {
+                switch (mn) {
+#define FLS_DK_CASE(B)                                                                                       \
+    case B: lane_cnt = count_delta_full<B, VPB>(sg.words, sg.vmeta, sg.bases, tile.v0, nfull, lo, span); \
+        break;
+                    FLS_WIDTH_CASES(FLS_DK_CASE)
+#undef FLS_DK_CASE
+                    default: break;
+                }
+            } else {
+                // widths differ inside the tile: one vector per half-wave at its own width
+                for (uint32_t vb = hw; vb < nfull; vb += VPB) {
+                    const uint64_t v = tile.v0 + vb;
+                    const fls_vmeta m = ldg(sg.vmeta + v);
+                    uint32_t c = 0;
+                    auto f = [&](int, uint32_t x) { c += (uint32_t)(x - lo <= span); };
+                    walk_delta(sg.words + FLS_VMETA_OFFSET(m) + dev::lane32(), FLS_VMETA_BW(m),
+                               ldg(sg.bases + v * 32u + dev::lane32()), f);
+                    lane_cnt += c;
+                }
+            }
+        } else if (sg.enc == FLS_ENC_RLE) {
+            lane_cnt = count_rle_full<VPB, kRleK>(sg, tile.v0, nfull, lo, span, flags_lds);
+        } else {  // MIXED: every vector through the slow path
+            for (uint32_t vb = hw; vb < nfull; vb += VPB)
+                lane_cnt += count_vector_slow(sg, tile.v0 + vb, 0xffffffffu, lo, span);
         }
-        cnt = dev::wave_sum(cnt);
-        if (dev::is_wave_leader()) s_part[threadIdx.x >> 6] = cnt;
+        if (has_partial && hw == 0) {
+            const uint64_t v = seg_vecs - 1u;
+            lane_cnt += count_vector_slow(sg, v, live_row_mask((uint32_t)(sg.n_values - v * 1024u), sg.transposed != 0),
+                                          lo, span);
+        }
+        const uint32_t total_wave = dev::wave_sum(lane_cnt);
+        if (dev::is_wave_leader()) s_part[wave] = total_wave;
         __syncthreads();
         if (threadIdx.x == 0) {
             uint64_t total = 0;
 #pragma unroll
-            for (int w = 0; w < BLOCK / 64; ++w) total += s_part[w];
+            for (int w = 0; w < WAVES; ++w) total += s_part[w];
             if (total) atomicAdd((unsigned long long*)&counts[tile.seg], (unsigned long long)total);
         }
-        __syncthreads();  // s_part / s_tile are rewritten next iteration
+        __syncthreads();  // s_part / s_mm / s_tile / flags are rewritten next iteration
     }
 }
 
@@ -219,8 +518,9 @@ struct fls_batch {
     fls_vmeta* d_vmeta = nullptr;
     uint32_t* d_bases = nullptr;
     fls_rle_meta* d_rmeta = nullptr;
-    // tiles for the current tile size
-    uint32_t tile_vectors = 0;
+    // tiles for the current tiling (tile_vectors, class_tiles, class filter)
+    uint32_t tile_vectors = 0, class_tiles = 0;
+    int tile_filter = -2;
     fls::TileDev* d_tiles = nullptr;
     uint32_t n_tiles = 0;
     std::vector<uint32_t> seg_tile_off, seg_tile_cnt;
@@ -238,17 +538,37 @@ struct fls_batch {
 namespace fls {
 namespace {
 
-int set_tiles(fls_batch* b, uint32_t tile_vectors) {
-    if (tile_vectors == 0) return FLS_ERR_ARG;
-    if (b->d_tiles && b->tile_vectors == tile_vectors) return FLS_OK;
+// Tiling class of segment s: 2 * encoding + (more than 8 encoded bits per
+// value), so narrow and wide segments of each encoding can get their own tile
+// size (fls_tune.class_tiles).
+uint32_t seg_class(const fls_batch* b, uint32_t s) {
+    const fls_segment_info& i = b->info[s];
+    const uint64_t bits = i.n_values ? 32ull * i.n_words / i.n_values : 0u;
+    return 2u * (i.encoding < 3u ? i.encoding : 3u) + (bits > 8u ? 1u : 0u);
+}
+
+uint32_t class_tile(const fls_tune& t, uint32_t cls) {
+    const uint32_t nib = (t.class_tiles >> (4u * cls)) & 15u;
+    return nib ? (1u << nib) : t.tile_vectors;
+}
+
+// Cut the segments into tiles (only those of class `only`, when >= 0) and
+// upload the tile list, unless the current one already is that tiling.
+int set_tiles(fls_batch* b, const fls_tune& t, int only = -1) {
+    if (t.tile_vectors == 0) return FLS_ERR_ARG;
+    if (b->d_tiles && b->tile_vectors == t.tile_vectors && b->class_tiles == t.class_tiles && b->tile_filter == only)
+        return FLS_OK;
     std::vector<TileDev> tiles;
     b->seg_tile_off.assign(b->segs.size(), 0);
     b->seg_tile_cnt.assign(b->segs.size(), 0);
     for (uint32_t s = 0; s < b->segs.size(); ++s) {
-        const uint64_t nv = ceil_div(b->segs[s].n_values, 1024);
+        const uint32_t cls = seg_class(b, s);
         b->seg_tile_off[s] = (uint32_t)tiles.size();
-        for (uint64_t v0 = 0; v0 < nv; v0 += tile_vectors)
-            tiles.push_back(TileDev{s, (uint32_t)v0, (uint32_t)(nv - v0 < tile_vectors ? nv - v0 : tile_vectors), 0});
+        if (only >= 0 && (int)cls != only) continue;
+        const uint32_t tv = class_tile(t, cls);
+        const uint64_t nv = ceil_div(b->segs[s].n_values, 1024);
+        for (uint64_t v0 = 0; v0 < nv; v0 += tv)
+            tiles.push_back(TileDev{s, (uint32_t)v0, (uint32_t)(nv - v0 < tv ? nv - v0 : tv), 0});
         b->seg_tile_cnt[s] = (uint32_t)tiles.size() - b->seg_tile_off[s];
     }
     if (b->d_tiles) FLS_HIP_CHECK(hipFree(b->d_tiles));
@@ -257,7 +577,9 @@ int set_tiles(fls_batch* b, uint32_t tile_vectors) {
     if (!tiles.empty())
         FLS_HIP_CHECK(hipMemcpy(b->d_tiles, tiles.data(), sizeof(TileDev) * tiles.size(), hipMemcpyHostToDevice));
     b->n_tiles = (uint32_t)tiles.size();
-    b->tile_vectors = tile_vectors;
+    b->tile_vectors = t.tile_vectors;
+    b->class_tiles = t.class_tiles;
+    b->tile_filter = only;
     return FLS_OK;
 }
 
@@ -291,14 +613,15 @@ int check_tune(const fls_tune* t) {
     if (t->tile_vectors == 0 || t->tile_vectors > (1u << 20)) return FLS_ERR_ARG;
     if (t->block_threads != 256 && t->block_threads != 512) return FLS_ERR_ARG;
     if (t->schedule > FLS_SCHED_DYNAMIC) return FLS_ERR_ARG;
-    return FLS_OK;
+    return FLS_OK;  // every class_tiles nibble is valid (tiles of 2 .. 32768 vectors)
 }
 
-// zero counters + queue, then the persistent launch
-int run_batch(fls_batch* b, const fls_tune* t, uint64_t* d_counts, hipStream_t s) {
+// zero counters + queue, then the persistent launch over the tiles of class
+// `only` (all classes when < 0)
+int run_batch(fls_batch* b, const fls_tune* t, uint64_t* d_counts, hipStream_t s, int only = -1) {
     int st;
     if ((st = check_tune(t)) != FLS_OK) return st;
-    if ((st = set_tiles(b, t->tile_vectors)) != FLS_OK) return st;
+    if ((st = set_tiles(b, *t, only)) != FLS_OK) return st;
     if ((st = zero_u64(d_counts, b->segs.size(), s)) != FLS_OK) return st;
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, b->d_queue, 1u);
     const bool dynamic = t->schedule == FLS_SCHED_DYNAMIC;
@@ -353,9 +676,10 @@ bool cache_lookup(const std::string& path, const std::string& key, fls_tune* t) 
     unsigned tile, block, sched;
     double ms;
     bool found = false;
-    while (fscanf(f, "%199s %u %u %u %lf", k, &tile, &block, &sched, &ms) == 5) {
+    unsigned classes;
+    while (fscanf(f, "%199s %u %u %u %x %lf", k, &tile, &block, &sched, &classes, &ms) == 6) {
         if (key == k) {
-            *t = fls_tune{tile, block, sched, 0};
+            *t = fls_tune{tile, block, sched, classes};
             found = true;  // keep the last entry for the key
         }
     }
@@ -367,7 +691,7 @@ void cache_store(const std::string& path, const std::string& key, const fls_tune
     mkdirs_for(path);
     FILE* f = fopen(path.c_str(), "a");
     if (!f) return;
-    fprintf(f, "%s %u %u %u %.6f\n", key.c_str(), t.tile_vectors, t.block_threads, t.schedule, ms);
+    fprintf(f, "%s %u %u %u %x %.6f\n", key.c_str(), t.tile_vectors, t.block_threads, t.schedule, t.class_tiles, ms);
     fclose(f);
 }
 
@@ -396,7 +720,8 @@ int fls_batch_create(const fls_segment* segs, uint32_t n_segments, fls_batch** o
         }
         const uint32_t tr = c.encoding == FLS_ENC_BITPACK ? (c.transposed ? 1u : 0u) : 1u;
         b->segs.push_back(
-            SegDev{c.encoding, c.bw, tr, 0u, g.n_values, c.words, c.vmeta, c.bases, rmeta, c.aux, g.lo, g.hi});
+            SegDev{c.encoding, c.bw, tr, c.encoding == FLS_ENC_BITPACK ? c.base : 0u, g.n_values, c.words, c.vmeta,
+                   c.bases, rmeta, c.aux, g.lo, g.hi});
         b->info.push_back(fls_segment_info{c.encoding, c.bw, g.n_values, g.lo, g.hi, c.n_words});
         if (c.encoding <= FLS_ENC_RLE) b->count_by_enc[c.encoding]++;
         b->encoded_bytes += 4ull * c.n_words;
@@ -565,7 +890,7 @@ int fls_batch_segment_column(const fls_batch* b, uint32_t s, fls_segment* seg) {
     fls_segment g{};
     g.col.encoding = d.enc;
     g.col.bw = d.bw;
-    g.col.base = 0;
+    g.col.base = d.base;
     g.col.transposed = d.transposed;
     g.col.n_words = b->info[s].n_words;
     g.col.words = d.words;
@@ -590,7 +915,7 @@ int fls_batch_run_per_segment(fls_batch* b, const fls_tune* tune, uint64_t* d_co
     if (!b || !d_counts) return FLS_ERR_NULL;
     int st;
     if ((st = check_tune(tune)) != FLS_OK) return st;
-    if ((st = set_tiles(b, tune->tile_vectors)) != FLS_OK) return st;
+    if ((st = set_tiles(b, *tune)) != FLS_OK) return st;
     hipStream_t s = as_stream(stream);
     if ((st = zero_u64(d_counts, b->segs.size(), s)) != FLS_OK) return st;
     for (uint32_t i = 0; i < b->segs.size(); ++i) {
@@ -609,13 +934,14 @@ int fls_batch_graph_launch(fls_batch* b, const fls_tune* tune, uint64_t* d_count
     int st;
     if ((st = check_tune(tune)) != FLS_OK) return st;
     const bool same = b->exec && b->graph_counts == d_counts && b->graph_tune.tile_vectors == tune->tile_vectors &&
-                      b->graph_tune.block_threads == tune->block_threads && b->graph_tune.schedule == tune->schedule;
+                      b->graph_tune.block_threads == tune->block_threads && b->graph_tune.schedule == tune->schedule &&
+                      b->graph_tune.class_tiles == tune->class_tiles;
     if (!same) {
         if (b->exec) FLS_HIP_CHECK(hipGraphExecDestroy(b->exec));
         if (b->graph) FLS_HIP_CHECK(hipGraphDestroy(b->graph));
         b->exec = nullptr;
         b->graph = nullptr;
-        if ((st = set_tiles(b, tune->tile_vectors)) != FLS_OK) return st;  // uploads before capture
+        if ((st = set_tiles(b, *tune)) != FLS_OK) return st;  // uploads before capture
         if (!b->cap_stream) FLS_HIP_CHECK(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
         FLS_HIP_CHECK(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
         st = run_batch(b, tune, d_counts, b->cap_stream);
@@ -649,34 +975,65 @@ int fls_batch_autotune(fls_batch* b, const char* cache_path, int force, uint64_t
     hipEvent_t e0, e1;
     FLS_HIP_CHECK(hipEventCreate(&e0));
     FLS_HIP_CHECK(hipEventCreate(&e1));
+    // best of three timed runs after a warm-up (which also uploads the tiles)
+    auto timed = [&](const fls_tune& t, int only, double* out) -> int {
+        int rc = run_batch(b, &t, d_counts, s, only);
+        double tmin = 1e30;
+        for (int rep = 0; rep < 3 && rc == FLS_OK; ++rep) {
+            (void)hipEventRecord(e0, s);
+            rc = run_batch(b, &t, d_counts, s, only);
+            (void)hipEventRecord(e1, s);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            tmin = ms < tmin ? ms : tmin;
+        }
+        *out = tmin;
+        return rc;
+    };
     const uint32_t tiles[] = {8, 16, 32, 64, 128, 256, 512};
     const uint32_t blocks[] = {256, 512};
     double best_t = 1e30;
     int st = FLS_OK;
-    for (uint32_t tv : tiles) {
-        for (uint32_t bt : blocks) {
-            for (uint32_t sched = 0; sched <= FLS_SCHED_DYNAMIC; ++sched) {
-                fls_tune t{tv, bt, sched, 0};
-                if ((st = run_batch(b, &t, d_counts, s)) != FLS_OK) break;  // warm-up (uploads tiles)
-                double tmin = 1e30;
-                for (int rep = 0; rep < 3 && st == FLS_OK; ++rep) {
-                    (void)hipEventRecord(e0, s);
-                    st = run_batch(b, &t, d_counts, s);
-                    (void)hipEventRecord(e1, s);
-                    (void)hipEventSynchronize(e1);
-                    float ms = 0;
-                    (void)hipEventElapsedTime(&ms, e0, e1);
-                    tmin = ms < tmin ? ms : tmin;
-                }
-                if (st) break;
-                if (tmin < best_t) {
-                    best_t = tmin;
+    // 1) one tile size, block size and schedule for every segment
+    for (uint32_t tv : tiles)
+        for (uint32_t bt : blocks)
+            for (uint32_t sched = 0; sched <= FLS_SCHED_DYNAMIC && st == FLS_OK; ++sched) {
+                const fls_tune t{tv, bt, sched, 0};
+                double ms = 0;
+                if ((st = timed(t, -1, &ms)) == FLS_OK && ms < best_t) {
+                    best_t = ms;
                     *best = t;
                 }
             }
-            if (st) break;
+    // 2) at that block size and schedule, a tile size per (encoding, width)
+    //    class, each timed on the class's segments alone; kept if the whole
+    //    batch then runs faster than with one tile size
+    bool present[8] = {false, false, false, false, false, false, false, false};
+    for (uint32_t i = 0; i < b->segs.size(); ++i) present[seg_class(b, i)] = true;
+    uint32_t classes = 0;
+    for (uint32_t c = 0; c < 8 && st == FLS_OK; ++c) {
+        if (!present[c]) continue;
+        double cbest = 1e30;
+        uint32_t cl = 0;
+        for (uint32_t tv : tiles) {
+            const fls_tune t{tv, best->block_threads, best->schedule, 0};
+            double ms = 0;
+            if ((st = timed(t, (int)c, &ms)) != FLS_OK) break;
+            if (ms < cbest) {
+                cbest = ms;
+                cl = (uint32_t)__builtin_ctz(tv);
+            }
         }
-        if (st) break;
+        classes |= cl << (4u * c);
+    }
+    if (st == FLS_OK && classes) {
+        const fls_tune t{best->tile_vectors, best->block_threads, best->schedule, classes};
+        double ms = 0;
+        if ((st = timed(t, -1, &ms)) == FLS_OK && ms < best_t) {
+            best_t = ms;
+            *best = t;
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

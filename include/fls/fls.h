@@ -559,9 +559,12 @@ typedef struct fls_batch_info {
 
 enum { FLS_SCHED_STATIC = 0, FLS_SCHED_DYNAMIC = 1 };
 /* launch shape: vectors per tile (a tile never spans segments), work-items
- * per block (256 or 512), tiles handed out round-robin or from a work queue */
+ * per block (256 or 512), tiles handed out round-robin or from a work queue.
+ * class_tiles: 0, or a tile size per segment class c = 2*encoding + (more
+ * than 8 encoded bits per value): nibble c (bits 4c..4c+3) = log2 of that
+ * class's tile size, 0 meaning tile_vectors.                                */
 typedef struct fls_tune {
-    uint32_t tile_vectors, block_threads, schedule, reserved;
+    uint32_t tile_vectors, block_threads, schedule, class_tiles;
 } fls_tune;
 
 /* Segments the caller already holds on the device (pointers are borrowed). */
@@ -592,7 +595,10 @@ int fls_batch_run_per_segment(fls_batch* b, const fls_tune* tune, uint64_t* d_co
 /* fls_batch_run captured once in a hipGraph (re-captured when tune or
  * d_counts change) and replayed on `stream`                                  */
 int fls_batch_graph_launch(fls_batch* b, const fls_tune* tune, uint64_t* d_counts, void* stream);
-/* time every (tile, block, schedule) candidate on this batch, keep the best;
+/* time every (tile, block, schedule) candidate on this batch, then, at the
+ * best block and schedule, every tile size per segment class on that class's
+ * segments alone; keep the per-class tiling when the whole batch runs faster
+ * with it;
  * results are cached in cache_path (NULL: $FLS_AUTOTUNE_CACHE, else
  * ~/.cache/fls_mi355x/autotune.txt) under a key of device + batch shape and
  * reused unless force != 0.  *best_ms = -1 when the cache answered.          */

@@ -74,8 +74,8 @@ def test_autotune_and_cache(tmp_path):
     best, ms = b.autotune(d, cache_path=cache)
     assert ms > 0 and best.tile_vectors > 0 and os.path.exists(cache)
     best2, ms2 = b.autotune(d, cache_path=cache)
-    assert ms2 == -1.0 and (best2.tile_vectors, best2.block_threads, best2.schedule) == \
-        (best.tile_vectors, best.block_threads, best.schedule)
+    assert ms2 == -1.0 and (best2.tile_vectors, best2.block_threads, best2.schedule, best2.class_tiles) == \
+        (best.tile_vectors, best.block_threads, best.schedule, best.class_tiles)
     b.run(best, d)
     np.testing.assert_array_equal(counts_of(d), want)
 
@@ -85,7 +85,9 @@ def test_caller_described_segments():
     rng = np.random.default_rng(3)
     segs, want = [], []
     specs = [("bitpack", 5000, 9), ("delta", 70_000, 0), ("rle", 33_333, 14), ("bitpack", 1, 3), ("rle", 1024, 1),
-             ("mixed", 50_000, 13), ("rle_aux", 9000, 20), ("mixed", 700, 31), ("bitpack_t", 3 * 1024 + 777, 13)]
+             ("mixed", 50_000, 13), ("rle_aux", 9000, 20), ("mixed", 700, 31), ("bitpack_t", 3 * 1024 + 777, 13),
+             ("bitpack_for", 20_000 + 3, 10), ("rle_long", 40_000 + 9, 12), ("delta_steps", 64 * 1024, 0),
+             ("bitpack", 8 * 1024, 0)]
     from fastlanesgpu_amd import selector as sel
 
     keep = []
@@ -117,6 +119,26 @@ def test_caller_described_segments():
             x = rng.integers(0, 2**bw, n).astype(np.uint32)
             col = fls.Column(fls.ENC_BITPACK, n, fls.pack_column_device(T.to_dev(x), n, bw), bw=bw)
             rmeta = None
+        elif kind == "bitpack_for":  # frame of reference: values = packed + base
+            base = 5_000_000
+            x = (rng.integers(0, 2**bw, n) + base).astype(np.uint32)
+            col = fls.Column(fls.ENC_BITPACK, n, fls.pack_column_device(T.to_dev(x), n, bw, base=base), bw=bw,
+                             base=base)
+            rmeta = None
+        elif kind == "rle_long":  # runs of mean 300: many lanes lie inside one run (index width 0)
+            starts = rng.random(n) < 1 / 300
+            starts[0] = True
+            x = rng.integers(0, 2**bw, n).astype(np.uint32)[np.cumsum(starts) - 1]
+            w, vm, rm, bs = fls.rle_encode_device(T.to_dev(x), n)
+            col = fls.Column(fls.ENC_RLE, n, w, vmeta=vm, bases=bs, aux=rm)
+            rmeta = None
+        elif kind == "delta_steps":  # sorted, with a step size that changes every 1000 rows: widths vary per vector
+            steps = (rng.integers(0, 2, n) << rng.integers(0, 20, n // 1000 + 1).repeat(1000)[:n]).astype(np.uint64)
+            x = np.cumsum(steps).astype(np.uint32)
+            w, vm, bs = fls.delta_encode_device(T.to_dev(x), n)
+            col = fls.Column(fls.ENC_DELTA, n, w, vmeta=vm, bases=bs)
+            assert len(set((T.from_dev(vm, np.uint64) & 0xFF).tolist())) > 3
+            rmeta = None
         elif kind == "delta":
             x = np.sort(rng.integers(0, 2**30, n)).astype(np.uint32)
             w, vm, bs = fls.delta_encode_device(T.to_dev(x), n)
@@ -134,8 +156,14 @@ def test_caller_described_segments():
         want.append(int(((x >= lo) & (x <= hi)).sum()))
     b = fls.batch_from_segments(segs)
     d = torch.zeros(len(segs), dtype=torch.int64, device="cuda")
-    b.run(fls.tune(4, 256, fls.SCHED_DYNAMIC), d)
-    np.testing.assert_array_equal(counts_of(d), np.array(want, np.uint64))
+    for tile, block, classes in ((4, 256, None), (1, 512, None), (64, 256, None), (512, 512, None),
+                                 (8, 256, {0: 2, 1: 64, 2: 16, 4: 4, 5: 256, 6: 32})):
+        d.fill_(-1)
+        b.run(fls.tune(tile, block, fls.SCHED_DYNAMIC, class_tiles=classes), d)
+        np.testing.assert_array_equal(counts_of(d), np.array(want, np.uint64), err_msg=f"tile {tile} block {block}")
+        d.fill_(-1)
+        b.run_per_segment(fls.tune(tile, block, fls.SCHED_STATIC, class_tiles=classes), d)
+        np.testing.assert_array_equal(counts_of(d), np.array(want, np.uint64), err_msg=f"per segment, tile {tile}")
 
 
 def test_full_batch_4096_segments():
