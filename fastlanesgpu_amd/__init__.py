@@ -431,8 +431,18 @@ def build_pht(d_filter, filter_eq, d_keys, d_vals, n, d_ht, ht_len, keys_min, st
          keys_min, _stream(stream))
 
 
-def q21_build_pht(dims, params=None, stream=None):
-    """Reference-layout hash tables (build_hashtable_s/p/d) -> (Q21Pht, tensors)."""
+def pht_occupancy(d_ht, ht_len, pair, stream=None):
+    """Occupancy bitmap of a reference hash table (fls_pht_occupancy_u32)."""
+    import torch
+
+    bits = torch.zeros(max(1, (ht_len + 31) // 32), dtype=torch.int32, device="cuda")
+    call("fls_pht_occupancy_u32", dptr(d_ht), ht_len, int(pair), dptr(bits), _stream(stream))
+    return bits
+
+
+def q21_build_pht(dims, params=None, stream=None, occupancy=True):
+    """Reference-layout hash tables (build_hashtable_s/p/d) -> (Q21Pht, tensors);
+    with occupancy, also the supplier and part tables' occupancy bitmaps."""
     import torch
 
     params = params or q21_params()
@@ -443,9 +453,16 @@ def q21_build_pht(dims, params=None, stream=None):
     build_pht(dims["p_category"], params.category, keys["n_part"], dims["p_brand"], dims["n_part"], ht_p,
               dims["n_part"], 1, stream)
     build_pht(None, 0, keys["n_date"], dims["d_year"], dims["n_date"], ht_d, dims["n_date"], 1, stream)
+    keep = [ht_s, ht_p, ht_d]
+    occ_s = occ_p = None
+    if occupancy:
+        occ_s = pht_occupancy(ht_s, dims["n_supp"], False, stream)
+        occ_p = pht_occupancy(ht_p, dims["n_part"], True, stream)
+        keep += [occ_s, occ_p]
     pht = Q21Pht(ht_s.data_ptr(), dims["n_supp"], 1, ht_p.data_ptr(), dims["n_part"], 1, ht_d.data_ptr(),
-                 dims["n_date"], 1)
-    return pht, (ht_s, ht_p, ht_d)
+                 dims["n_date"], 1, dptr(occ_s) if occ_s is not None else None,
+                 dptr(occ_p) if occ_p is not None else None)
+    return pht, tuple(keep)
 
 
 def query_q21_pht(date, supp, part, rev, n_rows, pht, params=None, accumulate=False, d_out=None, stream=None):

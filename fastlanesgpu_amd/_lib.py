@@ -124,7 +124,8 @@ class Q21Params(ctypes.Structure):
 class Q21Pht(ctypes.Structure):
     """struct fls_q21_pht."""
     _fields_ = [("ht_s", c_vp), ("s_len", c_u32), ("s_min", c_u32), ("ht_p", c_vp), ("p_len", c_u32),
-                ("p_min", c_u32), ("ht_d", c_vp), ("d_len", c_u32), ("d_min", c_u32)]
+                ("p_min", c_u32), ("ht_d", c_vp), ("d_len", c_u32), ("d_min", c_u32), ("occ_s", c_vp),
+                ("occ_p", c_vp)]
 
 
 class Q21Dims(ctypes.Structure):
@@ -138,6 +139,7 @@ SIGNATURES.update({
     "fls_build_pht_u32": (c_int, [c_vp, c_u32, c_vp, c_vp, c_u32, c_vp, c_u32, c_u32, c_vp]),
     "fls_query_q21_pht": (c_int, [COLP, COLP, COLP, COLP, c_u64, ctypes.POINTER(Q21Pht), _Q21P, c_vp, c_vp, c_u32,
                                   c_vp]),
+    "fls_pht_occupancy_u32": (c_int, [c_vp, c_u32, c_u32, c_vp, c_vp]),
     "fls_q21_workspace_bytes": (c_u64, [c_u32, c_u32, c_u32]),
     "fls_q21_build": (c_int, [ctypes.POINTER(Q21Dims), _Q21P, c_vp, c_u64, c_vp]),
     "fls_query_q21": (c_int, [COLP, COLP, COLP, COLP, c_u64, ctypes.POINTER(Q21Dims), c_vp, _Q21P, c_vp, c_vp,

@@ -246,6 +246,28 @@ __device__ __forceinline__ void bitmap_filter(const Col& col, const uint32_t* __
     }
 }
 
+// bitmap_filter with the bit index of a key computed by `idx` (e.g. a
+// reference hash table's slot, (key - min) % len), which must be below n
+// or read as a miss.
+template <int B, typename Col, typename Idx>
+__device__ __forceinline__ void bitmap_filter_by(const Col& col, const uint32_t* __restrict bits, Idx idx,
+                                                 uint32_t n, uint32_t& m) {
+#pragma unroll
+    for (int rb = 0; rb < 32; rb += B) {
+        uint32_t k[B], wv[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            k[j] = idx(col.get(rb + j));
+            wv[j] = bits[(k[j] < n ? k[j] : 0u) >> 5];
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const uint32_t hit = k[j] < n ? (wv[j] >> (k[j] & 31u)) & 1u : 0u;
+            m &= ~((hit ^ 1u) << (rb + j));
+        }
+    }
+}
+
 // Row mask (bit r) of the rows of this lane that exist in a vector holding
 // `live` <= 1024 rows, in normal or transposed row order.  Closed forms (no
 // per-row index table, which the compiler would hoist into 32 registers):

@@ -110,6 +110,8 @@ struct Q21Dev {
     // PHT path
     const uint32_t *ht_s, *ht_p, *ht_d;
     uint32_t s_len, p_len, d_len, s_min, p_min, d_min;
+    const uint32_t *occ_s, *occ_p;  // occupancy bitmaps of ht_s / ht_p (bit-filtered probe)
+    FastDiv s_div, p_div, d_div;    // s_len, p_len, d_len
     // compact path
     const uint32_t *supp_bm, *part_bm;
     const uint16_t *brand, *year;
@@ -172,6 +174,11 @@ using Q21Hash = GroupHash<10>;  // 1,024 entries, 20 KB
 constexpr uint32_t kQueue = 64;
 constexpr size_t kQueueBytes = 16ull * 3u * kQueue * 4u;  // 16 waves of a 1024-item block
 
+// Compact tables: p = part index, d = date index.  Reference hash tables
+// (PHT): p = the part's ht_p slot (its bit was set: the slot holds
+// [key, brand]), d = the date key, probed here in ht_d (PHT_2: an empty slot
+// drops the row, else year = slot >> 32).
+template <bool PHT>
 __device__ __forceinline__ void q21_flush(const Q21Dev& a, uint32_t* q, uint32_t qn, const uint16_t* year_tab,
                                           const Q21Hash& h) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -180,7 +187,15 @@ __device__ __forceinline__ void q21_flush(const Q21Dev& a, uint32_t* q, uint32_t
     const uint32_t l = threadIdx.x & 63u;
     if (l < qn) {
         const uint32_t p = q[l], d = q[kQueue + l];
-        if (d < a.n_date) {
+        if constexpr (PHT) {
+            const uint64_t ds =
+                *reinterpret_cast<const uint64_t*>(a.ht_d + 2u * fast_mod(d - a.d_min, a.d_div));
+            if (ds != 0) {
+                const uint32_t brand = a.ht_p[2u * p + 1u];
+                const uint32_t slot = fast_mod(brand * a.n_years + ((uint32_t)(ds >> 32) - a.year_base), a.slot_div);
+                h.add(slot, q[2 * kQueue + l], a.sums, a.counts);
+            }
+        } else if (d < a.n_date) {
             const uint32_t slot = fast_mod(a.brand[p] * a.n_years + (year_tab[d] - a.year_base), a.slot_div);
             h.add(slot, q[2 * kQueue + l], a.sums, a.counts);
         }
@@ -190,7 +205,12 @@ __device__ __forceinline__ void q21_flush(const Q21Dev& a, uint32_t* q, uint32_t
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int DBW, int SBW, int PBW, int RBW, bool USE_LDS>
+// PHT: the probe of the reference's hash tables with their occupancy bitmaps
+// (supplier and part bits staged in LDS like the compact bitmaps, indexed by
+// the tables' slot (key - min) % len); only the rows that pass both read the
+// part table (brand) and the date table.  The same rows qualify as in the
+// Crystal-primitive probe, which reads a table slot per row.
+template <int DBW, int SBW, int PBW, int RBW, bool USE_LDS, bool PHT = false>
 __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
     extern __shared__ __align__(16) uint32_t lds[];
     // the group hash table first (8-byte fields at offset 0), then the
@@ -199,16 +219,17 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
     h.init(lds);
     uint32_t* q = lds + Q21Hash::kBytes / 4u + (threadIdx.x >> 6) * 3u * kQueue;
     uint32_t* stage = lds + (Q21Hash::kBytes + kQueueBytes) / 4u;
-    const uint32_t* supp_bm = a.supp_bm;
-    const uint32_t* part_bm = a.part_bm;
+    const uint32_t* supp_bm = PHT ? a.occ_s : a.supp_bm;
+    const uint32_t* part_bm = PHT ? a.occ_p : a.part_bm;
     const uint16_t* year_tab = a.year;
     if constexpr (USE_LDS) {
         uint32_t* s = stage;
         uint32_t* p = stage + a.supp_words;
         uint16_t* y = reinterpret_cast<uint16_t*>(p + a.part_words);
-        for (uint32_t i = threadIdx.x; i < a.supp_words; i += blockDim.x) s[i] = a.supp_bm[i];
-        for (uint32_t i = threadIdx.x; i < a.part_words; i += blockDim.x) p[i] = a.part_bm[i];
-        for (uint32_t i = threadIdx.x; i < a.n_date; i += blockDim.x) y[i] = a.year[i];
+        for (uint32_t i = threadIdx.x; i < a.supp_words; i += blockDim.x) s[i] = supp_bm[i];
+        for (uint32_t i = threadIdx.x; i < a.part_words; i += blockDim.x) p[i] = part_bm[i];
+        if constexpr (!PHT)
+            for (uint32_t i = threadIdx.x; i < a.n_date; i += blockDim.x) y[i] = a.year[i];
         supp_bm = s;
         part_bm = p;
         year_tab = y;
@@ -237,8 +258,13 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
         // supplier semi-join on every row
         const uint32_t km = a.keys_min;
         uint32_t m = 0;
+        if constexpr (PHT) {
 #pragma unroll
-        for (int r = 0; r < 32; ++r) m |= bit_at(supp_bm, sc.get(r), km, a.n_supp) << r;
+            for (int r = 0; r < 32; ++r) m |= bit_at(supp_bm, fast_mod(sc.get(r) - a.s_min, a.s_div), 0u, a.s_len) << r;
+        } else {
+#pragma unroll
+            for (int r = 0; r < 32; ++r) m |= bit_at(supp_bm, sc.get(r), km, a.n_supp) << r;
+        }
         if (!vlive) m = 0;
         if (v == a.n_vec - 1) m &= tail;
         pc.pin();
@@ -247,15 +273,18 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
         if (!__any(m != 0)) continue;
         // part semi-join: every row from the LDS bitmap; from L2 only the rows
         // that passed the supplier (rows no lane of the wave holds skipped)
+        // part slot (PHT) or index (compact) of a key
+        auto part_idx = [&](uint32_t key) { return PHT ? fast_mod(key - a.p_min, a.p_div) : key - km; };
+        const uint32_t n_part_bits = PHT ? a.p_len : a.n_part;
         if constexpr (USE_LDS) {
-            bitmap_filter<16>(pc, part_bm, km, a.n_part, m);
+            bitmap_filter_by<16>(pc, part_bm, part_idx, n_part_bits, m);
         } else {
 #pragma unroll
             for (int r = 0; r < 32; ++r) {
                 if (__any((m >> r) & 1u)) {
-                    const uint32_t p = pc.get(r) - km;
+                    const uint32_t p = part_idx(pc.get(r));
                     const bool live = (m >> r) & 1u;
-                    const uint32_t hit = (live && p < a.n_part) ? (part_bm[p >> 5] >> (p & 31u)) & 1u : 0u;
+                    const uint32_t hit = (live && p < n_part_bits) ? (part_bm[p >> 5] >> (p & 31u)) & 1u : 0u;
                     if (!hit) m &= ~(1u << r);
                 }
             }
@@ -271,21 +300,40 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
             if (bal == 0) continue;
             const uint32_t cnt = (uint32_t)__popcll(bal);
             if (qn + cnt > kQueue) {
-                q21_flush(a, q, qn, year_tab, h);
+                q21_flush<PHT>(a, q, qn, year_tab, h);
                 qn = 0;
             }
             if (live) {
                 const uint32_t pos =
                     qn + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                q[pos] = pc.get(r) - km;
-                q[kQueue + pos] = dc.get(r) - km;
+                q[pos] = part_idx(pc.get(r));
+                q[kQueue + pos] = PHT ? dc.get(r) : dc.get(r) - km;
                 q[2 * kQueue + pos] = rc.get(r);
             }
             qn += cnt;
         }
     }
-    if (qn) q21_flush(a, q, qn, year_tab, h);
+    if (qn) q21_flush<PHT>(a, q, qn, year_tab, h);
     h.flush(a.sums, a.counts);
+}
+
+// Occupancy bitmap of a reference hash table: lane i of a wave tests slot
+// base + i, one ballot per 64 slots gives two bitmap words.
+__global__ __launch_bounds__(256) void pht_occupancy_kernel(const uint32_t* __restrict ht, uint32_t n, uint32_t pair,
+                                                            uint32_t* __restrict bits) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint32_t words = (n + 31u) / 32u;
+    for (uint64_t base = wave * 64u; base < n; base += waves * 64u) {  // uniform per wave
+        const uint64_t slot = base + lane;
+        bool occ = false;
+        if (slot < n) occ = pair ? (ht[2u * slot] | ht[2u * slot + 1u]) != 0u : ht[slot] != 0u;
+        const uint64_t b = __builtin_amdgcn_ballot_w64(occ);
+        const uint32_t w = (uint32_t)(base >> 5);
+        if (lane == 0u) bits[w] = (uint32_t)b;
+        if (lane == 1u && w + 1u < words) bits[w + 1u] = (uint32_t)(b >> 32);
+    }
 }
 
 // Two 16-bit values per register, row r in half r & 1 of register r >> 1.
@@ -369,9 +417,10 @@ int launch_resident_q21(Kernel k, int block, size_t lds, const Q21Dev& a, hipStr
     return check_launch();
 }
 
-template <int DBW, int SBW, int PBW, int RBW>
+template <int DBW, int SBW, int PBW, int RBW, bool PHT = false>
 int launch_compact(const Q21Dev& a, bool in_lds, size_t lds, hipStream_t s) {
-    auto k = in_lds ? q21_compact_kernel<DBW, SBW, PBW, RBW, true> : q21_compact_kernel<DBW, SBW, PBW, RBW, false>;
+    auto k = in_lds ? q21_compact_kernel<DBW, SBW, PBW, RBW, true, PHT>
+                    : q21_compact_kernel<DBW, SBW, PBW, RBW, false, PHT>;
     const size_t bytes = Q21Hash::kBytes + kQueueBytes + (in_lds ? lds : 0);
     FLS_HIP_CHECK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     return launch_resident_q21(k, 1024, bytes, a, s);
@@ -482,6 +531,18 @@ int fls_build_pht_u32(const uint32_t* d_filter, uint32_t filter_eq, const uint32
     return check_launch();
 }
 
+int fls_pht_occupancy_u32(const uint32_t* d_ht, uint32_t ht_len, uint32_t pair, uint32_t* d_bits, void* stream) {
+    using namespace fls;
+    FLS_RETURN_IF(!d_ht || !d_bits, FLS_ERR_NULL);
+    FLS_RETURN_IF(ht_len == 0, FLS_ERR_ARG);
+    const uint32_t waves = (uint32_t)ceil_div(ht_len, 64);
+    const uint64_t want = ceil_div(waves, 4);
+    const uint32_t grid = (uint32_t)(want < 4096 ? want : 4096);
+    hipLaunchKernelGGL(pht_occupancy_kernel, dim3(grid), dim3(256), 0, as_stream(stream), d_ht, ht_len, pair ? 1u : 0u,
+                       d_bits);
+    return check_launch();
+}
+
 int fls_query_q21_pht(const fls_column* datekey, const fls_column* suppkey, const fls_column* partkey,
                       const fls_column* revenue, uint64_t n_rows, const fls_q21_pht* ht, const fls_q21_params* params,
                       int64_t* d_sums, uint64_t* d_counts, uint32_t flags, void* stream) {
@@ -503,9 +564,23 @@ int fls_query_q21_pht(const fls_column* datekey, const fls_column* suppkey, cons
     a.s_min = ht->s_min;
     a.p_min = ht->p_min;
     a.d_min = ht->d_min;
+    const bool fast = a.date.bw == 12 && a.supp.bw == 16 && a.part.bw == 20 && a.rev.bw == 24;
+    if (ht->occ_s && ht->occ_p && fast) {
+        // bit-filtered probe over the same tables (SSB widths; others take the per-row probe)
+        a.occ_s = ht->occ_s;
+        a.occ_p = ht->occ_p;
+        a.s_div = make_fastdiv(ht->s_len);
+        a.p_div = make_fastdiv(ht->p_len);
+        a.d_div = make_fastdiv(ht->d_len);
+        a.supp_words = (ht->s_len + 31u) / 32u;
+        a.part_words = (ht->p_len + 31u) / 32u;
+        a.n_groups = ceil_div(a.n_vec, 32);
+        const size_t lds = 4ull * (a.supp_words + a.part_words) + 16u;
+        const bool in_lds = lds + Q21Hash::kBytes + kQueueBytes <= 150u * 1024u;
+        return launch_compact<12, 16, 20, 24, true>(a, in_lds, lds, s);
+    }
     a.n_groups = ceil_div(a.n_vec, 8);
-    if (a.date.bw == 12 && a.supp.bw == 16 && a.part.bw == 20 && a.rev.bw == 24)
-        return launch_resident_q21(q21_pht_kernel<12, 16, 20, 24>, 256, 0, a, s);
+    if (fast) return launch_resident_q21(q21_pht_kernel<12, 16, 20, 24>, 256, 0, a, s);
     return launch_resident_q21(q21_pht_kernel<-1, -1, -1, -1>, 256, 0, a, s);
 }
 

@@ -239,7 +239,9 @@ typedef struct fls_q21_params {
 /* Reference perfect hash tables (crystal/join.cuh:6-275).  PHT_1 holds the key
  * in slot HASH(key) = (key - min) % len, 0 meaning empty; PHT_2 holds
  * [key, value] pairs.  ht_s: supplier PHT_1; ht_p: part PHT_2 -> brand; ht_d:
- * date PHT_2 -> year.                                                         */
+ * date PHT_2 -> year.  occ_s / occ_p: NULL, or the occupancy bitmaps of ht_s
+ * and ht_p (fls_pht_occupancy_u32); with both given, the probe tests the bits
+ * (staged in LDS) and reads the tables only for rows that pass both.       */
 typedef struct fls_q21_pht {
     const uint32_t* ht_s;
     uint32_t s_len, s_min;
@@ -247,7 +249,16 @@ typedef struct fls_q21_pht {
     uint32_t p_len, p_min;
     const uint32_t* ht_d;
     uint32_t d_len, d_min;
+    const uint32_t* occ_s;
+    const uint32_t* occ_p;
 } fls_q21_pht;
+
+/* Occupancy bitmap of a reference hash table: bit i of d_bits[i / 32] is set
+ * when slot i is not empty (PHT_1: ht[i] != 0; pair != 0, PHT_2: the 64-bit
+ * slot [ht[2i], ht[2i+1]] != 0), the test the reference's probe makes per
+ * row (crystal/join.cuh:8-149).  d_bits holds ceil(ht_len / 32) words.  No
+ * reference counterpart.                                                     */
+int fls_pht_occupancy_u32(const uint32_t* d_ht, uint32_t ht_len, uint32_t pair, uint32_t* d_bits, void* stream);
 
 /* build_hashtable_s/_p/_d (fls_q21.cu:197-263): rows with
  * d_filter[i] == filter_eq (every row when d_filter is NULL) enter the table
@@ -256,7 +267,11 @@ typedef struct fls_q21_pht {
  * (fls_q21.cu:294-297).  Keys must be non-zero.                              */
 int fls_build_pht_u32(const uint32_t* d_filter, uint32_t filter_eq, const uint32_t* d_keys, const uint32_t* d_vals,
                       uint32_t n, uint32_t* d_ht, uint32_t ht_len, uint32_t keys_min, void* stream);
-/* probe_v1 (fls_q21.cu:21-78) over reference hash tables: drop-in path.      */
+/* probe_v1 (fls_q21.cu:21-78) over reference hash tables: drop-in path.  With
+ * the occupancy bitmaps in *ht and columns at the SSB widths (date 12,
+ * supplier 16, part 20, revenue 24 bits), the bit-filtered probe; otherwise
+ * one table slot read per row and table through the Crystal primitives.  The
+ * same rows qualify either way.                                             */
 int fls_query_q21_pht(const fls_column* datekey, const fls_column* suppkey, const fls_column* partkey,
                       const fls_column* revenue, uint64_t n_rows, const fls_q21_pht* ht, const fls_q21_params* params,
                       int64_t* d_sums, uint64_t* d_counts, uint32_t flags, void* stream);

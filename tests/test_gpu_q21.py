@@ -22,9 +22,13 @@ def run_both(fact, dims_np, sizes, n, enc=None):
     cols = [c.to_device() for c in enc]
     ws = fls.q21_build(d)
     out_c = fls.query_q21(*cols, n, d, ws).cpu().numpy()
-    pht, keep = fls.q21_build_pht(d)
+    # reference tables probed per row, and through their occupancy bitmaps
+    pht, keep = fls.q21_build_pht(d, occupancy=False)
     out_p = fls.query_q21_pht(*cols, n, pht).cpu().numpy()
-    del keep
+    pht_b, keep_b = fls.q21_build_pht(d)
+    out_b = fls.query_q21_pht(*cols, n, pht_b).cpu().numpy()
+    np.testing.assert_array_equal(out_b, out_p)
+    del keep, keep_b
     return out_c, out_p
 
 
@@ -32,6 +36,23 @@ def check(out, want):
     sums, counts = want
     np.testing.assert_array_equal(out[7000:].view(np.uint64), counts)
     np.testing.assert_array_equal(out[:7000], sums)
+
+
+@pytest.mark.parametrize("pair", [False, True])
+@pytest.mark.parametrize("n", [1, 63, 64, 1000 + 7, 800_000])
+def test_pht_occupancy_bits(pair, n):
+    torch = T.torch_cuda()
+    rng = np.random.default_rng(n + pair)
+    words = 2 * n if pair else n
+    ht = rng.integers(1, 2**32, words, dtype=np.uint64).astype(np.uint32)
+    ht[rng.random(words) < 0.7] = 0
+    if pair:  # a slot is empty only when both words are
+        occ = (ht[0::2] != 0) | (ht[1::2] != 0)
+    else:
+        occ = ht != 0
+    bits = fls.pht_occupancy(torch.from_numpy(ht.view(np.int32)).cuda(), n, pair)
+    got = np.unpackbits(T.from_dev(bits, np.uint32).view(np.uint8), bitorder="little")[:n].astype(bool)
+    np.testing.assert_array_equal(got, occ)
 
 
 def test_generators_match_oracle():
@@ -79,6 +100,12 @@ def test_q21_keys_beyond_the_dimension_never_match(col, fixed):
     out = fls.query_q21(*cols, n, d, fls.q21_build(d)).cpu().numpy()
     kept = [np.ascontiguousarray(x[~bad]) for x in fact]
     check(out, T.o_q21(kept, dims, int((~bad).sum())))
+    if fixed:  # both reference-table probes wrap such keys by the same modulo
+        pht, keep = fls.q21_build_pht(d, occupancy=False)
+        pht_b, keep_b = fls.q21_build_pht(d)
+        np.testing.assert_array_equal(fls.query_q21_pht(*cols, n, pht_b).cpu().numpy(),
+                                      fls.query_q21_pht(*cols, n, pht).cpu().numpy())
+        del keep, keep_b
 
 
 def test_q21_runtime_widths():
