@@ -214,6 +214,24 @@ struct LaneCol {
     __device__ __forceinline__ void pin() { pin_words(w); }
 };
 
+// Row r of lane `lane` of vector v of a fixed-width bit-packed column, read
+// from memory: the word that holds its first bit and, when the value crosses
+// a word boundary, the next one (a late read for the few rows that survive a
+// query's filters; the rest of the vector's words are never fetched).
+template <int BW>
+__device__ __forceinline__ uint32_t read_row(const ColDev& c, uint64_t v, uint32_t lane, uint32_t r) {
+    static_assert(BW >= 0 && BW <= 32, "fixed widths only");
+    if constexpr (BW == 0) {
+        return c.base;
+    } else {
+        const uint32_t* p = c.words + v * (32u * BW) + lane;
+        const uint32_t bit = r * (uint32_t)BW, k = bit >> 5, s = bit & 31u;
+        const uint32_t k1 = s + BW > 32u ? k + 1u : k;  // k + 1 < BW whenever it is taken
+        const uint32_t x = __builtin_amdgcn_alignbit(p[32u * k1], p[32u * k], s);
+        return (BW == 32 ? x : x & dev::Mask<BW>::value) + c.base;
+    }
+}
+
 // Bit key - kmin of a bitmap of n >= 1 bits, 0 for keys out of range.  The
 // load reads word 0 for those and masks the result, so it is unconditional and
 // a lane's 32 rows' loads issue back to back (a load under a range test

@@ -168,9 +168,12 @@ __global__ __launch_bounds__(256) void q21_pht_kernel(const Q21Dev a) {
 // supplier and part bitmaps and the year table when they fit (USE_LDS).
 using Q21Hash = GroupHash<10>;  // 1,024 entries, 20 KB
 
-// Per-wave LDS queue of rows that passed every filter: part index, date
-// index and revenue, kQueue of each.  The flush runs with one lane per row:
-// brand from L2, year from the year table, then the block's hash table.
+// Per-wave LDS queue of rows that passed every filter, kQueue of each field.
+// The flush runs with one lane per row: brand from L2, year from the year
+// table, then the block's hash table.  With fixed-width date and revenue
+// columns (LATE) a row is queued as its part index and its place (vector;
+// lane << 5 | row), and the flush reads its date and revenue words from
+// memory beside the brand; otherwise the decoded date and revenue are queued.
 constexpr uint32_t kQueue = 64;
 constexpr size_t kQueueBytes = 16ull * 3u * kQueue * 4u;  // 16 waves of a 1024-item block
 
@@ -178,7 +181,7 @@ constexpr size_t kQueueBytes = 16ull * 3u * kQueue * 4u;  // 16 waves of a 1024-
 // (PHT): p = the part's ht_p slot (its bit was set: the slot holds
 // [key, brand]), d = the date key, probed here in ht_d (PHT_2: an empty slot
 // drops the row, else year = slot >> 32).
-template <bool PHT>
+template <bool PHT, bool LATE, int DBW, int RBW>
 __device__ __forceinline__ void q21_flush(const Q21Dev& a, uint32_t* q, uint32_t qn, const uint16_t* year_tab,
                                           const Q21Hash& h) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -186,18 +189,29 @@ __device__ __forceinline__ void q21_flush(const Q21Dev& a, uint32_t* q, uint32_t
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const uint32_t l = threadIdx.x & 63u;
     if (l < qn) {
-        const uint32_t p = q[l], d = q[kQueue + l];
+        const uint32_t p = q[l];
+        uint32_t d, rev;
+        if constexpr (LATE) {
+            const uint64_t v = q[kQueue + l];
+            const uint32_t lane = q[2 * kQueue + l] >> 5, r = q[2 * kQueue + l] & 31u;
+            d = read_row<DBW>(a.date, v, lane, r);
+            rev = read_row<RBW>(a.rev, v, lane, r);
+            if constexpr (!PHT) d -= a.keys_min;
+        } else {
+            d = q[kQueue + l];
+            rev = q[2 * kQueue + l];
+        }
         if constexpr (PHT) {
             const uint64_t ds =
                 *reinterpret_cast<const uint64_t*>(a.ht_d + 2u * fast_mod(d - a.d_min, a.d_div));
             if (ds != 0) {
                 const uint32_t brand = a.ht_p[2u * p + 1u];
                 const uint32_t slot = fast_mod(brand * a.n_years + ((uint32_t)(ds >> 32) - a.year_base), a.slot_div);
-                h.add(slot, q[2 * kQueue + l], a.sums, a.counts);
+                h.add(slot, rev, a.sums, a.counts);
             }
         } else if (d < a.n_date) {
             const uint32_t slot = fast_mod(a.brand[p] * a.n_years + (year_tab[d] - a.year_base), a.slot_div);
-            h.add(slot, q[2 * kQueue + l], a.sums, a.counts);
+            h.add(slot, rev, a.sums, a.counts);
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -210,8 +224,13 @@ __device__ __forceinline__ void q21_flush(const Q21Dev& a, uint32_t* q, uint32_t
 // the tables' slot (key - min) % len); only the rows that pass both read the
 // part table (brand) and the date table.  The same rows qualify as in the
 // Crystal-primitive probe, which reads a table slot per row.
+// With fixed-width date and revenue columns (LATE) only the supplier and part
+// columns are read whole; a surviving row's date and revenue are read at the
+// flush (about 17 rows in 2,048 qualify, so most of those two columns' 128-byte
+// lines are never fetched).
 template <int DBW, int SBW, int PBW, int RBW, bool USE_LDS, bool PHT = false>
 __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
+    constexpr bool LATE = DBW >= 0 && RBW >= 0;
     extern __shared__ __align__(16) uint32_t lds[];
     // the group hash table first (8-byte fields at offset 0), then the
     // queues, then the staged tables
@@ -249,12 +268,14 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
         const uint64_t vl = vlive ? v : a.n_vec - 1;
         LaneCol<SBW> sc;
         LaneCol<PBW> pc;
-        LaneCol<DBW> dc;
-        LaneCol<RBW> rc;
+        LaneCol<LATE ? 0 : DBW> dc;
+        LaneCol<LATE ? 0 : RBW> rc;
         sc.load(a.supp, vl);
         pc.load(a.part, vl);
-        dc.load(a.date, vl);
-        rc.load(a.rev, vl);
+        if constexpr (!LATE) {
+            dc.load(a.date, vl);
+            rc.load(a.rev, vl);
+        }
         // supplier semi-join on every row
         const uint32_t km = a.keys_min;
         uint32_t m = 0;
@@ -268,8 +289,10 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
         if (!vlive) m = 0;
         if (v == a.n_vec - 1) m &= tail;
         pc.pin();
-        dc.pin();
-        rc.pin();
+        if constexpr (!LATE) {
+            dc.pin();
+            rc.pin();
+        }
         if (!__any(m != 0)) continue;
         // part semi-join: every row from the LDS bitmap; from L2 only the rows
         // that passed the supplier (rows no lane of the wave holds skipped)
@@ -300,20 +323,25 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
             if (bal == 0) continue;
             const uint32_t cnt = (uint32_t)__popcll(bal);
             if (qn + cnt > kQueue) {
-                q21_flush<PHT>(a, q, qn, year_tab, h);
+                q21_flush<PHT, LATE, DBW, RBW>(a, q, qn, year_tab, h);
                 qn = 0;
             }
             if (live) {
                 const uint32_t pos =
                     qn + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
                 q[pos] = part_idx(pc.get(r));
-                q[kQueue + pos] = PHT ? dc.get(r) : dc.get(r) - km;
-                q[2 * kQueue + pos] = rc.get(r);
+                if constexpr (LATE) {
+                    q[kQueue + pos] = (uint32_t)vl;  // n_vec < 2^32 (checked at launch)
+                    q[2 * kQueue + pos] = dev::lane32() << 5 | (uint32_t)r;
+                } else {
+                    q[kQueue + pos] = PHT ? dc.get(r) : dc.get(r) - km;
+                    q[2 * kQueue + pos] = rc.get(r);
+                }
             }
             qn += cnt;
         }
     }
-    if (qn) q21_flush<PHT>(a, q, qn, year_tab, h);
+    if (qn) q21_flush<PHT, LATE, DBW, RBW>(a, q, qn, year_tab, h);
     h.flush(a.sums, a.counts);
 }
 
@@ -419,6 +447,7 @@ int launch_resident_q21(Kernel k, int block, size_t lds, const Q21Dev& a, hipStr
 
 template <int DBW, int SBW, int PBW, int RBW, bool PHT = false>
 int launch_compact(const Q21Dev& a, bool in_lds, size_t lds, hipStream_t s) {
+    FLS_RETURN_IF(a.n_vec >> 32, FLS_ERR_ARG);  // the row queue holds a vector index in 32 bits
     auto k = in_lds ? q21_compact_kernel<DBW, SBW, PBW, RBW, true, PHT>
                     : q21_compact_kernel<DBW, SBW, PBW, RBW, false, PHT>;
     const size_t bytes = Q21Hash::kBytes + kQueueBytes + (in_lds ? lds : 0);
