@@ -241,35 +241,9 @@ constexpr uint32_t kQ31Queue = 128;
 constexpr size_t kQ31QueueBytes = (size_t)kStarVecs / 2u * 3u * kQ31Queue * 4u;  // 16 waves, 24 KB
 using Q31Hash = GroupHash<9>;  // 512 entries, 10 KB
 
-struct Q31Lds {
-    StarLds t;              // kQ31Codes / kQ31Staged: group table and code pointers
-    const uint32_t* c_bits;  // kQ31Bitmap
-    Q31Hash h;               // kQ31Bitmap
-};
-
+// kQ31Codes and kQ31Staged (the kQ31Bitmap probe is q31_bitmap_kernel)
 template <int MODE>
-__device__ __forceinline__ Q31Lds q31_stage(const StarDev& a, uint8_t* lds) {
-    Q31Lds s{};
-    if constexpr (MODE == kQ31Bitmap) {
-        s.h.init(lds);
-        uint32_t* cb = reinterpret_cast<uint32_t*>(lds + Q31Hash::kBytes);
-        uint8_t* sc = reinterpret_cast<uint8_t*>(cb + a.c_words);
-        uint8_t* yc = sc + a.n_supp;
-        for (uint32_t i = threadIdx.x; i < a.c_words; i += blockDim.x) cb[i] = a.c_bits[i];
-        for (uint32_t i = threadIdx.x; i < a.n_supp; i += blockDim.x) sc[i] = a.s_code[i];
-        for (uint32_t i = threadIdx.x; i < a.n_date; i += blockDim.x) yc[i] = a.y_code[i];
-        s.c_bits = cb;
-        s.t.s_code = sc;
-        s.t.y_code = yc;
-        __syncthreads();
-    } else {
-        s.t = star_stage<MODE == kQ31Staged, false>(a, lds);
-    }
-    return s;
-}
-
-template <int MODE>
-__device__ __forceinline__ void q31_flush(const StarDev& a, const Q31Lds& s, const uint32_t* q, uint32_t qn) {
+__device__ __forceinline__ void q31_flush(const StarDev& a, const StarLds& t, const uint32_t* q, uint32_t qn) {
     constexpr int kPer = kQ31Queue / 64;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -288,13 +262,10 @@ __device__ __forceinline__ void q31_flush(const StarDev& a, const Q31Lds& s, con
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const uint32_t i = l + 64u * j;
-        const uint32_t y = code_at(s.t.y_code, dk[j] & 0xFFFFFFu, 0u, a.n_date);
+        const uint32_t y = code_at(t.y_code, dk[j] & 0xFFFFFFu, 0u, a.n_date);
         if (cv[j] != kNone && y != kNone) {
             const uint32_t slot = fast_mod((dk[j] >> 24) * a.slot_mul_s + cv[j] * a.slot_mul_c + y, a.slot_div);
-            if constexpr (MODE == kQ31Bitmap)
-                s.h.add(slot, q[2u * kQ31Queue + i], a.sums, a.counts);
-            else
-                group_add_lds(s.t, slot, q[2u * kQ31Queue + i]);
+            group_add_lds(t, slot, q[2u * kQ31Queue + i]);
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -307,7 +278,8 @@ template <int DBW, int CBW, int SBW, int RBW, int MODE>
 __global__ __launch_bounds__(kStarBlock) void q31_kernel(const StarDev a) {
     extern __shared__ __align__(16) uint8_t lds[];
     uint32_t* q = reinterpret_cast<uint32_t*>(lds) + (threadIdx.x >> 6) * 3u * kQ31Queue;
-    const Q31Lds s = q31_stage<MODE>(a, lds + kQ31QueueBytes);
+    static_assert(MODE == kQ31Codes || MODE == kQ31Staged, "kQ31Bitmap runs q31_bitmap_kernel");
+    const StarLds t = star_stage<MODE == kQ31Staged, false>(a, lds + kQ31QueueBytes);
     const uint32_t tail = live_row_mask((uint32_t)(a.n_rows - (a.n_vec - 1) * 1024u), a.rev.transposed != 0);
     const uint32_t km = a.keys_min;
     uint32_t qn = 0;  // wave-uniform; the queue carries over from step to step
@@ -330,7 +302,7 @@ __global__ __launch_bounds__(kStarBlock) void q31_kernel(const StarDev a) {
         uint32_t m = 0, s8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
-            const uint32_t sn = code_at(s.t.s_code, sc.get(r), km, a.n_supp);
+            const uint32_t sn = code_at(t.s_code, sc.get(r), km, a.n_supp);
             put8(s8, r, sn);
             m |= (uint32_t)(sn != kNone) << r;
         }
@@ -340,15 +312,6 @@ __global__ __launch_bounds__(kStarBlock) void q31_kernel(const StarDev a) {
         dc.pin();
         rc.pin();
         if (!__any(m != 0)) continue;
-        if constexpr (MODE == kQ31Bitmap) {
-            // customer semi-join on every row from the LDS bitmap: only the
-            // rows that pass both filters (about 4 %) reach the queue
-            // one read per row: batching the reads (bitmap_filter) spills
-            // 28-40 VGPRs here, at any batch size from 4 to 16
-#pragma unroll
-            for (int r = 0; r < 32; ++r) m &= ~((bit_at(s.c_bits, cc.get(r), km, a.n_cust) ^ 1u) << r);
-            if (!__any(m != 0)) continue;
-        }
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
             // flush only when this row slot's rows would not fit: the flushes
@@ -356,7 +319,7 @@ __global__ __launch_bounds__(kStarBlock) void q31_kernel(const StarDev a) {
             const bool live = (m >> r) & 1u;
             const uint64_t bal = __ballot(live);
             if (qn + (uint32_t)__popcll(bal) > kQ31Queue) {
-                q31_flush<MODE>(a, s, q, qn);
+                q31_flush<MODE>(a, t, q, qn);
                 qn = 0;
             }
             if (live) {
@@ -370,11 +333,141 @@ __global__ __launch_bounds__(kStarBlock) void q31_kernel(const StarDev a) {
             qn += (uint32_t)__popcll(bal);
         }
     }
-    if (qn) q31_flush<MODE>(a, s, q, qn);
-    if constexpr (MODE == kQ31Bitmap)
-        s.h.flush(a.sums, a.counts);
-    else
-        star_flush(a, s.t);
+    if (qn) q31_flush<MODE>(a, t, q, qn);
+    star_flush(a, t);
+}
+
+// kQ31Bitmap probe.  LDS: the row queues, the group hash table, then the
+// supplier codes at a fixed offset (one byte per supplier and a kNone entry
+// at index n_supp), the year codes, and the customer filter bitmap (with a
+// zero bit at index n_cust).  A key k of a row tests entry min(k, n), so an
+// out-of-range key (below keys_min it wraps) reads the sentinel: one min per
+// lookup instead of a compare and two selects.  Rows are queued as raw
+// packed fields (customer; supplier | date << SBW; revenue) and the flush
+// adds the frame bases and keys_min, looks the supplier nation up again and
+// the year, and gathers the customer nation from L2.
+template <int DBW, int CBW, int SBW, int RBW>
+__device__ __forceinline__ void q31_bitmap_flush(const StarDev& a, const uint8_t* s_tab, const uint8_t* y_tab,
+                                                 const Q31Hash& h, const uint32_t* q, uint32_t qn) {
+    constexpr int kPer = kQ31Queue / 64;
+    constexpr uint32_t kSMask = (1u << SBW) - 1u;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t l = threadIdx.x & 63u;
+    const uint32_t km = a.keys_min;
+    uint32_t ck[kPer], sd[kPer], cv[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const uint32_t i = l + 64u * j;
+        const uint32_t x = q[i];
+        ck[j] = i < qn ? x + (a.cust.base - km) : 0xFFFFFFFFu;  // rows past the queue: out of range, dropped
+        sd[j] = q[kQ31Queue + i];
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) cv[j] = code_at(a.c_code, ck[j], 0u, a.n_cust);
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const uint32_t i = l + 64u * j;
+        const uint32_t sk = min((sd[j] & kSMask) + (a.supp.base - km), a.n_supp);
+        const uint32_t y = code_at(y_tab, (sd[j] >> SBW) + (a.date.base - km), 0u, a.n_date);
+        if (cv[j] != kNone && y != kNone) {
+            const uint32_t slot = fast_mod(s_tab[sk] * a.slot_mul_s + cv[j] * a.slot_mul_c + y, a.slot_div);
+            h.add(slot, q[2u * kQ31Queue + i] + a.rev.base, a.sums, a.counts);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr size_t kQ31SupplierLds = kQ31QueueBytes + Q31Hash::kBytes;  // byte offset of the supplier codes
+
+size_t q31_bitmap_lds(uint32_t n_supp, uint32_t n_date, uint32_t c_words) {
+    return ((kQ31SupplierLds + n_supp + 1u + n_date + 3u) & ~(size_t)3u) + 4ull * (c_words + 1u);
+}
+
+template <int DBW, int CBW, int SBW, int RBW>
+__global__ __launch_bounds__(kStarBlock) void q31_bitmap_kernel(const StarDev a) {
+    static_assert(DBW >= 0 && CBW >= 0 && SBW >= 0 && RBW >= 0 && DBW + SBW <= 32,
+                  "fixed widths; supplier and date share a queue word");
+    extern __shared__ __align__(16) uint8_t lds[];
+    uint32_t* q = reinterpret_cast<uint32_t*>(lds) + (threadIdx.x >> 6) * 3u * kQ31Queue;
+    Q31Hash h;
+    h.init(lds + kQ31QueueBytes);
+    uint8_t* s_tab = lds + kQ31SupplierLds;
+    uint8_t* y_tab = s_tab + a.n_supp + 1u;
+    uint32_t* c_tab =
+        reinterpret_cast<uint32_t*>(lds + ((kQ31SupplierLds + a.n_supp + 1u + a.n_date + 3u) & ~(size_t)3u));
+    for (uint32_t i = threadIdx.x; i <= a.n_supp; i += blockDim.x) s_tab[i] = i < a.n_supp ? a.s_code[i] : kNone;
+    for (uint32_t i = threadIdx.x; i < a.n_date; i += blockDim.x) y_tab[i] = a.y_code[i];
+    for (uint32_t i = threadIdx.x; i <= a.c_words; i += blockDim.x) c_tab[i] = i < a.c_words ? a.c_bits[i] : 0u;
+    __syncthreads();
+    const uint32_t tail = live_row_mask((uint32_t)(a.n_rows - (a.n_vec - 1) * 1024u), a.rev.transposed != 0);
+    const uint32_t km = a.keys_min;
+    const uint32_t d_s = a.supp.base - km, d_c = a.cust.base - km;
+    uint32_t qn = 0;  // wave-uniform; the queue carries over from step to step
+    for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
+        const uint64_t v = g * kStarVecs + (threadIdx.x >> 5);
+        // the whole wave stays in step (the queue flush needs every lane): a
+        // half-wave past the last vector reads the last vector again (in
+        // bounds) and keeps no rows
+        const bool vlive = v < a.n_vec;
+        if (!__any(vlive)) continue;
+        const uint64_t vl = vlive ? v : a.n_vec - 1;
+        LaneCol<SBW> sc;
+        LaneCol<CBW> cc;
+        LaneCol<DBW> dc;
+        LaneCol<RBW> rc;
+        sc.load(a.supp, vl);
+        cc.load(a.cust, vl);
+        dc.load(a.date, vl);
+        rc.load(a.rev, vl);
+        // supplier semi-join on every row: a miss is the kNone code
+        uint32_t miss = 0;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            const uint32_t k = min(dev::extract_at<SBW>(sc.w, r) + d_s, a.n_supp);
+            miss |= (((uint32_t)s_tab[k] + 1u) >> 8) << r;
+        }
+        uint32_t m = vlive ? ~miss : 0u;
+        if (v == a.n_vec - 1) m &= tail;
+        cc.pin();
+        dc.pin();
+        rc.pin();
+        if (!__any(m != 0)) continue;
+        // customer semi-join on every row from the LDS bitmap: only the rows
+        // that pass both filters (about 4 %) reach the queue
+        uint32_t hit = 0;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            const uint32_t k = min(dev::extract_at<CBW>(cc.w, r) + d_c, a.n_cust);
+            hit |= ((c_tab[k >> 5] >> (k & 31u)) & 1u) << r;
+        }
+        m &= hit;
+        if (!__any(m != 0)) continue;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            // flush only when this row slot's rows would not fit: the flushes
+            // run nearly full (a chain of round trips each, whatever their size)
+            const bool live = (m >> r) & 1u;
+            const uint64_t bal = __ballot(live);
+            if (qn + (uint32_t)__popcll(bal) > kQ31Queue) {
+                q31_bitmap_flush<DBW, CBW, SBW, RBW>(a, s_tab, y_tab, h, q, qn);
+                qn = 0;
+            }
+            if (live) {
+                const uint32_t pos =
+                    qn + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                q[pos] = dev::extract_at<CBW>(cc.w, r);
+                q[kQ31Queue + pos] = dev::extract_at<SBW>(sc.w, r) | dev::extract_at<DBW>(dc.w, r) << SBW;
+                q[2u * kQ31Queue + pos] = dev::extract_at<RBW>(rc.w, r);
+            }
+            qn += (uint32_t)__popcll(bal);
+        }
+    }
+    if (qn) q31_bitmap_flush<DBW, CBW, SBW, RBW>(a, s_tab, y_tab, h, q, qn);
+    h.flush(a.sums, a.counts);
 }
 
 // Q3.1 probe over columns of any encoding and width: one column decoded at a
@@ -708,8 +801,8 @@ int fls_query_q31(const fls_column* datekey, const fls_column* custkey, const fl
     const bool fast = all_bitpack(outs, 4) && a.date.bw == 12 && a.cust.bw == 20 && a.supp.bw == 16 &&
                       a.rev.bw == 24 && dims->n_date < (1u << 24);
     if (fast) {
-        const size_t bitmap = kQ31QueueBytes + Q31Hash::kBytes + 4ull * a.c_words + dims->n_supp + dims->n_date;
-        if (bitmap <= kLdsMax) return launch_star(q31_kernel<12, 20, 16, 24, kQ31Bitmap>, bitmap, a, s);
+        const size_t bitmap = q31_bitmap_lds(dims->n_supp, dims->n_date, a.c_words);
+        if (bitmap <= kLdsMax) return launch_star(q31_bitmap_kernel<12, 20, 16, 24>, bitmap, a, s);
         if (kQ31QueueBytes + staged <= kLdsMax)
             return launch_star(q31_kernel<12, 20, 16, 24, kQ31Staged>, kQ31QueueBytes + staged, a, s);
         return launch_star(q31_kernel<12, 20, 16, 24, kQ31Codes>, kQ31QueueBytes + table, a, s);
