@@ -23,7 +23,7 @@ constexpr uint32_t kMaxSlots = 8192;
 
 // workspace layout (byte offsets, 256-aligned)
 struct StarLayout {
-    uint64_t s_code, c_code, p_bits, y_code, c_bits, s_bits, total;
+    uint64_t s_code, c_code, p_bits, y_code, c_bits, s_bits, masks, total;
     uint32_t p_words, c_words, s_words;
 };
 
@@ -39,7 +39,8 @@ StarLayout star_layout(const fls_star_dims& d) {
     L.y_code = up(L.p_bits + 4ull * L.p_words);
     L.c_bits = up(L.y_code + d.n_date);
     L.s_bits = up(L.c_bits + 4ull * L.c_words);
-    L.total = up(L.s_bits + 4ull * L.s_words);
+    L.masks = up(L.s_bits + 4ull * L.s_words);  // Q3.1: qualifying supplier and customer nations, 8 words each
+    L.total = up(L.masks + 64u);
     return L;
 }
 
@@ -50,17 +51,32 @@ __global__ void star_zero_words(uint32_t* __restrict w, uint32_t n) {
 // Q3.1 builds (build_hashtable_s/_c/_d, fls_q31.cu:300-372): nation of the
 // qualifying suppliers and customers (and a bit per qualifying customer;
 // c_bits zeroed before), year offset of the dates in range
-__global__ void q31_build_kernel(fls_star_dims d, fls_q31_params p, uint8_t* __restrict s_code,
-                                 uint8_t* __restrict c_code, uint8_t* __restrict y_code,
-                                 uint32_t* __restrict c_bits) {
+// (and the set of nations that qualify on each side, 256 bits each in
+// `masks`, zeroed before: supplier nations in words 0-7, customer in 8-15)
+__global__ __launch_bounds__(256) void q31_build_kernel(fls_star_dims d, fls_q31_params p, uint8_t* __restrict s_code,
+                                                        uint8_t* __restrict c_code, uint8_t* __restrict y_code,
+                                                        uint32_t* __restrict c_bits, uint32_t* __restrict masks) {
+    __shared__ uint32_t nm[16];
+    if (threadIdx.x < 16) nm[threadIdx.x] = 0u;
+    __syncthreads();
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_supp; i += stride)
-        s_code[i] = (d.s_region[i] == p.s_region && d.s_nation[i] < kNone) ? (uint8_t)d.s_nation[i] : kNone;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_cust; i += stride) {
-        const bool hit = d.c_region[i] == p.c_region && d.c_nation[i] < kNone;
-        c_code[i] = hit ? (uint8_t)d.c_nation[i] : kNone;
-        if (hit) atomicOr(&c_bits[i >> 5], 1u << (i & 31u));
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_supp; i += stride) {
+        const uint32_t n = d.s_nation[i];
+        const bool hit = d.s_region[i] == p.s_region && n < kNone;
+        s_code[i] = hit ? (uint8_t)n : kNone;
+        if (hit) atomicOr(&nm[n >> 5], 1u << (n & 31u));
     }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_cust; i += stride) {
+        const uint32_t n = d.c_nation[i];
+        const bool hit = d.c_region[i] == p.c_region && n < kNone;
+        c_code[i] = hit ? (uint8_t)n : kNone;
+        if (hit) {
+            atomicOr(&c_bits[i >> 5], 1u << (i & 31u));
+            atomicOr(&nm[8u + (n >> 5)], 1u << (n & 31u));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 16 && nm[threadIdx.x]) atomicOr(&masks[threadIdx.x], nm[threadIdx.x]);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_date; i += stride) {
         const uint32_t y = d.d_year[i];
         y_code[i] = (y >= p.year_lo && y <= p.year_hi) ? (uint8_t)(y - p.year_base) : kNone;
@@ -94,7 +110,9 @@ struct StarDev {
     uint64_t n_rows, n_vec, n_groups;
     const uint8_t *s_code, *c_code, *y_code;
     const uint32_t *p_bits, *c_bits, *s_bits;
+    const uint32_t* masks;  // Q3.1: qualifying supplier nations (words 0-7) and customer nations (8-15)
     uint32_t n_supp, n_cust, n_part, n_date, keys_min, p_words, c_words, s_words;
+    uint32_t y_span;  // Q3.1: year codes run 0 .. y_span - 1
     uint32_t slot_mul_s, slot_mul_c, n_slots;  // slot = (s*slot_mul_s + c*slot_mul_c + y) % n_slots
     FastDiv slot_div;                          // n_slots
     int64_t* sums;
@@ -346,9 +364,38 @@ __global__ __launch_bounds__(kStarBlock) void q31_kernel(const StarDev a) {
 // packed fields (customer; supplier | date << SBW; revenue) and the flush
 // adds the frame bases and keys_min, looks the supplier nation up again and
 // the year, and gathers the customer nation from L2.
+// per wave: three fields of kQ31Queue entries and a spare entry each
+constexpr uint32_t kQ31Stride = kQ31Queue + 1u;
+constexpr size_t kQ31BitmapQueueBytes = (size_t)kStarVecs / 2u * 3u * kQ31Stride * 4u;
+constexpr size_t kQ31SupplierLds = kQ31BitmapQueueBytes + Q31Hash::kBytes;  // byte offset of the supplier codes
+
+// Dense group table (the hash table's LDS when it fits): one entry per
+// (qualifying supplier nation, qualifying customer nation, year code), ranked
+// among the nations that qualify -- SSB: 5 x 5 x 6 = 150 groups, added to
+// with no probing.  sum[kQ31Dense] (8 bytes), cnt[kQ31Dense], then per side
+// a nation's rank (0xFF: not qualifying) and the nation of a rank.
+constexpr uint32_t kQ31Dense = 760;
+struct Q31Groups {
+    uint32_t n_s, n_c, n_groups;
+    bool dense;
+    // the tables, at fixed offsets from the dynamic LDS base
+    static __device__ __forceinline__ unsigned long long* sum(uint8_t* lds) {
+        return reinterpret_cast<unsigned long long*>(lds + kQ31BitmapQueueBytes);
+    }
+    static __device__ __forceinline__ uint32_t* cnt(uint8_t* lds) {
+        return reinterpret_cast<uint32_t*>(lds + kQ31BitmapQueueBytes + 8u * kQ31Dense);
+    }
+    // rank of a nation: side 0 supplier, 1 customer; nation of a rank: 2, 3
+    static __device__ __forceinline__ uint8_t* byte_table(uint8_t* lds, uint32_t which) {
+        return lds + kQ31BitmapQueueBytes + 12u * kQ31Dense + 256u * which;
+    }
+};
+static_assert(kQ31Dense * 12u + 4u * 256u <= Q31Hash::kBytes, "dense groups share the hash table's LDS");
+
 template <int DBW, int CBW, int SBW, int RBW>
 __device__ __forceinline__ void q31_bitmap_flush(const StarDev& a, const uint8_t* s_tab, const uint8_t* y_tab,
-                                                 const Q31Hash& h, const uint32_t* q, uint32_t qn) {
+                                                 const Q31Hash& h, const Q31Groups& gt, uint8_t* lds,
+                                                 const uint32_t* q, uint32_t qn) {
     constexpr int kPer = kQ31Queue / 64;
     constexpr uint32_t kSMask = (1u << SBW) - 1u;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -362,7 +409,7 @@ __device__ __forceinline__ void q31_bitmap_flush(const StarDev& a, const uint8_t
         const uint32_t i = l + 64u * j;
         const uint32_t x = q[i];
         ck[j] = i < qn ? x + (a.cust.base - km) : 0xFFFFFFFFu;  // rows past the queue: out of range, dropped
-        sd[j] = q[kQ31Queue + i];
+        sd[j] = q[kQ31Stride + i];
     }
 #pragma unroll
     for (int j = 0; j < kPer; ++j) cv[j] = code_at(a.c_code, ck[j], 0u, a.n_cust);
@@ -372,8 +419,24 @@ __device__ __forceinline__ void q31_bitmap_flush(const StarDev& a, const uint8_t
         const uint32_t sk = min((sd[j] & kSMask) + (a.supp.base - km), a.n_supp);
         const uint32_t y = code_at(y_tab, (sd[j] >> SBW) + (a.date.base - km), 0u, a.n_date);
         if (cv[j] != kNone && y != kNone) {
-            const uint32_t slot = fast_mod(s_tab[sk] * a.slot_mul_s + cv[j] * a.slot_mul_c + y, a.slot_div);
-            h.add(slot, q[2u * kQ31Queue + i] + a.rev.base, a.sums, a.counts);
+            const uint32_t sn = s_tab[sk];
+            const uint32_t rev = q[2u * kQ31Stride + i] + a.rev.base;
+            if (gt.dense) {
+                const uint32_t sd = Q31Groups::byte_table(lds, 0)[sn], cd = Q31Groups::byte_table(lds, 1)[cv[j]];
+                if (sd < gt.n_s && cd < gt.n_c && y < a.y_span) {
+                    const uint32_t g = (sd * gt.n_c + cd) * a.y_span + y;
+                    atomicAdd(&Q31Groups::sum(lds)[g], (unsigned long long)rev);
+                    atomicAdd(&Q31Groups::cnt(lds)[g], 1u);
+                    continue;
+                }
+            }
+            const uint32_t slot = fast_mod(sn * a.slot_mul_s + cv[j] * a.slot_mul_c + y, a.slot_div);
+            if (gt.dense) {
+                atomicAdd(reinterpret_cast<unsigned long long*>(&a.sums[slot]), (unsigned long long)rev);
+                atomicAdd(reinterpret_cast<unsigned long long*>(&a.counts[slot]), 1ull);
+            } else {
+                h.add(slot, rev, a.sums, a.counts);
+            }
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -381,7 +444,6 @@ __device__ __forceinline__ void q31_bitmap_flush(const StarDev& a, const uint8_t
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr size_t kQ31SupplierLds = kQ31QueueBytes + Q31Hash::kBytes;  // byte offset of the supplier codes
 
 size_t q31_bitmap_lds(uint32_t n_supp, uint32_t n_date, uint32_t c_words) {
     return ((kQ31SupplierLds + n_supp + 1u + n_date + 3u) & ~(size_t)3u) + 4ull * (c_words + 1u);
@@ -392,9 +454,37 @@ __global__ __launch_bounds__(kStarBlock) void q31_bitmap_kernel(const StarDev a)
     static_assert(DBW >= 0 && CBW >= 0 && SBW >= 0 && RBW >= 0 && DBW + SBW <= 32,
                   "fixed widths; supplier and date share a queue word");
     extern __shared__ __align__(16) uint8_t lds[];
-    uint32_t* q = reinterpret_cast<uint32_t*>(lds) + (threadIdx.x >> 6) * 3u * kQ31Queue;
+    uint32_t* q = reinterpret_cast<uint32_t*>(lds) + (threadIdx.x >> 6) * 3u * kQ31Stride;
+    // the qualifying nations (every block reads the same 16 words and takes
+    // the same branch)
+    Q31Groups gt{};
+#pragma unroll
+    for (int w = 0; w < 8; ++w) {
+        gt.n_s += (uint32_t)__popc(a.masks[w]);
+        gt.n_c += (uint32_t)__popc(a.masks[8 + w]);
+    }
+    gt.n_groups = gt.n_s * gt.n_c * a.y_span;
+    gt.dense = a.y_span <= 255u && gt.n_groups <= kQ31Dense;
     Q31Hash h;
-    h.init(lds + kQ31QueueBytes);
+    if (gt.dense) {
+        for (uint32_t i = threadIdx.x; i < gt.n_groups; i += blockDim.x) {
+            Q31Groups::sum(lds)[i] = 0ull;
+            Q31Groups::cnt(lds)[i] = 0u;
+        }
+        if (threadIdx.x < 512u) {
+            const uint32_t side = threadIdx.x >> 8, n = threadIdx.x & 255u;  // side 0: supplier, 1: customer
+            const uint32_t* mk = a.masks + 8u * side;
+            uint32_t rank = 0;
+            for (uint32_t w = 0; w < (n >> 5); ++w) rank += (uint32_t)__popc(mk[w]);
+            const uint32_t mw = mk[n >> 5];
+            rank += (uint32_t)__popc(mw & ((1u << (n & 31u)) - 1u));
+            const bool set = (mw >> (n & 31u)) & 1u;
+            Q31Groups::byte_table(lds, side)[n] = set ? (uint8_t)rank : kNone;
+            if (set) Q31Groups::byte_table(lds, 2u + side)[rank] = (uint8_t)n;
+        }
+    } else {
+        h.init(lds + kQ31BitmapQueueBytes);
+    }
     uint8_t* s_tab = lds + kQ31SupplierLds;
     uint8_t* y_tab = s_tab + a.n_supp + 1u;
     uint32_t* c_tab =
@@ -446,28 +536,57 @@ __global__ __launch_bounds__(kStarBlock) void q31_bitmap_kernel(const StarDev a)
         }
         m &= hit;
         if (!__any(m != 0)) continue;
-#pragma unroll
-        for (int r = 0; r < 32; ++r) {
-            // flush only when this row slot's rows would not fit: the flushes
-            // run nearly full (a chain of round trips each, whatever their size)
-            const bool live = (m >> r) & 1u;
-            const uint64_t bal = __ballot(live);
-            if (qn + (uint32_t)__popcll(bal) > kQ31Queue) {
-                q31_bitmap_flush<DBW, CBW, SBW, RBW>(a, s_tab, y_tab, h, q, qn);
+        // queue slots: the wave's survivors in (lane, row) order, from a wave
+        // prefix sum of the per-lane counts; a dead row slot writes the spare
+        // entry kQ31Queue, so the writes need no branch.  When more rows
+        // survive than a queue holds (never at SSB selectivities) they go two
+        // row slots at a time (at most 128 rows).
+        uint32_t part = 0xFFFFFFFFu;  // row slots of this round (wave-uniform)
+        while (true) {
+            const uint32_t mm = m & part;
+            const uint32_t cnt = (uint32_t)__popc(mm);
+            const uint32_t incl = dev::wave_inclusive_sum(cnt);
+            const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+            if (total > kQ31Queue) {
+                part = 3u;
+                continue;
+            }
+            if (qn + total > kQ31Queue) {
+                q31_bitmap_flush<DBW, CBW, SBW, RBW>(a, s_tab, y_tab, h, gt, lds, q, qn);
                 qn = 0;
             }
-            if (live) {
-                const uint32_t pos =
-                    qn + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            const uint32_t first = qn + incl - cnt;
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                const uint32_t below = (uint32_t)__popc(mm & ((1u << r) - 1u));
+                const uint32_t pos = (mm >> r) & 1u ? first + below : kQ31Queue;
                 q[pos] = dev::extract_at<CBW>(cc.w, r);
-                q[kQ31Queue + pos] = dev::extract_at<SBW>(sc.w, r) | dev::extract_at<DBW>(dc.w, r) << SBW;
-                q[2u * kQ31Queue + pos] = dev::extract_at<RBW>(rc.w, r);
+                q[kQ31Stride + pos] = dev::extract_at<SBW>(sc.w, r) | dev::extract_at<DBW>(dc.w, r) << SBW;
+                q[2u * kQ31Stride + pos] = dev::extract_at<RBW>(rc.w, r);
             }
-            qn += (uint32_t)__popcll(bal);
+            qn += total;
+            m &= ~mm;
+            if (part == 0xFFFFFFFFu || part == 0xC0000000u) break;
+            part <<= 2;
         }
     }
-    if (qn) q31_bitmap_flush<DBW, CBW, SBW, RBW>(a, s_tab, y_tab, h, q, qn);
-    h.flush(a.sums, a.counts);
+    if (qn) q31_bitmap_flush<DBW, CBW, SBW, RBW>(a, s_tab, y_tab, h, gt, lds, q, qn);
+    if (gt.dense) {
+        __syncthreads();
+        for (uint32_t g = threadIdx.x; g < gt.n_groups; g += blockDim.x) {
+            const uint32_t c = Q31Groups::cnt(lds)[g];
+            if (c) {
+                const uint32_t y = g % a.y_span, sc = g / a.y_span;
+                const uint32_t slot = fast_mod(Q31Groups::byte_table(lds, 2)[sc / gt.n_c] * a.slot_mul_s +
+                                                   Q31Groups::byte_table(lds, 3)[sc % gt.n_c] * a.slot_mul_c + y,
+                                               a.slot_div);
+                atomicAdd(reinterpret_cast<unsigned long long*>(&a.sums[slot]), Q31Groups::sum(lds)[g]);
+                atomicAdd(reinterpret_cast<unsigned long long*>(&a.counts[slot]), (unsigned long long)c);
+            }
+        }
+    } else {
+        h.flush(a.sums, a.counts);
+    }
 }
 
 // Q3.1 probe over columns of any encoding and width: one column decoded at a
@@ -721,6 +840,7 @@ int star_workspace(StarDev& a, const fls_star_dims* dims, const void* d_ws) {
     a.p_bits = reinterpret_cast<const uint32_t*>(ws + L.p_bits);
     a.c_bits = reinterpret_cast<const uint32_t*>(ws + L.c_bits);
     a.s_bits = reinterpret_cast<const uint32_t*>(ws + L.s_bits);
+    a.masks = reinterpret_cast<const uint32_t*>(ws + L.masks);
     a.y_code = reinterpret_cast<const uint8_t*>(ws + L.y_code);
     a.n_supp = dims->n_supp;
     a.n_cust = dims->n_cust;
@@ -774,9 +894,11 @@ int fls_q31_build(const fls_star_dims* dims, const fls_q31_params* params, void*
     uint8_t* ws = static_cast<uint8_t*>(d_workspace);
     hipStream_t s = as_stream(stream);
     uint32_t* c_bits = reinterpret_cast<uint32_t*>(ws + L.c_bits);
+    uint32_t* masks = reinterpret_cast<uint32_t*>(ws + L.masks);
     hipLaunchKernelGGL(star_zero_words, dim3(256), dim3(256), 0, s, c_bits, L.c_words);
+    hipLaunchKernelGGL(star_zero_words, dim3(1), dim3(64), 0, s, masks, 16u);
     hipLaunchKernelGGL(q31_build_kernel, dim3(1024), dim3(256), 0, s, *dims, *params, ws + L.s_code, ws + L.c_code,
-                       ws + L.y_code, c_bits);
+                       ws + L.y_code, c_bits, masks);
     return check_launch();
 }
 
@@ -796,6 +918,7 @@ int fls_query_q31(const fls_column* datekey, const fls_column* custkey, const fl
     star_workspace(a, dims, d_workspace);
     a.slot_mul_s = params->n_nations * params->n_years;
     a.slot_mul_c = params->n_years;
+    a.y_span = params->year_hi - params->year_base + 1u;
     const size_t table = 12ull * a.n_slots;
     const size_t staged = table + dims->n_supp + dims->n_date;
     const bool fast = all_bitpack(outs, 4) && a.date.bw == 12 && a.cust.bw == 20 && a.supp.bw == 16 &&

@@ -25,5 +25,19 @@ __device__ __forceinline__ T half_sum(T x) {
 
 __device__ __forceinline__ bool is_wave_leader() { return (threadIdx.x & 63u) == 0u; }
 
+// Inclusive prefix sum over the 64 work-items of the wave (work-item i gets
+// x_0 + ... + x_i), with DPP: shifts by 1, 2, 4 and 8 within each 16-lane row
+// (lanes shifted in from outside the row read 0), then lane 15 of a row added
+// to rows 1 and 3, then lane 31 to rows 2 and 3.
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true);  // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true);  // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true);  // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true);  // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return x;
+}
+
 }  // namespace dev
 }  // namespace fls

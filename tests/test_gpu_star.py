@@ -180,8 +180,9 @@ def test_keys_beyond_the_dimension_never_match(col, widths):
 
 def test_q31_more_groups_than_the_block_hash():
     # every supplier and customer in the filtered region: 25 x 25 nations x 6
-    # years = 3,750 live groups, beyond the probe's 512-entry LDS hash table,
-    # so most groups take the global-atomic path
+    # years = 3,750 live groups, beyond the probe's dense group table (760
+    # groups) and its 512-entry LDS hash table, so the probe runs the hash
+    # and most groups take the global-atomic path
     n = 200_000 + 3
     f, d = small_table(36, n)
     d["s_region"][:] = 3
