@@ -214,6 +214,15 @@ struct LaneCol {
     __device__ __forceinline__ void pin() { pin_words(w); }
 };
 
+// The column with its words pointer hidden from loop-invariant code motion.
+// Loaded inside a loop, a kernel near its register limit otherwise keeps
+// words + lane of every column in a VGPR pair across the loop and spills
+// them; recomputing the address costs one instruction per column and step.
+__device__ __forceinline__ ColDev opaque_words(ColDev c) {
+    asm volatile("" : "+s"(c.words));
+    return c;
+}
+
 // Row r of lane `lane` of vector v of a fixed-width bit-packed column, read
 // from memory: the word that holds its first bit and, when the value crosses
 // a word boundary, the next one (a late read for the few rows that survive a

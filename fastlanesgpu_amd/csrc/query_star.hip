@@ -419,7 +419,7 @@ __device__ __forceinline__ void q31_bitmap_flush(const StarDev& a, const uint8_t
         const uint32_t sk = min((sd[j] & kSMask) + (a.supp.base - km), a.n_supp);
         const uint32_t y = code_at(y_tab, (sd[j] >> SBW) + (a.date.base - km), 0u, a.n_date);
         if (cv[j] != kNone && y != kNone) {
-            const uint32_t sn = s_tab[sk];
+            const uint32_t sn = s_tab[sk] - 1u;  // staged as nation + 1
             const uint32_t rev = q[2u * kQ31Stride + i] + a.rev.base;
             if (gt.dense) {
                 const uint32_t sd = Q31Groups::byte_table(lds, 0)[sn], cd = Q31Groups::byte_table(lds, 1)[cv[j]];
@@ -489,7 +489,10 @@ __global__ __launch_bounds__(kStarBlock) void q31_bitmap_kernel(const StarDev a)
     uint8_t* y_tab = s_tab + a.n_supp + 1u;
     uint32_t* c_tab =
         reinterpret_cast<uint32_t*>(lds + ((kQ31SupplierLds + a.n_supp + 1u + a.n_date + 3u) & ~(size_t)3u));
-    for (uint32_t i = threadIdx.x; i <= a.n_supp; i += blockDim.x) s_tab[i] = i < a.n_supp ? a.s_code[i] : kNone;
+    // supplier codes staged as nation + 1 (0: not qualifying, and the
+    // sentinel), so a lookup's hit bit is min(code, 1)
+    for (uint32_t i = threadIdx.x; i <= a.n_supp; i += blockDim.x)
+        s_tab[i] = i < a.n_supp ? (uint8_t)(a.s_code[i] + 1u) : (uint8_t)0;
     for (uint32_t i = threadIdx.x; i < a.n_date; i += blockDim.x) y_tab[i] = a.y_code[i];
     for (uint32_t i = threadIdx.x; i <= a.c_words; i += blockDim.x) c_tab[i] = i < a.c_words ? a.c_bits[i] : 0u;
     __syncthreads();
@@ -509,18 +512,22 @@ __global__ __launch_bounds__(kStarBlock) void q31_bitmap_kernel(const StarDev a)
         LaneCol<CBW> cc;
         LaneCol<DBW> dc;
         LaneCol<RBW> rc;
-        sc.load(a.supp, vl);
-        cc.load(a.cust, vl);
-        dc.load(a.date, vl);
-        rc.load(a.rev, vl);
-        // supplier semi-join on every row: a miss is the kNone code
-        uint32_t miss = 0;
+        sc.load(opaque_words(a.supp), vl);
+        cc.load(opaque_words(a.cust), vl);
+        dc.load(opaque_words(a.date), vl);
+        rc.load(opaque_words(a.rev), vl);
+        // supplier semi-join on every row
+        uint32_t m = 0;
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
             const uint32_t k = min(dev::extract_at<SBW>(sc.w, r) + d_s, a.n_supp);
-            miss |= (((uint32_t)s_tab[k] + 1u) >> 8) << r;
+            // the hit bit as one opaque min (the compiler turns min(code, 1)
+            // into a compare and a select)
+            uint32_t bit;
+            asm("v_min_u32 %0, 1, %1" : "=v"(bit) : "v"((uint32_t)s_tab[k]));
+            m |= bit << r;
         }
-        uint32_t m = vlive ? ~miss : 0u;
+        if (!vlive) m = 0;
         if (v == a.n_vec - 1) m &= tail;
         cc.pin();
         dc.pin();
@@ -532,7 +539,12 @@ __global__ __launch_bounds__(kStarBlock) void q31_bitmap_kernel(const StarDev a)
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
             const uint32_t k = min(dev::extract_at<CBW>(cc.w, r) + d_c, a.n_cust);
-            hit |= ((c_tab[k >> 5] >> (k & 31u)) & 1u) << r;
+            // the word index k >> 5 from an opaque shift, so the address is
+            // one shift-add (the compiler's (k >> 3) & ~3 form costs two
+            // more); v_bfe_u32 reads the bit offset from k's low five bits
+            uint32_t word;
+            asm("v_lshrrev_b32 %0, 5, %1" : "=v"(word) : "v"(k));
+            hit |= __builtin_amdgcn_ubfe(c_tab[word], k, 1u) << r;
         }
         m &= hit;
         if (!__any(m != 0)) continue;
