@@ -121,7 +121,22 @@ def unpack_global(d_in, d_out, n_vec, bw, stream=None):
     call("fls_unpack_u32", dptr(d_in), dptr(d_out), n_vec, bw, _stream(stream))
 
 
+CHECKSUM_WORDS = 1056  # FLS_CHECKSUM_WORDS: the result and the in-launch reduction's state
+
+
+def checksum_buffer():
+    """A zeroed device buffer for unpack_checksum (the result lands in word 0;
+    the rest is the reduction's state, which every call leaves zero)."""
+    import torch
+
+    return torch.zeros(CHECKSUM_WORDS, dtype=torch.int64, device="cuda")
+
+
 def unpack_checksum(d_in, d_out, n_values, bw, d_checksum, stream=None):
+    """Decode + checksum into d_checksum[0]; d_checksum is a checksum_buffer()
+    (CHECKSUM_WORDS 64-bit words, state zero before the first call)."""
+    if not isinstance(d_checksum, int) and d_checksum.numel() * d_checksum.element_size() < 8 * CHECKSUM_WORDS:
+        raise ValueError(f"d_checksum must hold {CHECKSUM_WORDS} 64-bit words (use checksum_buffer())")
     call("fls_unpack_u32_checksum", dptr(d_in), dptr(d_out), n_values, bw, dptr(d_checksum), _stream(stream))
 
 

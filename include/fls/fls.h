@@ -143,10 +143,16 @@ int fls_rle_pack_device(const uint32_t* d_in, uint64_t n, const fls_vmeta* d_vme
  * launched <<<n_vec, 32>>> (unpack.cuh:3436-3442, unpack_fused.cuh:3444-3453). */
 int fls_unpack_u32(const uint32_t* d_in, uint32_t* d_out, uint64_t n_vec, uint32_t bw, void* stream);
 /* Same decode that also produces the order-sensitive checksum
- * sum_i (i+1)*value_i mod 2^64 over the first n_values values into *d_checksum
- * (overwritten).  d_out may be NULL to decode without materialising; when
- * given, it receives whole vectors and must hold ceil(n_values/1024)*1024
- * words (the padding rows of the last vector are written too).              */
+ * sum_i (i+1)*value_i mod 2^64 over the first n_values values into
+ * d_checksum[0] (overwritten), in one launch.  d_checksum points to
+ * FLS_CHECKSUM_WORDS uint64 words: words 1.. hold the in-launch reduction's
+ * state, which must be zero before the first call on the buffer (allocate it
+ * zeroed once) and which every completed call leaves zero again.  Calls that
+ * overlap in time need buffers of their own.  d_out may be NULL to decode
+ * without materialising; when given, it receives whole vectors and must hold
+ * ceil(n_values/1024)*1024 words (the padding rows of the last vector are
+ * written too).                                                              */
+#define FLS_CHECKSUM_WORDS 1056u
 int fls_unpack_u32_checksum(const uint32_t* d_in, uint32_t* d_out, uint64_t n_values, uint32_t bw,
                             uint64_t* d_checksum, void* stream);
 /* FastLanes delta decode: unpack at each vector's bw + d_rsum_32 with the

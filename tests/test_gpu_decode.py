@@ -41,7 +41,7 @@ def test_c1_full_size_values_and_checksum():
     words = fls.pack_column(x, bw)
     d_in = T.to_dev(words)
     d_out = T.dev_empty(n)
-    d_sum = T.dev_empty(1, "u64")
+    d_sum = fls.checksum_buffer()
     fls.unpack_checksum(d_in, d_out, n, bw, d_sum)
     np.testing.assert_array_equal(T.from_dev(d_out), x)
     want = T.o_checksum(x)
@@ -59,10 +59,27 @@ def test_checksum_ragged(n, bw):
     d_in = T.to_dev(words if words.size else np.zeros(1, np.uint32))
     n_vec = (n + 1023) // 1024
     d_out = T.dev_empty(n_vec * 1024)
-    d_sum = T.dev_empty(1, "u64")
+    d_sum = fls.checksum_buffer()
     fls.unpack_checksum(d_in, d_out, n, bw, d_sum)
     np.testing.assert_array_equal(T.from_dev(d_out)[:n], x)
     assert int(T.from_dev(d_sum, np.uint64)[0]) == T.o_checksum(x)
+
+
+def test_checksum_buffer_reused_across_sizes():
+    # one buffer, calls of shrinking and growing grids (fewer blocks than the
+    # reduction's 64 groups, exactly 64, and many more): every result is
+    # right and the reduction state is zero again after each call
+    d_sum = fls.checksum_buffer()
+    for n, bw in ((3 * 8 * 1024 + 5, 9), (64 * 8 * 1024, 13), (1, 4), (700 * 8 * 1024 + 999, 11), (8 * 1024, 32)):
+        x = RNG.integers(0, 2**bw, n, dtype=np.uint64).astype(np.uint32)
+        d_in = T.to_dev(fls.pack_column(x, bw))
+        n_vec = (n + 1023) // 1024
+        fls.unpack_checksum(d_in, T.dev_empty(n_vec * 1024), n, bw, d_sum)
+        got = T.from_dev(d_sum, np.uint64)
+        assert int(got[0]) == T.o_checksum(x), (n, bw)
+        assert not got[1:].any(), (n, bw)
+    with pytest.raises(ValueError, match="checksum_buffer"):
+        fls.unpack_checksum(d_in, None, n, bw, T.dev_empty(1, "u64"))
 
 
 @pytest.mark.parametrize("kind", ["sorted", "random", "dates", "constant"])
