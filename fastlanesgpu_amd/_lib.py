@@ -212,6 +212,7 @@ SIGNATURES.update({
     "fls_trial_workspace_bytes": (c_u64, [c_u64]),
     "fls_mixed_encode_trial": (c_int, [c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "fls_mixed_pack_device": (c_int, [c_vp, c_u64, c_vp, c_vp, c_vp, c_vp]),
+    "fls_mixed_encode_device": (c_int, [c_vp, c_u64, c_vp, c_vp, ctypes.c_float, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "fls_mixed_decode_u32": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp, c_vp]),
     "fls_gen_train_vectors": (c_int, [c_u64, c_u64, c_u64, c_vp, c_vp, c_vp]),
     "fls_mlp_param_count": (c_u32, []),
@@ -222,6 +223,7 @@ SIGNATURES.update({
     "fls_mlp_adam": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                              ctypes.c_float, c_vp]),
     "fls_mlp_mfma_selftest": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "fls_mlp_mfma32_selftest": (c_int, [c_vp, c_vp, c_vp, c_vp]),
 })
 
 

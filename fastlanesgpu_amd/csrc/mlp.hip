@@ -1,7 +1,9 @@
 // The learned encoding selector: an MLP 16 -> 64 -> 64 -> 40 (ReLU hidden
-// layers, softmax over the 40 classes), BF16 operands on the CDNA matrix
-// cores (v_mfma_f32_16x16x16_bf16) with FP32 accumulation, trained on the GPU
-// by its own forward/backward kernels and an FP32 Adam step.
+// layers, softmax over the 40 classes), BF16 operands on the CDNA4 matrix
+// cores with FP32 accumulation -- v_mfma_f32_16x16x32_bf16 for the products
+// whose inner dimension is 32 or more, v_mfma_f32_16x16x16_bf16 for the
+// 16-deep ones (the 16 input features, the 16 samples of a tile) -- trained
+// on the GPU by its own forward/backward kernels and an FP32 Adam step.
 //
 // Parameters (FP32 master copy, and a BF16 copy the kernels read), in this
 // order: W1[16][64], b1[64], W2[64][64], b2[64], W3[64][40], b3[40]
@@ -12,11 +14,11 @@
 // tiles, so every product -- including the transposed ones of the backward
 // pass -- reads its operands with plain strides.
 //
-// MFMA 16x16x16 operand layout used here (lane L of the wave, item i < 4):
-//   A (16 x 16, M x K): A[L % 16][4*(L/16) + i]
-//   B (16 x 16, K x N): B[4*(L/16) + i][L % 16]
-//   D (16 x 16):        D[4*(L/16) + i][L % 16]
-// fls_mlp_mfma_selftest checks it against a host matmul.
+// MFMA operand layouts used here (lane L of the wave):
+//   16x16x16: A[L % 16][4*(L/16) + i], B[4*(L/16) + i][L % 16], i < 4
+//   16x16x32: A[L % 16][8*(L/16) + i], B[8*(L/16) + i][L % 16], i < 8
+//   D (16 x 16, both): D[4*(L/16) + i][L % 16], i < 4
+// fls_mlp_mfma_selftest checks both against a host matmul.
 #include <math.h>
 
 #include "fls/device/wave.hpp"
@@ -26,6 +28,8 @@ namespace fls {
 namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kIn = 16, kH = 64, kOut = 40, kOutPad = 48;
@@ -65,13 +69,38 @@ __device__ __forceinline__ bf16x4 frag_b(const View& B, int k0, int n0) {
     const int L = lane64(), n = n0 + (L & 15), k = k0 + 4 * (L >> 4);
     return bf16x4{(short)B.at(k, n), (short)B.at(k + 1, n), (short)B.at(k + 2, n), (short)B.at(k + 3, n)};
 }
+__device__ __forceinline__ s16x8 frag_a8(const View& A, int m0, int k0) {
+    const int L = lane64(), m = m0 + (L & 15), k = k0 + 8 * (L >> 4);
+    s16x8 f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) f[i] = (short)A.at(m, k + i);
+    return f;
+}
+__device__ __forceinline__ s16x8 frag_b8(const View& B, int k0, int n0) {
+    const int L = lane64(), n = n0 + (L & 15), k = k0 + 8 * (L >> 4);
+    s16x8 f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) f[i] = (short)B.at(k + i, n);
+    return f;
+}
 __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
-// C tile (m0, n0) of A[M x K] * B[K x N], K a multiple of 16
+__device__ __forceinline__ f32x4 mfma32(s16x8 a, s16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+}
+// C tile (m0, n0) of A[M x K] * B[K x N], K a multiple of 16: 32-deep steps
+// when K is a multiple of 32, else 16-deep ones.  One product never mixes the
+// two shapes on an accumulator: a 16x16x32 step followed by a 16x16x16 one
+// (K = 48) gave wrong sums in the gradient test on gfx950.
 __device__ __forceinline__ f32x4 tile_mm(const View& A, const View& B, int m0, int n0, int K) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 16) acc = mfma(frag_a(A, m0, k0), frag_b(B, k0, n0), acc);
+    if (K % 32 == 0) {
+        for (int k0 = 0; k0 < K; k0 += 32) acc = mfma32(frag_a8(A, m0, k0), frag_b8(B, k0, n0), acc);
+    } else {
+        for (int k0 = 0; k0 < K; k0 += 16) acc = mfma(frag_a(A, m0, k0), frag_b(B, k0, n0), acc);
+    }
     return acc;
 }
 // row of item i of the D tile held by this lane, and its column
@@ -92,7 +121,7 @@ struct WaveTiles {
 struct MlpLds {
     uint16_t w1[kIn * kH], w2[kH * kH], w3[kH * kOutPad];
     float b1[kH], b2[kH], b3[kOutPad];
-    WaveTiles t[4];
+    WaveTiles t[1];  // the gradient kernel runs one wave per block
 };
 
 __device__ void stage_weights(const uint16_t* __restrict wb, MlpLds& s) {
@@ -155,9 +184,10 @@ __device__ void forward_tile(const float* __restrict feat, uint64_t s0, uint64_t
 // Inference keeps its own LDS layout: every operand is stored so that an MFMA
 // fragment (4 consecutive k of one row m, or of one column n) is one aligned
 // 8-byte LDS read -- activations row-major, weights transposed (n-major) --
-// and rows are padded by 4 elements so the 16 rows a fragment load touches
-// fall in different banks.
-constexpr int kLdX = kIn + 4, kLdH = kH + 4;
+// and rows are padded (by 4 elements for the 16-deep input tile, by 8 for the
+// hidden tiles so 8-element fragments stay 16-byte aligned) so the 16 rows a
+// fragment load touches fall in different banks.
+constexpr int kLdX = kIn + 4, kLdH = kH + 8;
 
 struct alignas(16) InferWave {
     uint16_t x[16 * kLdX];
@@ -199,13 +229,27 @@ __device__ __forceinline__ bf16x4 frag_row(const uint16_t* __restrict a, int ld,
     const int L = lane64();
     return *reinterpret_cast<const bf16x4*>(a + (m0 + (L & 15)) * ld + k0 + 4 * (L >> 4));
 }
+__device__ __forceinline__ s16x8 frag_row8(const uint16_t* __restrict a, int ld, int m0, int k0) {
+    const int L = lane64();
+    return *reinterpret_cast<const s16x8*>(a + (m0 + (L & 15)) * ld + k0 + 8 * (L >> 4));
+}
+// C tile (0, n0) of A * W with A row-major and W stored transposed (n-major)
+__device__ __forceinline__ f32x4 infer_tile(const uint16_t* __restrict a, int lda, const uint16_t* __restrict wt,
+                                            int ldw, int K, int n0) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    if (K % 32 == 0) {
+        for (int k0 = 0; k0 < K; k0 += 32) acc = mfma32(frag_row8(a, lda, 0, k0), frag_row8(wt, ldw, n0, k0), acc);
+    } else {
+        for (int k0 = 0; k0 < K; k0 += 16) acc = mfma(frag_row(a, lda, 0, k0), frag_row(wt, ldw, n0, k0), acc);
+    }
+    return acc;
+}
 
 __device__ __forceinline__ void infer_layer(const uint16_t* __restrict a, int lda, const uint16_t* __restrict wt,
                                             int ldw, int K, int N, const float* __restrict bias,
                                             uint16_t* __restrict h, int ldh) {
     for (int n0 = 0; n0 < N; n0 += 16) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        for (int k0 = 0; k0 < K; k0 += 16) acc = mfma(frag_row(a, lda, 0, k0), frag_row(wt, ldw, n0, k0), acc);
+        const f32x4 acc = infer_tile(a, lda, wt, ldw, K, n0);
         for (int i = 0; i < 4; ++i) {
             const float v = acc[i] + bias[n0 + d_col()];
             h[d_row(i) * ldh + n0 + d_col()] = to_bf16(v > 0.f ? v : 0.f);
@@ -241,8 +285,7 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const uint16_t* __rest
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         for (int n0 = 0; n0 < kOutPad; n0 += 16) {
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            for (int k0 = 0; k0 < kH; k0 += 16) acc = mfma(frag_row(t.h2, kLdH, 0, k0), frag_row(s.w3t, kLdH, n0, k0), acc);
+            const f32x4 acc = infer_tile(t.h2, kLdH, s.w3t, kLdH, kH, n0);
             for (int i = 0; i < 4; ++i) t.z[d_row(i) * kOutPad + n0 + d_col()] = acc[i] + s.b3[n0 + d_col()];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -288,17 +331,21 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const uint16_t* __rest
 // Softmax cross-entropy, scaled by inv_batch.  Every wave accumulates the
 // gradients of its tiles in registers and writes them, with its summed loss,
 // to its own row of `part` (kPart floats); grad_reduce_kernel then adds the
-// rows in wave order, so a step's gradient is bit-reproducible.
+// rows in wave order, so a step's gradient is bit-reproducible.  One wave per
+// block and one 16-sample tile per wave up to kGradMaxWaves waves: a 4,096-
+// sample batch is 256 blocks, one per CU (the kernel's chain of dependent
+// products and LDS round trips is latency-bound, so spreading the tiles over
+// the CUs is what shortens it).
 constexpr int kPart = kParams + 8;  // [kParams] = loss, rest padding
-constexpr int kGradMaxBlocks = 64;
+constexpr int kGradMaxWaves = 1024;
 
-__global__ __launch_bounds__(256) void mlp_grad_kernel(const uint16_t* __restrict wb, const float* __restrict feat,
+__global__ __launch_bounds__(64) void mlp_grad_kernel(const uint16_t* __restrict wb, const float* __restrict feat,
                                                        const uint8_t* __restrict label, uint64_t n, float inv_batch,
                                                        float* __restrict part) {
     __shared__ MlpLds s;
     stage_weights(wb, s);
     __syncthreads();
-    WaveTiles& t = s.t[threadIdx.x >> 6];
+    WaveTiles& t = s.t[0];
     const int L = lane64();
     // register accumulators of this wave's weight gradients (D-tile layout)
     f32x4 gw1[4], gw2[16], gw3[12];
@@ -307,8 +354,7 @@ __global__ __launch_bounds__(256) void mlp_grad_kernel(const uint16_t* __restric
     for (int i = 0; i < 12; ++i) gw3[i] = f32x4{0.f, 0.f, 0.f, 0.f};
     float gb1 = 0.f, gb2 = 0.f, gb3 = 0.f, lsum = 0.f;  // lane L owns column L of each bias
     const uint64_t n_tiles = ceil_div(n, 16);
-    const uint64_t waves = (uint64_t)gridDim.x * 4;
-    for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < n_tiles; tile += waves) {
+    for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
         const uint64_t s0 = tile * 16;
         forward_tile(feat, s0, n, s, t);
         // softmax + dZ = (p - onehot) * inv_batch, rows past n contribute nothing
@@ -384,7 +430,7 @@ __global__ __launch_bounds__(256) void mlp_grad_kernel(const uint16_t* __restric
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     // this wave's row of partial sums: every parameter written exactly once
-    float* p = part + (uint64_t)(blockIdx.x * 4 + (threadIdx.x >> 6)) * kPart;
+    float* p = part + (uint64_t)blockIdx.x * kPart;
     for (int mi = 0; mi < 4; ++mi)
         for (int ni = 0; ni < 3; ++ni)
             for (int i = 0; i < 4; ++i) {
@@ -427,8 +473,7 @@ __global__ __launch_bounds__(256) void grad_reduce_kernel(const float* __restric
 
 uint32_t grad_blocks(uint64_t n) {
     const uint64_t tiles = ceil_div(n, 16);
-    const uint64_t g = ceil_div(tiles, 8);  // about two tiles per wave before the flush
-    return (uint32_t)(g < 1 ? 1 : g > kGradMaxBlocks ? kGradMaxBlocks : g);
+    return (uint32_t)(tiles < 1 ? 1 : tiles > kGradMaxWaves ? kGradMaxWaves : tiles);
 }
 
 // Adam on the FP32 master weights; refreshes the BF16 copy and zeroes grad
@@ -455,6 +500,14 @@ __global__ void to_bf16_kernel(const float* __restrict w, uint16_t* __restrict w
 __global__ void mfma_selftest_kernel(const uint16_t* __restrict a, const uint16_t* __restrict b, float* __restrict d) {
     const View A{a, 16, 1}, B{b, 16, 1};
     const f32x4 acc = tile_mm(A, B, 0, 0, 16);
+    for (int i = 0; i < 4; ++i) d[d_row(i) * 16 + d_col()] = acc[i];
+}
+
+// one 16x16x32 tile (A 16 x 32, B 32 x 16, row-major) through the K = 32 path
+__global__ void mfma32_selftest_kernel(const uint16_t* __restrict a, const uint16_t* __restrict b,
+                                       float* __restrict d) {
+    const View A{a, 32, 1}, B{b, 16, 1};
+    const f32x4 acc = tile_mm(A, B, 0, 0, 32);
     for (int i = 0; i < 4; ++i) d[d_row(i) * 16 + d_col()] = acc[i];
 }
 
@@ -488,7 +541,7 @@ int fls_mlp_forward(const uint16_t* d_wb, const float* d_feat, uint64_t n, uint8
 }
 
 uint64_t fls_mlp_grad_workspace_bytes(uint64_t n) {
-    return (uint64_t)fls::grad_blocks(n) * 4u * fls::kPart * sizeof(float);
+    return (uint64_t)fls::grad_blocks(n) * fls::kPart * sizeof(float);
 }
 
 int fls_mlp_grad(const uint16_t* d_wb, const float* d_feat, const uint8_t* d_label, uint64_t n, float inv_batch,
@@ -498,11 +551,11 @@ int fls_mlp_grad(const uint16_t* d_wb, const float* d_feat, const uint8_t* d_lab
     FLS_RETURN_IF(!d_wb || !d_feat || !d_label || !d_work || !d_grad, FLS_ERR_NULL);
     const uint32_t blocks = grad_blocks(n);
     float* part = static_cast<float*>(d_work);
-    hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks), dim3(256), 0, as_stream(stream), d_wb, d_feat, d_label, n,
+    hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks), dim3(64), 0, as_stream(stream), d_wb, d_feat, d_label, n,
                        inv_batch, part);
     if (const int rc = check_launch()) return rc;
     hipLaunchKernelGGL(grad_reduce_kernel, dim3((uint32_t)ceil_div(kParams + 1, 16)), dim3(256), 0,
-                       as_stream(stream), part, blocks * 4u, d_grad, d_loss);
+                       as_stream(stream), part, blocks, d_grad, d_loss);
     return check_launch();
 }
 
@@ -520,6 +573,12 @@ int fls_mlp_adam(float* d_w, float* d_m, float* d_v, float* d_grad, uint16_t* d_
 int fls_mlp_mfma_selftest(const uint16_t* d_a, const uint16_t* d_b, float* d_d, void* stream) {
     if (!d_a || !d_b || !d_d) return FLS_ERR_NULL;
     hipLaunchKernelGGL(fls::mfma_selftest_kernel, dim3(1), dim3(64), 0, fls::as_stream(stream), d_a, d_b, d_d);
+    return fls::check_launch();
+}
+
+int fls_mlp_mfma32_selftest(const uint16_t* d_a, const uint16_t* d_b, float* d_d, void* stream) {
+    if (!d_a || !d_b || !d_d) return FLS_ERR_NULL;
+    hipLaunchKernelGGL(fls::mfma32_selftest_kernel, dim3(1), dim3(64), 0, fls::as_stream(stream), d_a, d_b, d_d);
     return fls::check_launch();
 }
 

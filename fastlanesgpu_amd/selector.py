@@ -241,8 +241,20 @@ def mixed_pack(d_in, n, ws, stream=None):
     call("fls_mixed_pack_device", dptr(d_in), n, dptr(ws.vmeta), dptr(ws.aux), dptr(ws.words), _stream(stream))
 
 
+def mixed_encode(d_in, n, ws, conf=None, conf_min=0.0, stream=None):
+    """Single-pass encode with the classes in ws.cls (and the confidence
+    fallback when conf is given): plan, look-back offsets and pack in one
+    launch after a small fill.  The same column as plan + pack; slower at
+    120M rows (see mixed_encode_classes), kept for callers that want one
+    launch."""
+    call("fls_mixed_encode_device", dptr(d_in), n, dptr(ws.cls), dptr(conf), ctypes.c_float(conf_min),
+         dptr(ws.vmeta), dptr(ws.aux), dptr(ws.words), dptr(ws.total), dptr(ws.fallbacks), dptr(ws.scratch),
+         _stream(stream))
+
+
 def mixed_encode_classes(d_in, n, ws, stream=None):
-    """Encode with the classes already in ws.cls (plan + pack, no host sync)."""
+    """Encode with the classes already in ws.cls: plan + pack (no host sync),
+    which at 120M rows measured faster than the single-pass mixed_encode."""
     mixed_plan(d_in, n, ws, stream=stream)
     mixed_pack(d_in, n, ws, stream)
 

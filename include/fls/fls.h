@@ -502,6 +502,15 @@ int fls_mixed_plan_device(const uint32_t* d_in, uint64_t n, uint8_t* d_class, co
                           void* d_scratch, void* stream);
 int fls_mixed_pack_device(const uint32_t* d_in, uint64_t n, const fls_vmeta* d_vmeta, const uint64_t* d_aux,
                           uint32_t* d_out, void* stream);
+/* The same mixed column in one pass over the input: plan, offsets (a
+ * decoupled look-back over the blocks, in the scratch of
+ * fls_encode_scratch_bytes(n), cleared by a small fill launch first) and pack
+ * in one launch.  Arguments and results as plan + pack above; d_out receives
+ * the words.  Calls sharing one scratch must not overlap in time.  At 120M
+ * rows plan + pack is the faster of the two on MI355X.                     */
+int fls_mixed_encode_device(const uint32_t* d_in, uint64_t n, uint8_t* d_class, const float* d_conf, float conf_min,
+                            fls_vmeta* d_vmeta, uint64_t* d_aux, uint32_t* d_out, uint64_t* d_total_words,
+                            uint32_t* d_fallbacks, void* d_scratch, void* stream);
 /* Try-all-encodings search: every vector is encoded with each of the four
  * encodings (four full plan + pack passes into candidate columns held in
  * d_work, fls_trial_workspace_bytes(n) bytes), then the smallest candidate of
@@ -532,7 +541,9 @@ int fls_mixed_decode_u32(const uint32_t* d_words, const fls_vmeta* d_vmeta, cons
  *   adam: one Adam step (step >= 1) on the FP32 weights; refreshes d_wb and
  *         zeroes d_grad.
  *   mfma_selftest: D = A * B for one 16x16x16 BF16 tile (row-major) through
- *         the kernels' own operand path, to pin the matrix-core lane layout. */
+ *         the kernels' own operand path, to pin the matrix-core lane layout;
+ *         mfma32_selftest likewise for A 16x32, B 32x16 through the
+ *         16x16x32 instruction's path.                                     */
 uint32_t fls_mlp_param_count(void);
 int fls_mlp_to_bf16(const float* d_w, uint16_t* d_wb, void* stream);
 int fls_mlp_forward(const uint16_t* d_wb, const float* d_feat, uint64_t n, uint8_t* d_class, float* d_conf,
@@ -543,6 +554,7 @@ int fls_mlp_grad(const uint16_t* d_wb, const float* d_feat, const uint8_t* d_lab
 int fls_mlp_adam(float* d_w, float* d_m, float* d_v, float* d_grad, uint16_t* d_wb, uint32_t step, float lr, float beta1,
                  float beta2, float eps, void* stream);
 int fls_mlp_mfma_selftest(const uint16_t* d_a, const uint16_t* d_b, float* d_d, void* stream);
+int fls_mlp_mfma32_selftest(const uint16_t* d_a, const uint16_t* d_b, float* d_d, void* stream);
 /* Training vectors: n_vec x 1024 values from the four families of the
  * learned selector's workload (uniform, Zipf s=1.1 via d_zipf_cdf[4096], the
  * CDF of ranks 1..4096 scaled to 2^32, sorted + Gaussian noise sigma 16, runs

@@ -51,6 +51,28 @@ def test_mfma_layout_random():
     np.testing.assert_allclose(dd.cpu().numpy().reshape(16, 16), ref, rtol=1e-5, atol=1e-5)
 
 
+@pytest.mark.parametrize("ints", [True, False])
+def test_mfma32_layout(ints):
+    # the 16x16x32 instruction's operand path: A 16 x 32, B 32 x 16
+    torch = torch_cuda()
+    rng = np.random.default_rng(2)
+    if ints:
+        a = rng.integers(-8, 9, (16, 32)).astype(np.float32)
+        b = rng.integers(-8, 9, (32, 16)).astype(np.float32)
+    else:
+        a = bf16_val(bf16_bits(rng.normal(size=(16, 32))))
+        b = bf16_val(bf16_bits(rng.normal(size=(32, 16))))
+    da = torch.from_numpy(bf16_bits(a).view(np.int16)).cuda()
+    db = torch.from_numpy(bf16_bits(b).view(np.int16)).cuda()
+    dd = torch.zeros(256, dtype=torch.float32, device="cuda")
+    fl.call("fls_mlp_mfma32_selftest", da.data_ptr(), db.data_ptr(), dd.data_ptr(), None)
+    ref = a.astype(np.float64) @ b.astype(np.float64)
+    if ints:
+        np.testing.assert_array_equal(dd.cpu().numpy().reshape(16, 16), ref)
+    else:
+        np.testing.assert_allclose(dd.cpu().numpy().reshape(16, 16), ref, rtol=1e-5, atol=1e-5)
+
+
 # ------------------------------------------------------------- inputs
 def train_column(seed, first, n_vec):
     torch_cuda()
@@ -151,6 +173,12 @@ def test_mixed_encode_random_classes(k):
     d = to_dev(x)
     sel.mixed_encode_classes(d, n, ws)
     check_against_oracle(ws, x, classes)
+    # the single-pass encoder writes the same column
+    one = sel.MixedWorkspace(n)
+    one.cls[:nv] = torch.from_numpy(classes).cuda()
+    sel.mixed_encode(d, n, one)
+    for u, w in zip(column_parts(one, n), column_parts(ws, n)):
+        np.testing.assert_array_equal(u, w)
     out = from_dev(sel.mixed_decode(ws.column(n)))
     p = pad_vectors(x)
     for v in range(nv):
@@ -204,6 +232,15 @@ def test_confidence_fallback():
     expect = np.where(np.arange(nv) % 2 == 0, c_ex, c_rand)
     check_against_oracle(ws2, x, expect)
     np.testing.assert_array_equal(column_parts(ws2, n)[4], expect)
+    # the single-pass encode (plan, look-back offsets and pack in one launch)
+    # gives the same column and the same fallbacks as plan + pack
+    ws3 = sel.MixedWorkspace(n)
+    ws3.cls[:nv] = classes
+    ws3.fallbacks.zero_()
+    sel.mixed_encode(d, n, ws3, conf=conf2, conf_min=0.5)
+    assert int(ws3.fallbacks.item()) == (nv + 1) // 2
+    for u, w in zip(column_parts(ws3, n), column_parts(ws2, n)):
+        np.testing.assert_array_equal(u, w)
 
 
 @pytest.mark.parametrize("mode", ["exhaustive", "random"])
@@ -399,3 +436,37 @@ def test_learned_columns_serialize_and_query(trained):
         reloaded.append(fl.HostColumn.from_bytes(img).to_device())
         keep.append(ws)
     assert fl.query_q11(*reloaded, n) == o_q11(cols, n)
+
+
+def test_c5_full_scale():
+    """The learned-selector workload at its full size: train on 2^22
+    synthetic vectors (3 epochs, batch 4096, as the benchmark does), encode
+    the held-out 4 x 120M-row Q1.1 column set; the learned total is within 1%
+    of the exhaustive total and Q1.1 over the learned columns equals the
+    oracle's Q1.1 over the raw values."""
+    torch = torch_cuda()
+    feat, label, _ = sel.build_training_set(101, 1 << 22)
+    mlp = sel.Mlp(1)
+    losses = mlp.train(feat, label, epochs=3, batch=4096, lr=3e-3, seed=1)
+    del feat, label
+    assert losses[-1] < 0.5 * losses[0], losses
+    n = 120_000_000
+    cols = o_gen_c2(8, n)
+    learned = sel.LearnedSelector(mlp, conf_min=0.9)
+    enc, keep, tot_l, tot_x, fallbacks = [], [], 0, 0, 0
+    for c in cols:
+        d = to_dev(c)
+        ws, ex = sel.MixedWorkspace(n), sel.MixedWorkspace(n)
+        ws.fallbacks.zero_()
+        learned.encode(d, n, ws)
+        sel.mixed_encode_exhaustive(d, n, ex)
+        tot_l += size_of(ws)
+        tot_x += size_of(ex)
+        fallbacks += int(ws.fallbacks.item())
+        enc.append(ws.column(n))
+        keep.append(ws)
+        del d, ex
+        torch.cuda.empty_cache()
+    assert tot_l <= 1.01 * tot_x, (tot_l, tot_x)
+    assert fallbacks < 0.1 * 4 * sel.n_vectors(n), fallbacks
+    assert fl.query_q11(*enc, n) == o_q11(cols, n)
