@@ -273,6 +273,32 @@ __device__ __forceinline__ void bitmap_filter(const Col& col, const uint32_t* __
     }
 }
 
+// m &= {rows r of the lane whose key's bit is set}, for a fixed-width lane of
+// packed words w and a bitmap that holds a zero bit at index n: a key tests bit
+// min(raw + d, n), d = the column's base - kmin (keys out of range -- below
+// kmin they wrap -- read the zero bit), so a test is a min, a shift-add and a
+// bit extract.  B words are read before any is used.
+template <int B, int BW>
+__device__ __forceinline__ void bitmap_filter_min(const uint32_t (&w)[BW > 0 ? BW : 1],
+                                                  const uint32_t* __restrict bits, uint32_t d, uint32_t n,
+                                                  uint32_t& m) {
+    uint32_t hit = 0;
+#pragma unroll
+    for (int rb = 0; rb < 32; rb += B) {
+        uint32_t k[B], wv[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            k[j] = min(dev::extract_at<BW>(w, rb + j) + d, n);
+            uint32_t word;  // k >> 5 as an opaque shift: the compiler's (k >> 3) & ~3 costs one more
+            asm("v_lshrrev_b32 %0, 5, %1" : "=v"(word) : "v"(k[j]));
+            wv[j] = bits[word];
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j) hit |= __builtin_amdgcn_ubfe(wv[j], k[j], 1u) << (rb + j);
+    }
+    m &= hit;
+}
+
 // bitmap_filter with the bit index of a key computed by `idx` (e.g. a
 // reference hash table's slot, (key - min) % len), which must be below n
 // or read as a miss.

@@ -242,11 +242,13 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
     const uint32_t* part_bm = PHT ? a.occ_p : a.part_bm;
     const uint16_t* year_tab = a.year;
     if constexpr (USE_LDS) {
+        // each bitmap with one more zero word: the compact tables' bit n is zero
+        // (the sentinel of bitmap_filter_min)
         uint32_t* s = stage;
-        uint32_t* p = stage + a.supp_words;
-        uint16_t* y = reinterpret_cast<uint16_t*>(p + a.part_words);
-        for (uint32_t i = threadIdx.x; i < a.supp_words; i += blockDim.x) s[i] = supp_bm[i];
-        for (uint32_t i = threadIdx.x; i < a.part_words; i += blockDim.x) p[i] = part_bm[i];
+        uint32_t* p = stage + a.supp_words + 1u;
+        uint16_t* y = reinterpret_cast<uint16_t*>(p + a.part_words + 1u);
+        for (uint32_t i = threadIdx.x; i <= a.supp_words; i += blockDim.x) s[i] = i < a.supp_words ? supp_bm[i] : 0u;
+        for (uint32_t i = threadIdx.x; i <= a.part_words; i += blockDim.x) p[i] = i < a.part_words ? part_bm[i] : 0u;
         if constexpr (!PHT)
             for (uint32_t i = threadIdx.x; i < a.n_date; i += blockDim.x) y[i] = a.year[i];
         supp_bm = s;
@@ -282,6 +284,9 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
         if constexpr (PHT) {
 #pragma unroll
             for (int r = 0; r < 32; ++r) m |= bit_at(supp_bm, fast_mod(sc.get(r) - a.s_min, a.s_div), 0u, a.s_len) << r;
+        } else if constexpr (USE_LDS && SBW >= 0) {
+            m = 0xFFFFFFFFu;
+            bitmap_filter_min<16, SBW>(sc.w, supp_bm, sc.base - km, a.n_supp, m);
         } else {
 #pragma unroll
             for (int r = 0; r < 32; ++r) m |= bit_at(supp_bm, sc.get(r), km, a.n_supp) << r;
@@ -299,7 +304,9 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
         // part slot (PHT) or index (compact) of a key
         auto part_idx = [&](uint32_t key) { return PHT ? fast_mod(key - a.p_min, a.p_div) : key - km; };
         const uint32_t n_part_bits = PHT ? a.p_len : a.n_part;
-        if constexpr (USE_LDS) {
+        if constexpr (USE_LDS && !PHT && PBW >= 0) {
+            bitmap_filter_min<16, PBW>(pc.w, part_bm, pc.base - km, n_part_bits, m);
+        } else if constexpr (USE_LDS) {
             bitmap_filter_by<16>(pc, part_bm, part_idx, n_part_bits, m);
         } else {
 #pragma unroll

@@ -764,6 +764,167 @@ __global__ __launch_bounds__(kStarBlock) void q41_kernel(const StarDev a) {
     star_flush(a, t);
 }
 
+// kQ41 bitmap probe (fixed widths, SSB-sized dimensions).  LDS: the row
+// queues (3 fields of kQ41BQueue entries and a spare entry per wave), then at
+// fixed offsets the supplier and part filter bitmaps (a zero sentinel bit at
+// index n, so a key k tests bit min(k, n): out-of-range keys, below keys_min
+// they wrap, read the sentinel), the year codes and the 175-slot group table.
+// The survivors (8 % of the rows) are queued at slots from a wave prefix sum
+// of the lanes' counts (dead row slots write the spare entry) as raw packed
+// fields -- customer | date << CBW, revenue, supplycost -- and the flush adds
+// the frame bases and keys_min, gathers the customer nation from L2 and
+// looks the year up, four queued rows per lane.
+constexpr uint32_t kQ41BQueue = 256;
+constexpr uint32_t kQ41BStride = kQ41BQueue + 1u;
+constexpr size_t kQ41BQueueBytes = (size_t)kStarVecs / 2u * 3u * kQ41BStride * 4u;  // 16 waves, 49,344 B
+
+size_t q41_bitmap_lds(uint32_t s_words, uint32_t p_words, uint32_t n_date, uint32_t n_slots) {
+    const size_t y = kQ41BQueueBytes + 4ull * (s_words + 1u) + 4ull * (p_words + 1u);
+    return ((y + n_date + 7u) & ~(size_t)7u) + 12ull * n_slots;
+}
+
+template <int DBW, int CBW, int RBW, int KBW>
+__device__ __forceinline__ void q41_bitmap_flush(const StarDev& a, const uint8_t* __restrict y_tab,
+                                                 unsigned long long* __restrict sum, uint32_t* __restrict cnt,
+                                                 const uint32_t* q, uint32_t qn) {
+    constexpr int kPer = kQ41BQueue / 64;
+    constexpr uint32_t kCMask = CBW == 32 ? 0xFFFFFFFFu : (1u << CBW) - 1u;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t l = threadIdx.x & 63u;
+    const uint32_t km = a.keys_min;
+    uint32_t ck[kPer], cv[kPer], y[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const uint32_t i = l + 64u * j;
+        const uint32_t w = q[i];
+        ck[j] = i < qn ? (w & kCMask) + (a.cust.base - km) : 0xFFFFFFFFu;  // rows past the queue: dropped
+        y[j] = code_at(y_tab, (w >> CBW) + (a.date.base - km), 0u, a.n_date);
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) cv[j] = code_at(a.c_code, ck[j], 0u, a.n_cust);
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const uint32_t i = l + 64u * j;
+        if (cv[j] != kNone && y[j] != kNone) {
+            const uint32_t rev = q[kQ41BStride + i] + a.rev.base, cost = q[2u * kQ41BStride + i] + a.cost.base;
+            const uint32_t slot = fast_mod(cv[j] * a.slot_mul_c + y[j], a.slot_div);
+            atomicAdd(&sum[slot], (unsigned long long)((int64_t)rev - (int64_t)cost));
+            atomicAdd(&cnt[slot], 1u);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int DBW, int CBW, int SBW, int PBW, int RBW, int KBW>
+__global__ __launch_bounds__(kStarBlock) void q41_bitmap_kernel(const StarDev a) {
+    static_assert(DBW + CBW <= 32, "customer and date share a queue word");
+    extern __shared__ __align__(16) uint8_t lds[];
+    uint32_t* q = reinterpret_cast<uint32_t*>(lds) + (threadIdx.x >> 6) * 3u * kQ41BStride;
+    uint32_t* sb = reinterpret_cast<uint32_t*>(lds + kQ41BQueueBytes);
+    uint32_t* pb = sb + a.s_words + 1u;
+    uint8_t* y_tab = reinterpret_cast<uint8_t*>(pb + a.p_words + 1u);
+    const size_t g_off = (kQ41BQueueBytes + 4ull * (a.s_words + 1u) + 4ull * (a.p_words + 1u) + a.n_date + 7u) &
+                         ~(size_t)7u;
+    unsigned long long* sum = reinterpret_cast<unsigned long long*>(lds + g_off);
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(sum + a.n_slots);
+    for (uint32_t i = threadIdx.x; i <= a.s_words; i += blockDim.x) sb[i] = i < a.s_words ? a.s_bits[i] : 0u;
+    for (uint32_t i = threadIdx.x; i <= a.p_words; i += blockDim.x) pb[i] = i < a.p_words ? a.p_bits[i] : 0u;
+    for (uint32_t i = threadIdx.x; i < a.n_date; i += blockDim.x) y_tab[i] = a.y_code[i];
+    for (uint32_t i = threadIdx.x; i < a.n_slots; i += blockDim.x) {
+        sum[i] = 0ull;
+        cnt[i] = 0u;
+    }
+    __syncthreads();
+    const uint32_t tail = live_row_mask((uint32_t)(a.n_rows - (a.n_vec - 1) * 1024u), a.rev.transposed != 0);
+    const uint32_t km = a.keys_min;
+    const uint32_t d_s = a.supp.base - km, d_p = a.part.base - km;
+    uint32_t qn = 0;  // wave-uniform; the queue carries over from step to step
+    for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
+        const uint64_t v = g * kStarVecs + (threadIdx.x >> 5);
+        // the whole wave stays in step (the queue flush needs every lane): a
+        // half-wave past the last vector reads the last vector again (in
+        // bounds) and keeps no rows
+        const bool vlive = v < a.n_vec;
+        if (!__any(vlive)) continue;
+        const uint64_t vl = vlive ? v : a.n_vec - 1;
+        uint32_t m = 0;
+        {
+            LaneCol<SBW> sc;
+            LaneCol<PBW> pc;
+            sc.load(opaque_words(a.supp), vl);
+            pc.load(opaque_words(a.part), vl);
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                const uint32_t ks = min(dev::extract_at<SBW>(sc.w, r) + d_s, a.n_supp);
+                const uint32_t kp = min(dev::extract_at<PBW>(pc.w, r) + d_p, a.n_part);
+                // word indices as opaque shifts (one shift-add per address);
+                // v_bfe_u32 reads the bit offset from the key's low five bits
+                uint32_t ws, wp;
+                asm("v_lshrrev_b32 %0, 5, %1" : "=v"(ws) : "v"(ks));
+                asm("v_lshrrev_b32 %0, 5, %1" : "=v"(wp) : "v"(kp));
+                m |= (__builtin_amdgcn_ubfe(sb[ws], ks, 1u) & __builtin_amdgcn_ubfe(pb[wp], kp, 1u)) << r;
+            }
+        }
+        if (!vlive) m = 0;
+        if (v == a.n_vec - 1) m &= tail;
+        LaneCol<CBW> cc;
+        LaneCol<DBW> dc;
+        LaneCol<RBW> rc;
+        LaneCol<KBW> kc;
+        cc.load(opaque_words(a.cust), vl);
+        dc.load(opaque_words(a.date), vl);
+        rc.load(opaque_words(a.rev), vl);
+        kc.load(opaque_words(a.cost), vl);
+        cc.pin();
+        dc.pin();
+        rc.pin();
+        kc.pin();
+        if (!__any(m != 0)) continue;
+        // queue slots from a wave prefix sum of the lanes' counts; a dead row
+        // slot writes the spare entry.  More survivors than a queue holds
+        // (not at SSB selectivities) go four row slots at a time.
+        uint32_t part = 0xFFFFFFFFu;  // row slots of this round (wave-uniform)
+        while (true) {
+            const uint32_t mm = m & part;
+            const uint32_t n_mine = (uint32_t)__popc(mm);
+            const uint32_t incl = dev::wave_inclusive_sum(n_mine);
+            const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+            if (total > kQ41BQueue) {
+                part = 0xFu;
+                continue;
+            }
+            if (qn + total > kQ41BQueue) {
+                q41_bitmap_flush<DBW, CBW, RBW, KBW>(a, y_tab, sum, cnt, q, qn);
+                qn = 0;
+            }
+            const uint32_t first = qn + incl - n_mine;
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                const uint32_t below = (uint32_t)__popc(mm & ((1u << r) - 1u));
+                const uint32_t pos = (mm >> r) & 1u ? first + below : kQ41BQueue;
+                q[pos] = dev::extract_at<CBW>(cc.w, r) | dev::extract_at<DBW>(dc.w, r) << CBW;
+                q[kQ41BStride + pos] = dev::extract_at<RBW>(rc.w, r);
+                q[2u * kQ41BStride + pos] = dev::extract_at<KBW>(kc.w, r);
+            }
+            qn += total;
+            m &= ~mm;
+            if (part == 0xFFFFFFFFu || part == 0xF0000000u) break;
+            part <<= 4;
+        }
+    }
+    if (qn) q41_bitmap_flush<DBW, CBW, RBW, KBW>(a, y_tab, sum, cnt, q, qn);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < a.n_slots; i += blockDim.x)
+        if (cnt[i]) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(&a.sums[i]), sum[i]);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&a.counts[i]), (unsigned long long)cnt[i]);
+        }
+}
+
 // Q4.1 probe over columns of any encoding and width, one column at a time.
 // Revenue and supplycost enter the group sums in two passes (+revenue with the
 // row count, then -supplycost): the 64-bit wrapping sums are the same.
@@ -990,6 +1151,8 @@ int fls_query_q41(const fls_column* datekey, const fls_column* custkey, const fl
     const bool fast = all_bitpack(outs, 6) && a.date.bw == 12 && a.cust.bw == 20 && a.supp.bw == 16 &&
                       a.part.bw == 20 && a.rev.bw == 24 && a.cost.bw == 16;
     if (fast) {
+        const size_t bitmap = q41_bitmap_lds(a.s_words, a.p_words, dims->n_date, a.n_slots);
+        if (bitmap <= kLdsMax) return launch_star(q41_bitmap_kernel<12, 20, 16, 20, 24, 16>, bitmap, a, s);
         const size_t bits = table + 4ull * a.p_words + 4ull * a.s_words + dims->n_date;
         const bool fits = kQ41QueueBytes + bits <= kLdsMax;
         const size_t lds = kQ41QueueBytes + (fits ? bits : table);
