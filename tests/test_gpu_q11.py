@@ -112,3 +112,49 @@ def test_q11_rle_and_delta_columns(layout):
     want = T.o_q11(cols, n)
     for pred in (False, True):
         assert fls.query_q11(*enc, n, pred_load=pred) == want, (layout, pred)
+
+
+def test_q11_reduction_slots_reused_and_concurrent():
+    """150 calls in a row (overwriting and adding, both kernels, a one-block
+    and a 123-block grid), then 8 streams with calls in flight at once, each
+    into its own output."""
+    torch = T.torch_cuda()
+    sizes = [3 * 1024 + 77, 1_000_003]  # one block; 123 blocks (more than the 64 groups)
+    data = []
+    for i, n in enumerate(sizes):
+        cols = T.o_gen_c2(40 + i, n)
+        data.append((n, [c.to_device() for c in fls.encode_c2_table(*cols)], T.o_q11(cols, n)))
+    acc = torch.zeros(2, dtype=torch.int64, device="cuda")
+    outs = []
+    for k in range(150):
+        n, dev, _ = data[k % 2]
+        outs.append(fls.query_q11(*dev, n, pred_load=bool(k % 3 == 0), sync=False))
+        fls.query_q11(*dev, n, accumulate=True, d_out=acc, sync=False)
+    for k, o in enumerate(outs):
+        assert tuple(o.cpu().tolist()) == data[k % 2][2], k
+    want = [75 * (data[0][2][j] + data[1][2][j]) for j in range(2)]
+    assert acc.cpu().tolist() == want
+
+    streams = [torch.cuda.Stream() for _ in range(8)]
+    n, dev, w = data[1]
+    torch.cuda.synchronize()
+    res = []
+    for rep in range(4):
+        for s in streams:
+            with torch.cuda.stream(s):
+                out = torch.empty(2, dtype=torch.int64, device="cuda")
+                res.append(fls.query_q11(*dev, n, d_out=out, stream=s.cuda_stream, sync=False))
+    torch.cuda.synchronize()
+    for o in res:
+        assert tuple(o.cpu().tolist()) == w
+
+
+def test_q11_zero_rows():
+    torch = T.torch_cuda()
+    n = 1024
+    dev = [c.to_device() for c in fls.encode_c2_table(*T.o_gen_c2(3, n))]
+    out = torch.full((2,), 7, dtype=torch.int64, device="cuda")
+    assert fls.query_q11(*dev, 0, d_out=out) == (0, 0)
+    out.fill_(7)
+    fls.query_q11(*dev, 0, accumulate=True, d_out=out)
+    assert out.cpu().tolist() == [7, 7]
