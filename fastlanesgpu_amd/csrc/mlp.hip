@@ -9,10 +9,11 @@
 // order: W1[16][64], b1[64], W2[64][64], b2[64], W3[64][40], b3[40]
 // (W[in][out]); 7,848 values.
 //
-// A wave computes 16 samples at a time (one 16-row MFMA tile).  Weights are
-// staged once per block in LDS as BF16; activations go through per-wave LDS
-// tiles, so every product -- including the transposed ones of the backward
-// pass -- reads its operands with plain strides.
+// A wave computes 16 samples at a time (one 16-row MFMA tile).  Training:
+// weights staged once per block in LDS as BF16, activations through per-wave
+// LDS tiles, so every product -- including the transposed ones of the
+// backward pass -- reads its operands with plain strides.  Inference keeps
+// activations in registers between layers (see mlp_forward_kernel).
 //
 // MFMA operand layouts used here (lane L of the wave):
 //   16x16x16: A[L % 16][4*(L/16) + i], B[4*(L/16) + i][L % 16], i < 4
@@ -181,149 +182,171 @@ __device__ void forward_tile(const float* __restrict feat, uint64_t s0, uint64_t
 }
 
 // ------------------------------------------------------------- inference
-// Inference keeps its own LDS layout: every operand is stored so that an MFMA
-// fragment (4 consecutive k of one row m, or of one column n) is one aligned
-// 8-byte LDS read -- activations row-major, weights transposed (n-major) --
-// and rows are padded (by 4 elements for the 16-deep input tile, by 8 for the
-// hidden tiles so 8-element fragments stay 16-byte aligned) so the 16 rows a
-// fragment load touches fall in different banks.
-constexpr int kLdX = kIn + 4, kLdH = kH + 8;
+// The forward holds the weights and the activations in registers and runs
+// the transposed products D = W^T X^T: a wave's 16 samples are the columns of
+// every product, the hidden units its rows.  A D tile leaves lane L holding
+// rows 4(L/16) + i (i < 4) -- units 16t + 4(L/16) + i of tile t -- of sample
+// L % 16, and the next layer's 32-deep B fragment wants 8 k of that sample.
+// The k order of a product is free, so k slot 32c + 8(L/16) + j of chunk c is
+// taken to be unit 16(2c + j/4) + 4(L/16) + j%4: exactly the lane's own
+// values of tiles 2c and 2c + 1, converted to BF16 in place.  The A fragments
+// (W^T rows in that k order) are built once per block; the biases initialise
+// the accumulators.  The 40 classes sit in 48 output rows so that
+// row 16t + r is encoding r % 4, width bucket 3(r / 4) + t (buckets 10 and 11:
+// zero weights, -inf bias): a lane then holds three buckets of every encoding
+// and the softmax reduces within the lane and over the 4 lanes of a sample.
+__device__ __forceinline__ uint32_t unit_of_slot(uint32_t c, uint32_t q, uint32_t j) {
+    return 16u * (2u * c + (j >> 2)) + 4u * q + (j & 3u);
+}
+__device__ __forceinline__ int class_of_row(int t, int r) {
+    const int w = 3 * (r >> 2) + t;
+    return w < 10 ? 10 * (r & 3) + w : -1;
+}
 
-struct alignas(16) InferWave {
-    uint16_t x[16 * kLdX];
-    uint16_t h1[16 * kLdH];
-    uint16_t h2[16 * kLdH];
-    float z[16 * kOutPad];
-};
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+// two FP32 values to a BF16 pair, round to nearest even (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint32_t cvt2(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2v));
+}
+// the B fragment of chunk c from the ReLU of D tiles 2c and 2c + 1
+__device__ __forceinline__ s16x8 relu_frag(const f32x4& lo, const f32x4& hi) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 u = {cvt2(fmaxf(lo[0], 0.f), fmaxf(lo[1], 0.f)), cvt2(fmaxf(lo[2], 0.f), fmaxf(lo[3], 0.f)),
+                     cvt2(fmaxf(hi[0], 0.f), fmaxf(hi[1], 0.f)), cvt2(fmaxf(hi[2], 0.f), fmaxf(hi[3], 0.f))};
+    return __builtin_bit_cast(s16x8, u);
+}
+
+// A-fragment images, one entry per (tile, chunk, lane): a wave reads the
+// second layer's into registers once and the first and last layers' from LDS
+// per tile (VGPRs for all three spill at four waves per SIMD).
 struct alignas(16) InferLds {
-    uint16_t w1t[kH * kLdX];       // W1^T: [n][k], n < 64, k < 16
-    uint16_t w2t[kH * kLdH];       // W2^T: [n][k]
-    uint16_t w3t[kOutPad * kLdH];  // W3^T: [n][k], rows n >= 40 zero
-    float b1[kH], b2[kH], b3[kOutPad];
-    InferWave t[4];
+    bf16x4 f1[4][64];     // A[16t + m][4q + i] = W1[4q + i][16t + m]
+    s16x8 f2[4][2][64];   // A[16t + m][slot (c, q, j)] = W2[unit][16t + m]
+    s16x8 f3[3][2][64];   // A[16t + m][slot (c, q, j)] = W3[unit][class of row 16t + m]
+    float b1[kH], b2[kH], b3[kOutPad];  // b3 by output row, -inf on the padding rows
 };
 
-__device__ void stage_weights_t(const uint16_t* __restrict wb, InferLds& s) {
-    for (int i = threadIdx.x; i < kIn * kH; i += blockDim.x) {
-        const int k = i / kH, n = i % kH;  // wb[oW1 + k*64 + n] = W1[k][n]
-        s.w1t[n * kLdX + k] = wb[oW1 + i];
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void mlp_forward_kernel(
+    const uint16_t* __restrict wb, const float* __restrict feat, uint64_t n, uint8_t* __restrict cls,
+    float* __restrict conf, float* __restrict logits) {
+    __shared__ InferLds s;
+    for (int e = threadIdx.x; e < 4 * 64; e += blockDim.x) {
+        const int t = e >> 6, lq = (e >> 4) & 3, lm = e & 15;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s.f1[t][e & 63][i] = (short)wb[oW1 + (4 * lq + i) * kH + 16 * t + lm];
     }
-    for (int i = threadIdx.x; i < kH * kH; i += blockDim.x) {
-        const int k = i / kH, n = i % kH;
-        s.w2t[n * kLdH + k] = wb[oW2 + i];
+    for (int e = threadIdx.x; e < 8 * 64; e += blockDim.x) {
+        const int t = e >> 7, c = (e >> 6) & 1, lq = (e >> 4) & 3, lm = e & 15;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s.f2[t][c][e & 63][j] = (short)wb[oW2 + unit_of_slot(c, lq, j) * kH + 16 * t + lm];
     }
-    for (int i = threadIdx.x; i < kH * kOutPad; i += blockDim.x) {
-        const int n = i / kH, k = i % kH;
-        s.w3t[n * kLdH + k] = n < kOut ? wb[oW3 + k * kOut + n] : (uint16_t)0;
+    for (int e = threadIdx.x; e < 6 * 64; e += blockDim.x) {
+        const int t = e >> 7, c = (e >> 6) & 1, lq = (e >> 4) & 3, lm = e & 15;
+        const int cl = class_of_row(t, lm);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            s.f3[t][c][e & 63][j] = cl >= 0 ? (short)wb[oW3 + unit_of_slot(c, lq, j) * kOut + cl] : (short)0;
     }
     for (int i = threadIdx.x; i < kH; i += blockDim.x) {
         s.b1[i] = from_bf16(wb[oB1 + i]);
         s.b2[i] = from_bf16(wb[oB2 + i]);
     }
-    for (int i = threadIdx.x; i < kOutPad; i += blockDim.x) s.b3[i] = i < kOut ? from_bf16(wb[oB3 + i]) : 0.f;
-}
-
-// A fragment (rows m0.., k0..k0+15) of a row-major tile with leading dim ld,
-// and a B fragment (k0.., columns n0..) of a matrix stored transposed.
-__device__ __forceinline__ bf16x4 frag_row(const uint16_t* __restrict a, int ld, int m0, int k0) {
-    const int L = lane64();
-    return *reinterpret_cast<const bf16x4*>(a + (m0 + (L & 15)) * ld + k0 + 4 * (L >> 4));
-}
-__device__ __forceinline__ s16x8 frag_row8(const uint16_t* __restrict a, int ld, int m0, int k0) {
-    const int L = lane64();
-    return *reinterpret_cast<const s16x8*>(a + (m0 + (L & 15)) * ld + k0 + 8 * (L >> 4));
-}
-// C tile (0, n0) of A * W with A row-major and W stored transposed (n-major)
-__device__ __forceinline__ f32x4 infer_tile(const uint16_t* __restrict a, int lda, const uint16_t* __restrict wt,
-                                            int ldw, int K, int n0) {
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    if (K % 32 == 0) {
-        for (int k0 = 0; k0 < K; k0 += 32) acc = mfma32(frag_row8(a, lda, 0, k0), frag_row8(wt, ldw, n0, k0), acc);
-    } else {
-        for (int k0 = 0; k0 < K; k0 += 16) acc = mfma(frag_row(a, lda, 0, k0), frag_row(wt, ldw, n0, k0), acc);
+    for (int i = threadIdx.x; i < kOutPad; i += blockDim.x) {
+        const int c = class_of_row(i >> 4, i & 15);
+        s.b3[i] = c >= 0 ? from_bf16(wb[oB3 + c]) : -INFINITY;
     }
-    return acc;
-}
-
-__device__ __forceinline__ void infer_layer(const uint16_t* __restrict a, int lda, const uint16_t* __restrict wt,
-                                            int ldw, int K, int N, const float* __restrict bias,
-                                            uint16_t* __restrict h, int ldh) {
-    for (int n0 = 0; n0 < N; n0 += 16) {
-        const f32x4 acc = infer_tile(a, lda, wt, ldw, K, n0);
-        for (int i = 0; i < 4; ++i) {
-            const float v = acc[i] + bias[n0 + d_col()];
-            h[d_row(i) * ldh + n0 + d_col()] = to_bf16(v > 0.f ? v : 0.f);
-        }
-    }
-}
-
-__global__ __launch_bounds__(256) void mlp_forward_kernel(const uint16_t* __restrict wb, const float* __restrict feat,
-                                                          uint64_t n, uint8_t* __restrict cls, float* __restrict conf,
-                                                          float* __restrict logits) {
-    __shared__ InferLds s;
-    stage_weights_t(wb, s);
     __syncthreads();
-    InferWave& t = s.t[threadIdx.x >> 6];
-    const int L = lane64();
+    const int L = lane64(), q = L >> 4, m = L & 15;
+    s16x8 a2[4][2];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) a2[t][c] = s.f2[t][c][L];
     const uint64_t n_tiles = ceil_div(n, 16);
-    const uint64_t waves = (uint64_t)gridDim.x * 4;
-    for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < n_tiles; tile += waves) {
-        const uint64_t s0 = tile * 16;
-        for (int i = L; i < 16 * kIn; i += 64) {
-            const uint64_t row = s0 + i / kIn;
-            t.x[(i / kIn) * kLdX + (i % kIn)] = to_bf16(row < n ? feat[row * kIn + (i % kIn)] : 0.f);
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    for (uint64_t tile = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < n_tiles; tile += waves) {
+        const uint64_t row = tile * 16 + m;
+        // an index the compiler cannot prove loop-invariant: the first and
+        // last layers' fragments are read from LDS per tile, not hoisted
+        // into registers (which spill)
+        int Lx = L;
+        asm volatile("" : "+v"(Lx));
+        f32x4 f = {0.f, 0.f, 0.f, 0.f};
+        if (row < n) f = *reinterpret_cast<const f32x4*>(feat + row * kIn + 4 * q);
+        typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+        const bf16x4 x = __builtin_bit_cast(bf16x4, u32x2{cvt2(f[0], f[1]), cvt2(f[2], f[3])});
+        f32x4 d[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) d[t] = mfma(s.f1[t][Lx], x, *reinterpret_cast<const f32x4*>(s.b1 + 16 * t + 4 * q));
+        const s16x8 h0 = relu_frag(d[0], d[1]), h1 = relu_frag(d[2], d[3]);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            d[t] = mfma32(a2[t][1], h1, mfma32(a2[t][0], h0, *reinterpret_cast<const f32x4*>(s.b2 + 16 * t + 4 * q)));
+        const s16x8 g0 = relu_frag(d[0], d[1]), g1 = relu_frag(d[2], d[3]);
+        f32x4 z[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+            z[t] = mfma32(s.f3[t][1][Lx], g1,
+                          mfma32(s.f3[t][0][Lx], g0, *reinterpret_cast<const f32x4*>(s.b3 + 16 * t + 4 * q)));
+        // z[t][i]: encoding i, bucket 3q + t.  Per encoding the largest logit
+        // and its bucket (the lowest bucket on a tie), over the lane's three
+        // and then over the sample's four lanes
+        float gm[4];
+        int gb[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            gm[i] = z[0][i];
+            gb[i] = 3 * q;
+#pragma unroll
+            for (int t = 1; t < 3; ++t)
+                if (z[t][i] > gm[i]) {
+                    gm[i] = z[t][i];
+                    gb[i] = 3 * q + t;
+                }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        infer_layer(t.x, kLdX, s.w1t, kLdX, kIn, kH, s.b1, t.h1, kLdH);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        infer_layer(t.h1, kLdH, s.w2t, kLdH, kH, kH, s.b2, t.h2, kLdH);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (int n0 = 0; n0 < kOutPad; n0 += 16) {
-            const f32x4 acc = infer_tile(t.h2, kLdH, s.w3t, kLdH, kH, n0);
-            for (int i = 0; i < 4; ++i) t.z[d_row(i) * kOutPad + n0 + d_col()] = acc[i] + s.b3[n0 + d_col()];
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // softmax over all 64 lanes: lane L takes row r = L % 16 and the ten
-        // classes of encoding g = L / 16; the encoding with the largest mass
-        // wins, its most likely width bucket is the class, its mass the
-        // confidence
-        const int r = L & 15, g = L >> 4;
-        const float* z = t.z + r * kOutPad + 10 * g;
-        float gmax = z[0];
-        int bi = 0;
-        for (int c = 1; c < 10; ++c)
-            if (z[c] > gmax) {
-                gmax = z[c];
-                bi = c;
+#pragma unroll
+        for (int off = 16; off <= 32; off <<= 1)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float om = __shfl_xor(gm[i], off, 64);
+                const int ob = __shfl_xor(gb[i], off, 64);
+                if (om > gm[i] || (om == gm[i] && ob < gb[i])) {
+                    gm[i] = om;
+                    gb[i] = ob;
+                }
             }
-        float rmax = fmaxf(gmax, __shfl_xor(gmax, 16, 64));
-        rmax = fmaxf(rmax, __shfl_xor(rmax, 32, 64));
-        float mass = 0.f;
-        for (int c = 0; c < 10; ++c) mass += expf(z[c] - rmax);
-        float m[4];
-        int b[4];
-        for (int k = 0; k < 4; ++k) {
-            m[k] = __shfl(mass, r + 16 * k, 64);
-            b[k] = __shfl(bi, r + 16 * k, 64);
+        const float rmax = fmaxf(fmaxf(gm[0], gm[1]), fmaxf(gm[2], gm[3]));
+        float mass[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            mass[i] = __expf(z[0][i] - rmax) + __expf(z[1][i] - rmax) + __expf(z[2][i] - rmax);
+            mass[i] += __shfl_xor(mass[i], 16, 64);
+            mass[i] += __shfl_xor(mass[i], 32, 64);
         }
-        int be = 0;
-        for (int k = 1; k < 4; ++k)
-            if (m[k] > m[be]) be = k;
-        const uint64_t row = s0 + r;
-        if (g == 0 && row < n) {
-            cls[row] = (uint8_t)(10 * be + b[be]);
-            if (conf) conf[row] = m[be] / (m[0] + m[1] + m[2] + m[3]);
+        // the encoding with the largest mass wins (the first on a tie); its
+        // most likely bucket is the class, its share of the mass the confidence
+        int be = 0, bucket = gb[0];
+        float mb = mass[0];
+#pragma unroll
+        for (int i = 1; i < 4; ++i)
+            if (mass[i] > mb) {
+                mb = mass[i];
+                be = i;
+                bucket = gb[i];
+            }
+        if (row < n) {
+            if (q == 0) {
+                cls[row] = (uint8_t)(10 * be + bucket);
+                if (conf) conf[row] = mb / (mass[0] + mass[1] + mass[2] + mass[3]);
+            }
+            if (logits)
+#pragma unroll
+                for (int t = 0; t < 3; ++t)
+                    if (3 * q + t < 10)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) logits[row * kOut + 10 * i + 3 * q + t] = z[t][i];
         }
-        if (logits && row < n)
-            for (int c = 0; c < 10; ++c) logits[row * kOut + 10 * g + c] = z[c];
     }
 }
 
