@@ -11,7 +11,15 @@
 //  * predicates become per-row lane masks that the compiler keeps in SGPR
 //    pairs, so AND-ing predicates is scalar work;
 //  * the int64 revenue and the row count are reduced per wave with DPP/shuffle,
-//    per block through LDS, and flushed with one 64-bit atomic each per block.
+//    per block through LDS, and across the blocks inside the launch (see
+//    q11_block_finish): per-block sums spread over 64 lines, a count per line,
+//    the last block of the last line collecting the lines.  Two 64-bit atomics
+//    per block on the output instead (the reference's form) cost 8 us of the
+//    120M-row scan's 85: 2,048 same-address atomics serialise.
+#include <atomic>
+#include <mutex>
+#include <vector>
+
 #include "fls/device/unpack.hpp"
 #include "fls/device/wave.hpp"
 #include "fls_runtime.hpp"
@@ -29,7 +37,63 @@ struct Q11Dev {
     uint32_t date_gt, date_lt, qty_lt, disc_lo, disc_hi;
     int64_t* revenue;
     uint64_t* count;
+    unsigned long long* slot;  // this call's reduction slot (zero at entry and exit)
+    uint32_t accumulate;
 };
+
+// Cross-block reduction state: per slot, 64 group lines of 128 bytes ([0]
+// revenue, [1] count, [2] low word: blocks of the group done) and one line
+// whose low word counts the groups done.  A static per-device array of 64
+// slots, taken round-robin per call: calls that overlap on one device use
+// different slots, up to 64 of them at once.
+constexpr uint32_t kQ11Groups = 64;
+constexpr uint32_t kQ11SlotWords = 16u * (kQ11Groups + 1u);
+constexpr uint32_t kQ11Slots = 64;
+static __device__ __align__(256) unsigned long long q11_slots[kQ11Slots][kQ11SlotWords];
+
+// The block's sums (per-wave partials in s_acc / s_cnt) into the launch's
+// reduction; every work-item calls it after the block barrier, wave 0 does
+// the work.  Block b adds its sums to line b % 64 (lanes 0 and 1, one atomic
+// each, their returns waited for) and then counts itself there; the block
+// that completes a line counts the line; the block that completes the last
+// line reads and clears every line (a lane each) and writes, or with
+// FLS_Q_ACCUMULATE adds, the results.  Each count wraps to zero as it
+// completes (atomicInc), so the slot is zero again when the launch ends.
+__device__ __forceinline__ void q11_block_finish(const Q11Dev& a, const uint64_t* s_acc, const uint64_t* s_cnt) {
+    if (threadIdx.x >= 64) return;
+    const uint32_t l = threadIdx.x, grid = gridDim.x, g = blockIdx.x % kQ11Groups;
+    const uint32_t n_g = (grid - 1u - g) / kQ11Groups + 1u;  // blocks of line g
+    const uint32_t n_lines = grid < kQ11Groups ? grid : kQ11Groups;
+    unsigned long long* line = a.slot + 16u * g;
+    uint32_t last = 0;
+    if (l < 2) {  // a per-lane address: one instruction, both adds in flight together
+        unsigned long long v = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) v += l ? s_cnt[w] : s_acc[w];
+        const unsigned long long prev = atomicAdd(&line[l], v);
+        asm volatile("" : : "v"(prev) : "memory");  // performed before the count below
+    }
+    if (l == 0 && atomicInc(reinterpret_cast<unsigned int*>(&line[2]), n_g - 1u) == n_g - 1u)
+        last = atomicInc(reinterpret_cast<unsigned int*>(a.slot + 16u * kQ11Groups), n_lines - 1u) ==
+               n_lines - 1u;
+    if (!__shfl(last, 0, 64)) return;
+    unsigned long long r = 0, c = 0;
+    if (l < n_lines) {
+        r = atomicExch(a.slot + 16u * l, 0ull);
+        c = atomicExch(a.slot + 16u * l + 1u, 0ull);
+    }
+    r = dev::wave_sum(r);
+    c = dev::wave_sum(c);
+    if (l == 0) {
+        if (a.accumulate) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(a.revenue), r);
+            if (a.count) atomicAdd(reinterpret_cast<unsigned long long*>(a.count), c);
+        } else {
+            *reinterpret_cast<unsigned long long*>(a.revenue) = r;
+            if (a.count) *a.count = c;
+        }
+    }
+}
 
 // Generic path: any encoding per column (bit-pack + FOR, delta, RLE, mixed),
 // run-time widths.  Every column is walked row by row at its own width
@@ -88,16 +152,7 @@ __global__ __launch_bounds__(kBlock) void q11_kernel(const Q11Dev a) {
         s_cnt[wave] = c64;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t ta = 0, tc = 0;
-#pragma unroll
-        for (int w = 0; w < kBlock / 64; ++w) {
-            ta += s_acc[w];
-            tc += s_cnt[w];
-        }
-        if (ta) atomicAdd((unsigned long long*)a.revenue, (unsigned long long)ta);
-        if (a.count && tc) atomicAdd((unsigned long long*)a.count, (unsigned long long)tc);
-    }
+    q11_block_finish(a, s_acc, s_cnt);
 }
 
 // Row mask of the existing rows of this work-item (bit r = row r).
@@ -198,16 +253,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         s_cnt[wave] = c64;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t ta = 0, tc = 0;
-#pragma unroll
-        for (int w = 0; w < kBlock / 64; ++w) {
-            ta += s_acc[w];
-            tc += s_cnt[w];
-        }
-        if (ta) atomicAdd((unsigned long long*)a.revenue, (unsigned long long)ta);
-        if (a.count && tc) atomicAdd((unsigned long long*)a.count, (unsigned long long)tc);
-    }
+    q11_block_finish(a, s_acc, s_cnt);
 }
 
 template <typename Kernel>
@@ -217,6 +263,30 @@ int launch_resident(Kernel k, const Q11Dev& a, hipStream_t s) {
     if (st) return st;
     hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, a);
     return check_launch();
+}
+
+// This call's reduction slot: the device address of q11_slots (looked up
+// once per device, with that device current) plus the next slot in
+// round-robin order.
+int q11_slot(unsigned long long** out) {
+    static std::mutex mu;
+    static std::vector<unsigned long long*> base;  // per device
+    static std::atomic<uint32_t> next{0};
+    int dev = 0;
+    FLS_HIP_CHECK(hipGetDevice(&dev));
+    unsigned long long* b = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        if ((int)base.size() <= dev) base.resize(dev + 1, nullptr);
+        if (!base[dev]) {
+            void* p = nullptr;
+            FLS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(q11_slots)));
+            base[dev] = static_cast<unsigned long long*>(p);
+        }
+        b = base[dev];
+    }
+    *out = b + (uint64_t)(next.fetch_add(1u) % kQ11Slots) * kQ11SlotWords;
+    return FLS_OK;
 }
 
 int launch_generic(const Q11Dev& a, bool pred, hipStream_t s) {
@@ -258,12 +328,14 @@ extern "C" int fls_query_q11(const fls_column* orderdate, const fls_column* quan
     a.disc_hi = p->disc_hi;
     a.revenue = d_revenue;
     a.count = d_count;
+    a.accumulate = (flags & FLS_Q_ACCUMULATE) != 0;
     hipStream_t s = as_stream(stream);
-    if (!(flags & FLS_Q_ACCUMULATE)) {
+    if (n_rows == 0) {  // no launch writes the results: zero them unless adding
+        if (a.accumulate) return FLS_OK;
         if ((st = zero_u64(reinterpret_cast<uint64_t*>(d_revenue), 1, s)) != FLS_OK) return st;
-        if (d_count && (st = zero_u64(d_count, 1, s)) != FLS_OK) return st;
+        return d_count ? zero_u64(d_count, 1, s) : FLS_OK;
     }
-    if (n_rows == 0) return FLS_OK;
+    if ((st = q11_slot(&a.slot)) != FLS_OK) return st;
     const bool pred = (flags & FLS_Q_PRED_LOAD) != 0;
     const bool fast = a.date.enc == FLS_ENC_DELTA && a.qty.enc == FLS_ENC_BITPACK && a.disc.enc == FLS_ENC_BITPACK &&
                       a.price.enc == FLS_ENC_BITPACK && a.qty.bw == 6 && a.disc.bw == 4 && a.price.bw == 24;

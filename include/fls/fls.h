@@ -205,9 +205,11 @@ typedef struct fls_column {
  *   date_gt < orderdate < date_lt  AND  quantity < qty_lt  AND
  *   disc_lo <= discount <= disc_hi
  * contribute price*discount (64-bit product) to *d_revenue; *d_count (may be
- * NULL) gets the number of qualifying rows.  Both are overwritten (zeroed by a
- * small kernel first) unless flags has FLS_Q_ACCUMULATE.
- * orderdate may be DELTA or BITPACK; the others BITPACK.                     */
+ * NULL) gets the number of qualifying rows.  Both are overwritten unless flags
+ * has FLS_Q_ACCUMULATE.  One kernel launch per call: the blocks' sums are
+ * reduced inside it through one of 64 reduction slots of a static per-device
+ * array, taken round-robin, so up to 64 calls may run at the same time on one
+ * device.  orderdate may be DELTA or BITPACK; the others BITPACK.           */
 typedef struct fls_q11_params {
     uint32_t date_gt, date_lt, qty_lt, disc_lo, disc_hi;
 } fls_q11_params;
@@ -218,8 +220,7 @@ typedef struct fls_q11_params {
  * every column is read (full scan).                                          */
 #define FLS_Q_PRED_LOAD 1u
 /* Add into *d_revenue / *d_count instead of overwriting them: the caller
- * zeroes them first, as query<>() does with cudaMemset (fls_q11.cu:146-148).
- * The call is then exactly one kernel launch.                               */
+ * zeroes them first, as query<>() does with cudaMemset (fls_q11.cu:146-148). */
 #define FLS_Q_ACCUMULATE 2u
 
 int fls_query_q11(const fls_column* orderdate, const fls_column* quantity, const fls_column* discount,
