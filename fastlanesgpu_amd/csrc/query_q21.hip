@@ -20,7 +20,9 @@
 // (fls_q21.cu:71).
 #include <string.h>
 
+#include <atomic>
 #include <mutex>
+#include <vector>
 
 #include "fls/device/crystal.hpp"
 #include "fls/device/unpack.hpp"
@@ -122,7 +124,46 @@ struct Q21Dev {
     FastDiv slot_div;  // n_slots
     int64_t* sums;
     uint64_t* counts;
+    unsigned long long* copies;  // compact probe: this call's set of partial tables, or null (adds to sums / counts)
 };
+
+// The compact probe's blocks add their group tables to one of kQ21Copies
+// partial tables (block b to copy b % kQ21Copies) and a second kernel sums the
+// copies into the output, clearing them.  Adding every block's groups to the
+// output directly cost 24-28 us of the 120M-row probe's 164: the ~280 groups
+// of SSB Q2.1 sit in 18 cache lines, and the 245 blocks' atomics on them,
+// issued as the blocks finish together, serialise.  A copy is a sums and a
+// counts array of kQ21MaxSlots; a set (kQ21Copies copies) belongs to one call,
+// taken round-robin from a static per-device array: up to kQ21Sets calls may
+// run at the same time on one device.  Tables of more slots add to the output
+// directly.
+constexpr uint32_t kQ21Copies = 16, kQ21MaxSlots = 8192, kQ21Sets = 8;
+constexpr uint64_t kQ21SetWords = 2ull * kQ21Copies * kQ21MaxSlots;  // 2 MiB
+static __device__ __align__(256) unsigned long long q21_copies[kQ21Sets][kQ21SetWords];
+
+// One work-item per slot: the copies' sums and counts, read and cleared with
+// atomics (the probe's atomics are performed at the device's coherence point),
+// written to the output or, with FLS_Q_ACCUMULATE, added to it.
+__global__ __launch_bounds__(256) void q21_collect_kernel(unsigned long long* __restrict set, uint32_t n_slots,
+                                                          int64_t* __restrict sums, uint64_t* __restrict counts,
+                                                          uint32_t accumulate) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n_slots) return;
+    unsigned long long sm = 0, cn = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kQ21Copies; ++k) {
+        unsigned long long* cp = set + (uint64_t)k * (2u * kQ21MaxSlots);
+        sm += atomicExch(&cp[i], 0ull);
+        cn += atomicExch(&cp[kQ21MaxSlots + i], 0ull);
+    }
+    if (accumulate) {
+        sums[i] = (int64_t)((unsigned long long)sums[i] + sm);
+        counts[i] += cn;
+    } else {
+        sums[i] = (int64_t)sm;
+        counts[i] = cn;
+    }
+}
 
 __device__ __forceinline__ void group_add(const Q21Dev& a, uint32_t brand, uint32_t year, uint32_t revenue) {
     const uint32_t slot = fast_mod(brand * a.n_years + (year - a.year_base), a.slot_div);
@@ -183,7 +224,7 @@ constexpr size_t kQueueBytes = 16ull * 3u * kQueue * 4u;  // 16 waves of a 1024-
 // drops the row, else year = slot >> 32).
 template <bool PHT, bool LATE, int DBW, int RBW>
 __device__ __forceinline__ void q21_flush(const Q21Dev& a, uint32_t* q, uint32_t qn, const uint16_t* year_tab,
-                                          const Q21Hash& h) {
+                                          const Q21Hash& h, int64_t* gs, uint64_t* gc) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -207,11 +248,11 @@ __device__ __forceinline__ void q21_flush(const Q21Dev& a, uint32_t* q, uint32_t
             if (ds != 0) {
                 const uint32_t brand = a.ht_p[2u * p + 1u];
                 const uint32_t slot = fast_mod(brand * a.n_years + ((uint32_t)(ds >> 32) - a.year_base), a.slot_div);
-                h.add(slot, rev, a.sums, a.counts);
+                h.add(slot, rev, gs, gc);
             }
         } else if (d < a.n_date) {
             const uint32_t slot = fast_mod(a.brand[p] * a.n_years + (year_tab[d] - a.year_base), a.slot_div);
-            h.add(slot, rev, a.sums, a.counts);
+            h.add(slot, rev, gs, gc);
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -241,6 +282,13 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
     const uint32_t* supp_bm = PHT ? a.occ_s : a.supp_bm;
     const uint32_t* part_bm = PHT ? a.occ_p : a.part_bm;
     const uint16_t* year_tab = a.year;
+    int64_t* gs = a.sums;  // where this block's groups go
+    uint64_t* gc = a.counts;
+    if (a.copies) {
+        unsigned long long* cp = a.copies + (uint64_t)(blockIdx.x % kQ21Copies) * (2u * kQ21MaxSlots);
+        gs = reinterpret_cast<int64_t*>(cp);
+        gc = reinterpret_cast<uint64_t*>(cp + kQ21MaxSlots);
+    }
     if constexpr (USE_LDS) {
         // each bitmap with one more zero word: the compact tables' bit n is zero
         // (the sentinel of bitmap_filter_min)
@@ -330,7 +378,7 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
             if (bal == 0) continue;
             const uint32_t cnt = (uint32_t)__popcll(bal);
             if (qn + cnt > kQueue) {
-                q21_flush<PHT, LATE, DBW, RBW>(a, q, qn, year_tab, h);
+                q21_flush<PHT, LATE, DBW, RBW>(a, q, qn, year_tab, h, gs, gc);
                 qn = 0;
             }
             if (live) {
@@ -348,8 +396,8 @@ __global__ __launch_bounds__(1024) void q21_compact_kernel(const Q21Dev a) {
             qn += cnt;
         }
     }
-    if (qn) q21_flush<PHT, LATE, DBW, RBW>(a, q, qn, year_tab, h);
-    h.flush(a.sums, a.counts);
+    if (qn) q21_flush<PHT, LATE, DBW, RBW>(a, q, qn, year_tab, h, gs, gc);
+    h.flush(gs, gc);
 }
 
 // Occupancy bitmap of a reference hash table: lane i of a wave tests slot
@@ -452,14 +500,52 @@ int launch_resident_q21(Kernel k, int block, size_t lds, const Q21Dev& a, hipStr
     return check_launch();
 }
 
+// The output of the paths that add to it directly: zeroed first unless the
+// caller accumulates.
+int zero_groups(const Q21Dev& a, uint32_t flags, hipStream_t s) {
+    if (flags & FLS_Q_ACCUMULATE) return FLS_OK;
+    const int st = zero_u64(reinterpret_cast<uint64_t*>(a.sums), a.n_slots, s);
+    return st ? st : zero_u64(a.counts, a.n_slots, s);
+}
+
+// This call's set of partial tables: the device address of q21_copies (looked
+// up once per device, with that device current) plus the next set in
+// round-robin order.
+int q21_set(unsigned long long** out) {
+    static std::mutex mu;
+    static std::vector<unsigned long long*> base;  // per device
+    static std::atomic<uint32_t> next{0};
+    int dev = 0;
+    FLS_HIP_CHECK(hipGetDevice(&dev));
+    unsigned long long* b = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        if ((int)base.size() <= dev) base.resize(dev + 1, nullptr);
+        if (!base[dev]) {
+            void* p = nullptr;
+            FLS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(q21_copies)));
+            base[dev] = static_cast<unsigned long long*>(p);
+        }
+        b = base[dev];
+    }
+    *out = b + (uint64_t)(next.fetch_add(1u) % kQ21Sets) * kQ21SetWords;
+    return FLS_OK;
+}
+
 template <int DBW, int SBW, int PBW, int RBW, bool PHT = false>
-int launch_compact(const Q21Dev& a, bool in_lds, size_t lds, hipStream_t s) {
+int launch_compact(Q21Dev a, bool in_lds, size_t lds, uint32_t flags, hipStream_t s) {
     FLS_RETURN_IF(a.n_vec >> 32, FLS_ERR_ARG);  // the row queue holds a vector index in 32 bits
     auto k = in_lds ? q21_compact_kernel<DBW, SBW, PBW, RBW, true, PHT>
                     : q21_compact_kernel<DBW, SBW, PBW, RBW, false, PHT>;
     const size_t bytes = Q21Hash::kBytes + kQueueBytes + (in_lds ? lds : 0);
     FLS_HIP_CHECK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    return launch_resident_q21(k, 1024, bytes, a, s);
+    const bool spread = a.n_slots <= kQ21MaxSlots;
+    int st = spread ? q21_set(&a.copies) : zero_groups(a, flags, s);
+    if (st) return st;
+    if ((st = launch_resident_q21(k, 1024, bytes, a, s)) != FLS_OK || !spread) return st;
+    hipLaunchKernelGGL(q21_collect_kernel, dim3((a.n_slots + 255u) / 256u), dim3(256), 0, s, a.copies, a.n_slots,
+                       a.sums, a.counts, (flags & FLS_Q_ACCUMULATE) ? 1u : 0u);
+    return check_launch();
 }
 
 // allow_var: the compact probe decodes any encoding (decode_regs); the
@@ -481,8 +567,7 @@ int fill_fact(Q21Dev& a, const fls_column* date, const fls_column* supp, const f
     return FLS_OK;
 }
 
-int fill_groups(Q21Dev& a, const fls_q21_params* p, int64_t* d_sums, uint64_t* d_counts, uint32_t flags,
-                hipStream_t s) {
+int fill_groups(Q21Dev& a, const fls_q21_params* p, int64_t* d_sums, uint64_t* d_counts) {
     if (!p || !d_sums || !d_counts) return FLS_ERR_NULL;
     if (p->n_slots == 0 || p->n_years == 0) return FLS_ERR_ARG;
     a.year_base = p->year_base;
@@ -491,11 +576,6 @@ int fill_groups(Q21Dev& a, const fls_q21_params* p, int64_t* d_sums, uint64_t* d
     a.slot_div = make_fastdiv(p->n_slots);
     a.sums = d_sums;
     a.counts = d_counts;
-    if (!(flags & FLS_Q_ACCUMULATE)) {
-        int st = zero_u64(reinterpret_cast<uint64_t*>(d_sums), p->n_slots, s);
-        if (st) return st;
-        return zero_u64(d_counts, p->n_slots, s);
-    }
     return FLS_OK;
 }
 
@@ -589,8 +669,8 @@ int fls_query_q21_pht(const fls_column* datekey, const fls_column* suppkey, cons
     int st = fill_fact(a, datekey, suppkey, partkey, revenue, n_rows, false);
     if (st) return st;
     hipStream_t s = as_stream(stream);
-    if ((st = fill_groups(a, params, d_sums, d_counts, flags, s)) != FLS_OK) return st;
-    if (n_rows == 0) return FLS_OK;
+    if ((st = fill_groups(a, params, d_sums, d_counts)) != FLS_OK) return st;
+    if (n_rows == 0) return zero_groups(a, flags, s);
     a.ht_s = ht->ht_s;
     a.ht_p = ht->ht_p;
     a.ht_d = ht->ht_d;
@@ -613,8 +693,9 @@ int fls_query_q21_pht(const fls_column* datekey, const fls_column* suppkey, cons
         a.n_groups = ceil_div(a.n_vec, 32);
         const size_t lds = 4ull * (a.supp_words + a.part_words) + 16u;
         const bool in_lds = lds + Q21Hash::kBytes + kQueueBytes <= 150u * 1024u;
-        return launch_compact<12, 16, 20, 24, true>(a, in_lds, lds, s);
+        return launch_compact<12, 16, 20, 24, true>(a, in_lds, lds, flags, s);
     }
+    if ((st = zero_groups(a, flags, s)) != FLS_OK) return st;
     a.n_groups = ceil_div(a.n_vec, 8);
     if (fast) return launch_resident_q21(q21_pht_kernel<12, 16, 20, 24>, 256, 0, a, s);
     return launch_resident_q21(q21_pht_kernel<-1, -1, -1, -1>, 256, 0, a, s);
@@ -644,8 +725,8 @@ int fls_query_q21(const fls_column* datekey, const fls_column* suppkey, const fl
     int st = fill_fact(a, datekey, suppkey, partkey, revenue, n_rows, true);
     if (st) return st;
     hipStream_t s = as_stream(stream);
-    if ((st = fill_groups(a, params, d_sums, d_counts, flags, s)) != FLS_OK) return st;
-    if (n_rows == 0) return FLS_OK;
+    if ((st = fill_groups(a, params, d_sums, d_counts)) != FLS_OK) return st;
+    if (n_rows == 0) return zero_groups(a, flags, s);
     const Q21Layout L = q21_layout(dims->n_supp, dims->n_part, dims->n_date);
     const char* ws = static_cast<const char*>(d_workspace);
     a.supp_bm = reinterpret_cast<const uint32_t*>(ws + L.supp_bm);
@@ -664,7 +745,8 @@ int fls_query_q21(const fls_column* datekey, const fls_column* suppkey, const fl
     const bool fast = a.date.enc == FLS_ENC_BITPACK && a.supp.enc == FLS_ENC_BITPACK &&
                       a.part.enc == FLS_ENC_BITPACK && a.rev.enc == FLS_ENC_BITPACK && a.date.bw == 12 &&
                       a.supp.bw == 16 && a.part.bw == 20 && a.rev.bw == 24;
-    if (fast) return launch_compact<12, 16, 20, 24>(a, in_lds, lds, s);
+    if (fast) return launch_compact<12, 16, 20, 24>(a, in_lds, lds, flags, s);
+    if ((st = zero_groups(a, flags, s)) != FLS_OK) return st;
     if (in_lds) {
         auto k = q21_any_kernel<true>;
         FLS_HIP_CHECK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

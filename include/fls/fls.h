@@ -234,7 +234,12 @@ int fls_query_q11(const fls_column* orderdate, const fls_column* quantity, const
  *   slot = (brand * n_years + (year - year_base)) % n_slots    (fls_q21.cu:71)
  * into d_sums[n_slots] (int64) with d_counts[n_slots] qualifying rows per
  * slot; a group exists where its count is non-zero.  Both tables are
- * overwritten (zeroed first) unless flags has FLS_Q_ACCUMULATE.            */
+ * overwritten unless flags has FLS_Q_ACCUMULATE.  The compact probe and the
+ * bit-filtered reference-table probe (n_slots <= 8192) add each block's groups
+ * to one of 16 partial tables and a second launch sums those into the output;
+ * the partial tables are one of 8 sets of a static per-device array, taken
+ * round-robin, so up to 8 such calls may run at the same time on one device.
+ * The other paths zero the output first and add to it.                      */
 typedef struct fls_q21_params {
     uint32_t category;  /* part filter: p_category == category */
     uint32_t region;    /* supplier filter: s_region == region */

@@ -227,3 +227,46 @@ def test_q21_full_size_and_result_rows():
     slots = [(b * 7 + (y - 1992)) % 7000 for y, b, _ in rows]
     assert slots == sorted(slots)
     del sizes, keep
+
+
+def test_q21_partial_tables_reused_and_concurrent():
+    """The compact probe sums its blocks' groups through sets of partial
+    tables that a second launch reads and clears: 20 calls in a row (more than
+    the 8 sets, overwriting and adding, the compact and the bit-filtered
+    reference-table probes), then 8 streams with calls in flight at once, each
+    into its own output."""
+    torch = T.torch_cuda()
+    n = 300_000 + 11
+    fact, dims = T.o_gen_c3(23, n, **SMALL)
+    fact[1] %= SMALL["n_supp"]
+    fact[2] %= SMALL["n_part"]
+    want = T.o_q21(fact, dims, n)
+    d = dev_dims(dims, SMALL)
+    cols = [c.to_device() for c in fls.encode_c3_table(*fact, n_date=SMALL["n_date"], n_supp=SMALL["n_supp"],
+                                                         n_part=SMALL["n_part"])]
+    ws = fls.q21_build(d)
+    pht_b, keep_b = fls.q21_build_pht(d)
+    acc = torch.zeros(14000, dtype=torch.int64, device="cuda")
+    outs = []
+    for k in range(20):
+        if k % 2:
+            outs.append(fls.query_q21_pht(*cols, n, pht_b))
+        else:
+            outs.append(fls.query_q21(*cols, n, d, ws))
+        fls.query_q21(*cols, n, d, ws, accumulate=True, d_out=acc)
+    for o in outs:
+        check(o.cpu().numpy(), want)
+    total = acc.cpu().numpy()
+    np.testing.assert_array_equal(total[:7000], 20 * want[0])
+    np.testing.assert_array_equal(total[7000:].view(np.uint64), 20 * want[1])
+    streams = [torch.cuda.Stream() for _ in range(8)]
+    torch.cuda.synchronize()
+    res = []
+    for s in streams:
+        with torch.cuda.stream(s):
+            out = torch.empty(14000, dtype=torch.int64, device="cuda")
+            res.append(fls.query_q21(*cols, n, d, ws, d_out=out, stream=s.cuda_stream))
+    torch.cuda.synchronize()
+    for o in res:
+        check(o.cpu().numpy(), want)
+    del keep_b
