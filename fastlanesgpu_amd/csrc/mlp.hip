@@ -558,6 +558,7 @@ int fls_mlp_forward(const uint16_t* d_wb, const float* d_feat, uint64_t n, uint8
     using namespace fls;
     if (n == 0) return FLS_OK;
     FLS_RETURN_IF(!d_wb || !d_feat || !d_class, FLS_ERR_NULL);
+    FLS_RETURN_IF(reinterpret_cast<uintptr_t>(d_feat) % 16u, FLS_ERR_ARG);  // a lane reads 4 features at once
     hipLaunchKernelGGL(mlp_forward_kernel, dim3(grid_of(n)), dim3(256), 0, as_stream(stream), d_wb, d_feat, n, d_class,
                        d_conf, d_logits);
     return check_launch();

@@ -538,7 +538,8 @@ int fls_mixed_decode_u32(const uint32_t* d_words, const fls_vmeta* d_vmeta, cons
  * refresh it).  No reference counterpart.
  *   forward: per feature row, the encoding e with the largest softmax mass
  *            (classes 10e..10e+9), the class = its most likely width bucket,
- *            and conf = that mass; d_conf / d_logits may be NULL.
+ *            and conf = that mass; d_conf / d_logits may be NULL.  d_feat
+ *            (n rows of 16 floats) must be 16-byte aligned (FLS_ERR_ARG).
  *   grad: adds the gradient of the mean cross-entropy (inv_batch = 1/batch)
  *         over n rows into d_grad, the summed loss into *d_loss (may be NULL);
  *         d_work holds fls_mlp_grad_workspace_bytes(n) bytes.  Per-wave

@@ -43,3 +43,11 @@ def test_reference_shaped_test_library_loads():
                  "rs_unpack_global", "rs_prims", "rs_simd16", "rs_n_pred_out"):
         assert hasattr(lib, name), name
     assert lib.rs_n_pred_out() == 22
+
+
+def test_mlp_forward_rejects_unaligned_features():
+    # the argument checks run before any device work, so they run here too
+    lib = fls.load()
+    err_arg = -6  # FLS_ERR_ARG
+    got = lib.fls_mlp_forward(0x10000, 0x10004, 16, 0x20000, None, None, None)
+    assert got == err_arg, got
