@@ -20,36 +20,51 @@ namespace {
 
 constexpr int kTrBlock = 256;                 // 4 waves, one vector each
 constexpr int kTrVecs = kTrBlock / 64;
-constexpr uint32_t kTrPad = 1024u + 16u;      // words per staged vector
+constexpr uint32_t kTrPad = 1024u + 64u;      // words per staged vector
 
-// LDS slot of index k: one padding word per 64 keeps the strided gathers of
-// the permutation (stride 64 words) on distinct banks.
-__device__ __forceinline__ uint32_t tr_slot(uint32_t k) { return k + (k >> 6); }
+// LDS slot of index k: four padding words per 64 keep 16-byte groups aligned
+// and the strided gathers of the permutation on distinct banks.
+__device__ __forceinline__ uint32_t tr_slot(uint32_t k) { return k + 4u * (k >> 6); }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// A wave per vector: four 16-byte loads per work-item (a work-item holds
+// words 4t + 256j .. +3) into LDS in stored order, then four 16-byte stores
+// of four consecutive output words, each gathered from LDS through the
+// permutation.  (One word per load and per store, 16 each, took 205 us over
+// 117,188 vectors.)
 template <bool INVERSE>
 __global__ __launch_bounds__(kTrBlock) void transpose_kernel(const uint32_t* __restrict in, uint64_t n_vec,
                                                              uint32_t* __restrict out) {
-    __shared__ uint32_t stage[kTrVecs][kTrPad];
+    __shared__ __align__(16) uint32_t stage[kTrVecs][kTrPad];
     const uint32_t w = threadIdx.x >> 6, t = threadIdx.x & 63u;
     uint32_t* st = stage[w];
     for (uint64_t g = blockIdx.x; g * kTrVecs < n_vec; g += gridDim.x) {
         const uint64_t v = g * kTrVecs + w;
         if (v < n_vec) {  // wave-uniform: one vector per wave
-            const uint32_t* src = in + v * 1024u;
+            const u32x4* src = reinterpret_cast<const u32x4*>(in + v * 1024u);
+            u32x4 x[4];
 #pragma unroll
-            for (int k = 0; k < 16; ++k) st[tr_slot(t + 64u * k)] = src[t + 64u * k];
+            for (int j = 0; j < 4; ++j) x[j] = __builtin_nontemporal_load(src + t + 64u * j);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) *reinterpret_cast<u32x4*>(st + tr_slot(4u * t + 256u * j)) = x[j];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (v < n_vec) {
-            uint32_t* dst = out + v * 1024u;
+            u32x4* dst = reinterpret_cast<u32x4*>(out + v * 1024u);
 #pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const uint32_t p = t + 64u * k;
-                // transpose_i: position p holds original index untranspose_index(p);
-                // transpose_o: original index p sits at position transpose_index(p)
-                dst[p] = st[tr_slot(INVERSE ? dev::transpose_index(p) : dev::untranspose_index(p))];
+            for (int j = 0; j < 4; ++j) {
+                u32x4 y;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const uint32_t p = 4u * t + 256u * j + i;
+                    // transpose_i: position p holds original index untranspose_index(p);
+                    // transpose_o: original index p sits at position transpose_index(p)
+                    y[i] = st[tr_slot(INVERSE ? dev::transpose_index(p) : dev::untranspose_index(p))];
+                }
+                __builtin_nontemporal_store(y, dst + t + 64u * j);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -127,6 +142,7 @@ int fls_transpose_u32(const uint32_t* d_in, uint64_t n_vec, uint32_t inverse, ui
     if (n_vec == 0) return FLS_OK;
     FLS_RETURN_IF(!d_in || !d_out, FLS_ERR_NULL);
     FLS_RETURN_IF(d_in == d_out, FLS_ERR_ARG);
+    FLS_RETURN_IF((reinterpret_cast<uintptr_t>(d_in) | reinterpret_cast<uintptr_t>(d_out)) % 16u, FLS_ERR_ARG);
     uint32_t grid = 0;
     const uint64_t work = ceil_div(n_vec, kTrVecs);  // a grid of every group: the grid-stride loop runs once
     FLS_RETURN_IF(work > 0x7fffffffu, FLS_ERR_SIZE);

@@ -406,7 +406,8 @@ int fls_query_q41(const fls_column* datekey, const fls_column* custkey, const fl
  * transpose_i (inverse == 0) or transpose_o (inverse != 0) of n_vec whole
  * vectors, d_in != d_out: the unified transposed layout of
  * fastlanes/src/transpose.cpp:4110-5135 and its true inverse (the reference's
- * transpose_o at transpose.cpp:5136-6161 repeats transpose_i's permutation).  */
+ * transpose_o at transpose.cpp:5136-6161 repeats transpose_i's permutation).
+ * d_in and d_out must be 16-byte aligned (FLS_ERR_ARG otherwise).            */
 int fls_transpose_u32(const uint32_t* d_in, uint64_t n_vec, uint32_t inverse, uint32_t* d_out, void* stream);
 
 /* Decode a column of any encoding into plain values for a downstream operator

@@ -89,3 +89,6 @@ def test_errors():
     with pytest.raises(fls.FlsError):  # in place is not a permutation
         d = T.to_dev(np.arange(1024, dtype=np.uint32))
         fls.transpose_vectors(d, 1, out=d)
+    with pytest.raises(fls.FlsError):  # vectors are moved 16 bytes at a time
+        d = T.to_dev(np.arange(2048, dtype=np.uint32))
+        fls.transpose_vectors(d[1:1025], 1)
