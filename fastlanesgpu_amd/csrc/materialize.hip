@@ -86,7 +86,7 @@ constexpr uint32_t kRowPad = 33u;       // words per 32 rows of a staged vector
 // (33 words per 32 rows) and read back by rows 32 r + l, so the stores are
 // coalesced.  ROWS writes rows < n_rows only; stored order writes whole
 // vectors.
-template <bool ROWS>
+template <bool ROWS, bool VEC = false>
 __global__ __launch_bounds__(kMatBlock) void materialize_kernel(const ColDev c, uint64_t n_rows, uint64_t n_vec,
                                                                 uint32_t* __restrict out) {
     __shared__ uint32_t stage[ROWS ? kMatVecs : 1][ROWS ? 32u * kRowPad : 1u];
@@ -121,9 +121,26 @@ __global__ __launch_bounds__(kMatBlock) void materialize_kernel(const ColDev c, 
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if constexpr (VEC) {
+                    // rows 4l + 128j .. +3 per work-item: four LDS reads (on
+                    // distinct banks across the half-wave) and one 16-byte store
 #pragma unroll
-                for (int r = 0; r < 32; ++r)
-                    if (32u * r + l < rows) __builtin_nontemporal_store(st[kRowPad * r + l], dst + 32u * r + l);
+                    for (int j = 0; j < 8; ++j) {
+                        const uint32_t r0 = 4u * l + 128u * (uint32_t)j;
+                        const uint32_t* sp = st + r0 + (r0 >> 5);
+                        if (r0 + 4u <= rows) {
+                            typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+                            __builtin_nontemporal_store(u32x4{sp[0], sp[1], sp[2], sp[3]},
+                                                        reinterpret_cast<u32x4*>(dst + r0));
+                        } else {
+                            for (uint32_t i = 0; r0 + i < rows; ++i) dst[r0 + i] = sp[i];
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 32; ++r)
+                        if (32u * r + l < rows) __builtin_nontemporal_store(st[kRowPad * r + l], dst + 32u * r + l);
+                }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -172,8 +189,10 @@ int fls_decode_column_u32(const fls_column* col, uint64_t n_rows, uint32_t order
     uint32_t grid = 0;
     if (order == FLS_ORDER_ROWS) {
         grid = (uint32_t)work;
-        hipLaunchKernelGGL(materialize_kernel<true>, dim3(grid), dim3(kMatBlock), 0, as_stream(stream), c, n_rows,
-                           n_vec, d_out);
+        // 16-byte stores where the output allows them
+        auto k = reinterpret_cast<uintptr_t>(d_out) % 16u ? materialize_kernel<true, false>
+                                                           : materialize_kernel<true, true>;
+        hipLaunchKernelGGL(k, dim3(grid), dim3(kMatBlock), 0, as_stream(stream), c, n_rows, n_vec, d_out);
     } else {
         grid = (uint32_t)work;
         hipLaunchKernelGGL(materialize_kernel<false>, dim3(grid), dim3(kMatBlock), 0, as_stream(stream), c, n_rows,

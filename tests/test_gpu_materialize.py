@@ -51,6 +51,23 @@ def test_decode_rows_is_the_encoded_column(kind, n):
     np.testing.assert_array_equal(got, x)
 
 
+@pytest.mark.parametrize("kind", ["bitpack_t", "delta", "rle"])
+@pytest.mark.parametrize("n", [5_000 + 7, 4 * 1024 + 1, 4 * 1024 + 130])
+def test_decode_rows_into_an_unaligned_output(kind, n):
+    # row order into an output that is not 16-byte aligned takes the
+    # one-word stores; the sizes end the last vector inside a 4-row group
+    torch = T.torch_cuda()
+    x = column_values(n, n + 3)
+    col = encode(x, kind)
+    buf = torch.full((n + 8,), -1, dtype=torch.int32, device="cuda")
+    fls.decode_column(col, n, fls.ORDER_ROWS, out=buf[1:n + 1])
+    got = T.from_dev(buf)
+    np.testing.assert_array_equal(got[1:n + 1], x)
+    assert got[0] == 0xFFFFFFFF and np.all(got[n + 1:] == 0xFFFFFFFF)
+    aligned = T.from_dev(fls.decode_column(col, n, fls.ORDER_ROWS))
+    np.testing.assert_array_equal(aligned, x)
+
+
 @pytest.mark.parametrize("kind", KINDS)
 def test_decode_stored_order(kind):
     n = 9 * 1024 + 500
