@@ -34,6 +34,25 @@ def test_unpack_many_vectors_every_width():
         np.testing.assert_array_equal(T.from_dev(d_out), x, err_msg=f"bw={bw}")
 
 
+@pytest.mark.parametrize("bw", [3, 11, 32])
+def test_unpack_into_an_unaligned_output(bw):
+    # 16-byte stores need a 16-byte aligned output; any other output takes the
+    # one-word stores, with the same values and checksum
+    torch = T.torch_cuda()
+    n_vec = 5
+    x = RNG.integers(0, 2**bw, n_vec * 1024, dtype=np.uint64).astype(np.uint32)
+    d_in = T.to_dev(fls.pack_column(x, bw))
+    buf = torch.full((n_vec * 1024 + 8,), -1, dtype=torch.int32, device="cuda")
+    fls.unpack_global(d_in, buf[1:], n_vec, bw)
+    got = T.from_dev(buf)
+    np.testing.assert_array_equal(got[1:n_vec * 1024 + 1], x)
+    assert got[0] == 0xFFFFFFFF and np.all(got[n_vec * 1024 + 1:] == 0xFFFFFFFF)
+    d_sum = fls.checksum_buffer()
+    fls.unpack_checksum(d_in, buf[3:], n_vec * 1024, bw, d_sum)
+    np.testing.assert_array_equal(T.from_dev(buf)[3:n_vec * 1024 + 3], x)
+    assert int(T.from_dev(d_sum, np.uint64)[0]) == T.o_checksum(x)
+
+
 def test_c1_full_size_values_and_checksum():
     # plain bit-unpack workload: 2^24 values uniform in [0, 2^11), bw 11
     n, bw, seed = 1 << 24, 11, 42
