@@ -177,3 +177,32 @@ def test_full_batch_4096_segments():
     got = counts_of(d).copy()
     want, lo, hi = T.o_c4_counts(1, n_seg, 10, 24, threads=min(16, os.cpu_count() or 1))
     np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("bw", [1, 2, 4, 8, 3, 12])
+def test_narrow_bitpack_ranges(bw):
+    """Bit-packed segments (FOR base or none) under ranges that start below
+    the base, end above the largest value, hold one value, every value, none
+    above the values, and lo > hi (no value): the batched counts equal
+    numpy's lo <= x <= hi on the values."""
+    torch = T.torch_cuda()
+    rng = np.random.default_rng(bw)
+    n = 20 * 1024 + 333
+    segs, want, keep = [], [], []
+    for base in (0, 1000):
+        x = (rng.integers(0, 2**bw, n) + base).astype(np.uint32)
+        col = fls.Column(fls.ENC_BITPACK, n, fls.pack_column_device(T.to_dev(x), n, bw, base=base), bw=bw,
+                         base=base)
+        keep.append(col)
+        top = base + 2**bw - 1
+        for lo, hi in ((base + 2**bw // 4, base + 3 * 2**bw // 4), (0, base + 2**bw // 2), (0, 0xFFFFFFFF),
+                       (base + 2**bw // 2, 0xFFFFFFFF), (base, base), (top, top), (top + 1, top + 100),
+                       (base + 1, base), (5, 3), (0, max(base, 1) - 1)):
+            segs.append({"col": col, "rmeta": None, "n_values": n, "lo": lo, "hi": hi})
+            want.append(int(((x >= lo) & (x <= hi)).sum()))
+    b = fls.batch_from_segments(segs)
+    d = torch.zeros(len(segs), dtype=torch.int64, device="cuda")
+    for tile, block in ((4, 256), (64, 512)):
+        d.fill_(-1)
+        b.run(fls.tune(tile, block, fls.SCHED_DYNAMIC), d)
+        np.testing.assert_array_equal(counts_of(d), np.array(want, np.uint64), err_msg=f"bw {bw} tile {tile}")
