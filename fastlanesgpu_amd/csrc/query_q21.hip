@@ -17,7 +17,8 @@
 // Both probe kernels are persistent, issue every fact column's loads before
 // any data-dependent branch (full scan), and aggregate with 64-bit atomics on
 // the n_slots-entry group table: slot = (brand*n_years + year-year_base) % n_slots
-// (fls_q21.cu:71).
+// (fls_q21.cu:71) -- the compact probe per block through an LDS hash table,
+// then into partial tables that a second kernel sums into the output.
 #include <string.h>
 
 #include <atomic>
