@@ -734,7 +734,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the fused query shards by 1024-row vector ranges but multi-GPU runs are not part of this build",
+              "the fused query shards by 1024-row vector ranges; this benchmark measures one GPU",
               file=sys.stderr)
         sys.exit(2)
     import numpy as np

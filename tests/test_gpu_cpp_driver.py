@@ -1,7 +1,7 @@
 """The hot path driven from C++ through the C ABI alone, as the reference's
 query driver does it (fls_q11.cu:137-223: host encode, copy to the device,
 one fused launch, copy the sum back, check against a scalar loop): no Python
-between the caller and libfls.so.  Built with hipcc on the GPU box."""
+between the caller and libfls.so.  Built with hipcc where the test runs."""
 import shutil
 import subprocess
 
