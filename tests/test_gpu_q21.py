@@ -270,3 +270,30 @@ def test_q21_partial_tables_reused_and_concurrent():
     for o in res:
         check(o.cpu().numpy(), want)
     del keep_b
+
+
+@pytest.mark.parametrize("accumulate", [False, True])
+def test_q21_more_slots_than_the_partial_tables(accumulate):
+    """A slot table larger than the partial tables (8192 slots) adds to the
+    output directly: the compact probe equals the reference-table probe per
+    row on the same slots, and the totals equal the oracle's."""
+    torch = T.torch_cuda()
+    n = 200_000 + 5
+    fact, dims = T.o_gen_c3(29, n, **SMALL)
+    fact[1] %= SMALL["n_supp"]
+    fact[2] %= SMALL["n_part"]
+    want = T.o_q21(fact, dims, n)
+    d = dev_dims(dims, SMALL)
+    cols = [c.to_device() for c in fls.encode_c3_table(*fact, n_date=SMALL["n_date"], n_supp=SMALL["n_supp"],
+                                                         n_part=SMALL["n_part"])]
+    prm = fls.q21_params(n_slots=9001)
+    ws = fls.q21_build(d, prm)
+    pht, keep = fls.q21_build_pht(d, occupancy=False)
+    start = torch.full((2 * 9001,), 5 if accumulate else -1, dtype=torch.int64, device="cuda")
+    out_c = fls.query_q21(*cols, n, d, ws, params=prm, accumulate=accumulate, d_out=start.clone()).cpu().numpy()
+    out_p = fls.query_q21_pht(*cols, n, pht, params=prm, accumulate=accumulate, d_out=start.clone()).cpu().numpy()
+    np.testing.assert_array_equal(out_c, out_p)
+    base = 5 * 9001 if accumulate else 0
+    assert int(out_c[:9001].sum()) - base == int(want[0].sum())
+    assert int(out_c[9001:].sum()) - base == int(want[1].sum())
+    del keep
