@@ -244,41 +244,70 @@ __global__ __launch_bounds__(kBlock) void rle_pack_kernel(const uint32_t* __rest
 // *total = sum of all.  Three phases over blocks of 1024 entries.
 constexpr int kScanBlock = 1024;
 
-__device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t x, uint64_t* sh, uint64_t& total) {
-    // warp-free Hillis-Steele over 1024 entries in LDS
-    const int t = threadIdx.x;
-    sh[t] = x;
+constexpr int kScanWaves = kScanBlock / 64;
+
+// Exclusive scan of one 32-bit value per work-item over the block: a DPP
+// prefix sum per wave, then the 16 wave totals read by every work-item (two
+// barriers; a block's values here are vector word counts, at most about 2^20
+// in total).  sh: kScanWaves words of LDS.
+__device__ __forceinline__ uint32_t block_exclusive_scan32(uint32_t x, uint32_t* sh, uint32_t& total) {
+    const uint32_t incl = dev::wave_inclusive_sum(x);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 63u) sh[w] = incl;
     __syncthreads();
-    for (int off = 1; off < kScanBlock; off <<= 1) {
-        const uint64_t add = t >= off ? sh[t - off] : 0;
-        __syncthreads();
-        sh[t] += add;
-        __syncthreads();
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < kScanWaves; ++k) {
+        const uint32_t t = sh[k];
+        before += k < w ? t : 0u;
+        tot += t;
     }
-    total = sh[kScanBlock - 1];
-    const uint64_t incl = sh[t];
+    total = tot;
+    __syncthreads();  // sh is written again by the next scan
+    return before + incl - x;
+}
+
+// The same for 64-bit values (the block sums): a shuffle prefix sum per wave.
+__device__ __forceinline__ uint64_t block_exclusive_scan64(uint64_t x, uint64_t* sh, uint64_t& total) {
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t incl = x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t y = __shfl_up(incl, off, 64);
+        if (l >= off) incl += y;
+    }
+    if (l == 63) sh[w] = incl;
     __syncthreads();
-    return incl - x;
+    uint64_t before = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < kScanWaves; ++k) {
+        const uint64_t t = sh[k];
+        before += k < w ? t : 0u;
+        tot += t;
+    }
+    total = tot;
+    __syncthreads();
+    return before + incl - x;
 }
 
 __global__ __launch_bounds__(kScanBlock) void scan_reduce_kernel(const uint32_t* __restrict words, uint64_t n,
                                                                   uint64_t* __restrict block_sums) {
-    __shared__ uint64_t sh[kScanBlock];
+    __shared__ uint32_t sh[kScanWaves];
     const uint64_t i = (uint64_t)blockIdx.x * kScanBlock + threadIdx.x;
-    uint64_t total;
-    block_exclusive_scan(i < n ? words[i] : 0u, sh, total);
+    uint32_t total;
+    block_exclusive_scan32(i < n ? words[i] : 0u, sh, total);
     if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
 }
 
 __global__ __launch_bounds__(kScanBlock) void scan_blocks_kernel(uint64_t* __restrict block_sums, uint64_t n_blocks,
                                                                   uint64_t* __restrict total_out) {
-    __shared__ uint64_t sh[kScanBlock];
+    __shared__ uint64_t sh[kScanWaves];
     uint64_t carry = 0;
     for (uint64_t b0 = 0; b0 < n_blocks; b0 += kScanBlock) {
         const uint64_t i = b0 + threadIdx.x;
         const uint64_t x = i < n_blocks ? block_sums[i] : 0u;
         uint64_t total;
-        const uint64_t ex = block_exclusive_scan(x, sh, total);
+        const uint64_t ex = block_exclusive_scan64(x, sh, total);
         if (i < n_blocks) block_sums[i] = carry + ex;
         carry += total;
     }
@@ -289,10 +318,10 @@ __global__ __launch_bounds__(kScanBlock) void scan_apply_kernel(const uint32_t* 
                                                                  const uint32_t* __restrict bws, uint64_t n,
                                                                  const uint64_t* __restrict block_sums,
                                                                  fls_vmeta* __restrict vmeta) {
-    __shared__ uint64_t sh[kScanBlock];
+    __shared__ uint32_t sh[kScanWaves];
     const uint64_t i = (uint64_t)blockIdx.x * kScanBlock + threadIdx.x;
-    uint64_t total;
-    const uint64_t ex = block_exclusive_scan(i < n ? words[i] : 0u, sh, total);
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan32(i < n ? words[i] : 0u, sh, total);
     if (i < n) vmeta[i] = FLS_VMETA(block_sums[blockIdx.x] + ex, bws[i]);
 }
 
