@@ -206,3 +206,37 @@ def test_narrow_bitpack_ranges(bw):
         d.fill_(-1)
         b.run(fls.tune(tile, block, fls.SCHED_DYNAMIC), d)
         np.testing.assert_array_equal(counts_of(d), np.array(want, np.uint64), err_msg=f"bw {bw} tile {tile}")
+
+
+def test_empty_ranges_every_encoding():
+    """hi < lo counts no value whatever the segment's encoding (the fast paths,
+    the one-lane path of partial and MIXED vectors, and the delta path with
+    widths that change inside a tile)."""
+    torch = T.torch_cuda()
+    rng = np.random.default_rng(11)
+    n = 9 * 1024 + 77
+    segs, want, keep = [], [], []
+    x_sorted = np.sort(rng.integers(0, 2**20, n)).astype(np.uint32)
+    w, vm, bs = fls.delta_encode_device(T.to_dev(x_sorted), n)
+    keep.append(fls.Column(fls.ENC_DELTA, n, w, vmeta=vm, bases=bs))
+    starts = rng.random(n) < 0.125
+    starts[0] = True
+    x_rle = rng.integers(0, 2**12, n).astype(np.uint32)[np.cumsum(starts) - 1]
+    w2, vm2, rm2, bs2 = fls.rle_encode_device(T.to_dev(x_rle), n)
+    keep.append(fls.Column(fls.ENC_RLE, n, w2, vmeta=vm2, bases=bs2, aux=rm2))
+    x_bp = rng.integers(0, 2**9, n).astype(np.uint32)
+    keep.append(fls.Column(fls.ENC_BITPACK, n, fls.pack_column_device(T.to_dev(x_bp), n, 9), bw=9))
+    for col, x in zip(keep, (x_sorted, x_rle, x_bp)):
+        mid = int(np.median(x))
+        for lo, hi in ((mid + 1, mid), (0xFFFFFFFF, 0), (mid, mid)):
+            segs.append({"col": col, "rmeta": None, "n_values": n, "lo": lo, "hi": hi})
+            want.append(int(((x >= lo) & (x <= hi)).sum()))
+    b = fls.batch_from_segments(segs)
+    d = torch.zeros(len(segs), dtype=torch.int64, device="cuda")
+    for tile in (4, 64):
+        d.fill_(-1)
+        b.run(fls.tune(tile, 256, fls.SCHED_DYNAMIC), d)
+        np.testing.assert_array_equal(counts_of(d), np.array(want, np.uint64), err_msg=f"tile {tile}")
+        d.fill_(-1)
+        b.run_per_segment(fls.tune(tile, 256, fls.SCHED_STATIC), d)
+        np.testing.assert_array_equal(counts_of(d), np.array(want, np.uint64), err_msg=f"per segment, tile {tile}")
