@@ -207,6 +207,22 @@ struct LaneCol {
             decode_regs(c, v, w);
         }
     }
+    // BW > 0 only: vector `first + slot` of the column as a wave-uniform base
+    // (vector `first`, in scalar registers) and the lane's 32-bit word offset
+    // from it, read through the global address space.  Each load is then one
+    // scalar base, one offset register and an immediate, and its result is
+    // waited for by the in-order memory counter alone: generic (flat) loads
+    // also count on the LDS counter, so the first LDS read after them waits
+    // for every one still in flight.
+    __device__ __forceinline__ void load_at(const ColDev& c, uint64_t first, uint32_t slot) {
+        static_assert(BW > 0, "fixed-width bit-packed columns");
+        typedef const __attribute__((address_space(1))) uint32_t* gptr;
+        base = c.base;
+        const gptr p = (gptr)(c.words + first * (32u * BW));
+        const uint32_t off = slot * (32u * BW) + dev::lane32();
+#pragma unroll
+        for (int k = 0; k < BW; ++k) w[k] = __builtin_nontemporal_load(p + off + 32u * k);
+    }
     __device__ __forceinline__ uint32_t get(int r) const {
         if constexpr (BW >= 0) return dev::extract_at<BW>(w, r) + base;
         else return w[r] + base;

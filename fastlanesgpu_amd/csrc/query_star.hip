@@ -503,19 +503,23 @@ __global__ __launch_bounds__(kStarBlock) void q31_bitmap_kernel(const StarDev a)
     for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
         const uint64_t v = g * kStarVecs + (threadIdx.x >> 5);
         // the whole wave stays in step (the queue flush needs every lane): a
-        // half-wave past the last vector reads the last vector again (in
-        // bounds) and keeps no rows
+        // half-wave past the last vector reads the group's first vector (in
+        // bounds: the group has a live vector) and keeps no rows
         const bool vlive = v < a.n_vec;
         if (!__any(vlive)) continue;
-        const uint64_t vl = vlive ? v : a.n_vec - 1;
+        const uint64_t g0 = g * kStarVecs;
+        const uint32_t slot = vlive ? (threadIdx.x >> 5) : 0u;
         LaneCol<SBW> sc;
         LaneCol<CBW> cc;
         LaneCol<DBW> dc;
         LaneCol<RBW> rc;
-        sc.load(opaque_words(a.supp), vl);
-        cc.load(opaque_words(a.cust), vl);
-        dc.load(opaque_words(a.date), vl);
-        rc.load(opaque_words(a.rev), vl);
+        sc.load_at(a.supp, g0, slot);
+        cc.load_at(a.cust, g0, slot);
+        dc.load_at(a.date, g0, slot);
+        rc.load_at(a.rev, g0, slot);
+        // every load in flight before the first row is tested (the scheduler
+        // otherwise interleaves address arithmetic that waits for early loads)
+        __builtin_amdgcn_sched_barrier(0);
         // supplier semi-join on every row
         uint32_t m = 0;
 #pragma unroll
