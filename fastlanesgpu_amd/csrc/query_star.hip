@@ -850,17 +850,18 @@ __global__ __launch_bounds__(kStarBlock) void q41_bitmap_kernel(const StarDev a)
     for (uint64_t g = blockIdx.x; g < a.n_groups; g += gridDim.x) {
         const uint64_t v = g * kStarVecs + (threadIdx.x >> 5);
         // the whole wave stays in step (the queue flush needs every lane): a
-        // half-wave past the last vector reads the last vector again (in
-        // bounds) and keeps no rows
+        // half-wave past the last vector reads the group's first vector (in
+        // bounds: the group has a live vector) and keeps no rows
         const bool vlive = v < a.n_vec;
         if (!__any(vlive)) continue;
-        const uint64_t vl = vlive ? v : a.n_vec - 1;
+        const uint64_t g0 = g * kStarVecs;
+        const uint32_t slot = vlive ? (threadIdx.x >> 5) : 0u;
         uint32_t m = 0;
         {
             LaneCol<SBW> sc;
             LaneCol<PBW> pc;
-            sc.load(opaque_words(a.supp), vl);
-            pc.load(opaque_words(a.part), vl);
+            sc.load_at(a.supp, g0, slot);
+            pc.load_at(a.part, g0, slot);
 #pragma unroll
             for (int r = 0; r < 32; ++r) {
                 const uint32_t ks = min(dev::extract_at<SBW>(sc.w, r) + d_s, a.n_supp);
@@ -879,10 +880,10 @@ __global__ __launch_bounds__(kStarBlock) void q41_bitmap_kernel(const StarDev a)
         LaneCol<DBW> dc;
         LaneCol<RBW> rc;
         LaneCol<KBW> kc;
-        cc.load(opaque_words(a.cust), vl);
-        dc.load(opaque_words(a.date), vl);
-        rc.load(opaque_words(a.rev), vl);
-        kc.load(opaque_words(a.cost), vl);
+        cc.load_at(a.cust, g0, slot);
+        dc.load_at(a.date, g0, slot);
+        rc.load_at(a.rev, g0, slot);
+        kc.load_at(a.cost, g0, slot);
         cc.pin();
         dc.pin();
         rc.pin();
