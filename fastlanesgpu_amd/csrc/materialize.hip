@@ -85,11 +85,14 @@ constexpr uint32_t kRowPad = 33u;       // words per 32 rows of a staged vector
 // untranspose_index(l + 32 r); the vector is staged in LDS by original row
 // (33 words per 32 rows) and read back by rows 32 r + l, so the stores are
 // coalesced.  ROWS writes rows < n_rows only; stored order writes whole
-// vectors.
+// vectors.  VEC (a 16-byte aligned output): the vector goes through the LDS
+// stage in either order (stored order: word 32 r + l at its own index) and
+// each work-item stores four consecutive words at a time.
 template <bool ROWS, bool VEC = false>
 __global__ __launch_bounds__(kMatBlock) void materialize_kernel(const ColDev c, uint64_t n_rows, uint64_t n_vec,
                                                                 uint32_t* __restrict out) {
-    __shared__ uint32_t stage[ROWS ? kMatVecs : 1][ROWS ? 32u * kRowPad : 1u];
+    constexpr bool kStage = ROWS || VEC;
+    __shared__ uint32_t stage[kStage ? kMatVecs : 1][kStage ? 32u * kRowPad : 1u];
     const uint32_t l = dev::lane32();
     const uint32_t slot = threadIdx.x >> 5;
     for (uint64_t g = blockIdx.x; g * kMatVecs < n_vec; g += gridDim.x) {
@@ -98,14 +101,14 @@ __global__ __launch_bounds__(kMatBlock) void materialize_kernel(const ColDev c, 
         uint32_t x[32];
         if (live) decode_regs(c, v, x);
         uint32_t* dst = out + v * 1024u;
-        if constexpr (!ROWS) {
+        if constexpr (!kStage) {
             if (live) {
 #pragma unroll
                 for (int r = 0; r < 32; ++r) __builtin_nontemporal_store(x[r], dst + 32u * r + l);
             }
         } else {
-            const uint32_t rows = live ? (uint32_t)min<uint64_t>(1024u, n_rows - v * 1024u) : 0u;
-            if (!c.transposed) {
+            const uint32_t rows = !live ? 0u : ROWS ? (uint32_t)min<uint64_t>(1024u, n_rows - v * 1024u) : 1024u;
+            if (ROWS && !c.transposed) {
 #pragma unroll
                 for (int r = 0; r < 32; ++r)
                     if (32u * r + l < rows) __builtin_nontemporal_store(x[r], dst + 32u * r + l);
@@ -114,7 +117,7 @@ __global__ __launch_bounds__(kMatBlock) void materialize_kernel(const ColDev c, 
                 if (live) {
 #pragma unroll
                     for (int r = 0; r < 32; ++r) {
-                        const uint32_t i = dev::untranspose_index(l + 32u * r);
+                        const uint32_t i = ROWS ? dev::untranspose_index(l + 32u * r) : l + 32u * r;
                         st[i + (i >> 5)] = x[r];
                     }
                 }
@@ -195,8 +198,9 @@ int fls_decode_column_u32(const fls_column* col, uint64_t n_rows, uint32_t order
         hipLaunchKernelGGL(k, dim3(grid), dim3(kMatBlock), 0, as_stream(stream), c, n_rows, n_vec, d_out);
     } else {
         grid = (uint32_t)work;
-        hipLaunchKernelGGL(materialize_kernel<false>, dim3(grid), dim3(kMatBlock), 0, as_stream(stream), c, n_rows,
-                           n_vec, d_out);
+        auto k = reinterpret_cast<uintptr_t>(d_out) % 16u ? materialize_kernel<false, false>
+                                                           : materialize_kernel<false, true>;
+        hipLaunchKernelGGL(k, dim3(grid), dim3(kMatBlock), 0, as_stream(stream), c, n_rows, n_vec, d_out);
     }
     return check_launch();
 }

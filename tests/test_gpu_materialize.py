@@ -68,6 +68,23 @@ def test_decode_rows_into_an_unaligned_output(kind, n):
     np.testing.assert_array_equal(aligned, x)
 
 
+@pytest.mark.parametrize("kind", ["bitpack_t", "delta", "rle"])
+def test_decode_stored_order_into_an_unaligned_output(kind):
+    # stored order into an output that is not 16-byte aligned takes the
+    # one-word stores; the aligned output takes the LDS stage and 16-byte
+    # stores: both hold the same words
+    torch = T.torch_cuda()
+    n = 3 * 1024 + 300
+    x = column_values(n, n + 5)
+    col = encode(x, kind)
+    aligned = T.from_dev(fls.decode_column(col, n, fls.ORDER_STORED))
+    buf = torch.full((4 * 1024 + 8,), -1, dtype=torch.int32, device="cuda")
+    fls.decode_column(col, n, fls.ORDER_STORED, out=buf[1:4 * 1024 + 1])
+    got = T.from_dev(buf)
+    np.testing.assert_array_equal(got[1:4 * 1024 + 1], aligned)
+    assert got[0] == 0xFFFFFFFF and np.all(got[4 * 1024 + 1:] == 0xFFFFFFFF)
+
+
 @pytest.mark.parametrize("kind", KINDS)
 def test_decode_stored_order(kind):
     n = 9 * 1024 + 500
