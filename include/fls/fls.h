@@ -140,7 +140,9 @@ int fls_rle_pack_device(const uint32_t* d_in, uint64_t n, const fls_vmeta* d_vme
 /* ========================================================== GPU decode (gfx950)
  * Grid-level decode of n_vec fixed-width vectors into normal order
  * out[v*1024 + 32*r + l].  Replaces __global__ unpack_global(in, out, bw)
- * launched <<<n_vec, 32>>> (unpack.cuh:3436-3442, unpack_fused.cuh:3444-3453). */
+ * launched <<<n_vec, 32>>> (unpack.cuh:3436-3442, unpack_fused.cuh:3444-3453).
+ * A 16-byte aligned d_out takes 16-byte stores; any 4-byte aligned d_out is
+ * correct.                                                                    */
 int fls_unpack_u32(const uint32_t* d_in, uint32_t* d_out, uint64_t n_vec, uint32_t bw, void* stream);
 /* Same decode that also produces the order-sensitive checksum
  * sum_i (i+1)*value_i mod 2^64 over the first n_values values into
@@ -416,7 +418,9 @@ int fls_transpose_u32(const uint32_t* d_in, uint64_t n_vec, uint32_t inverse, ui
  *                     MIXED and transposed BITPACK); whole vectors, so d_out
  *                     holds ceil(n_rows / 1024) * 1024 values;
  *   FLS_ORDER_ROWS    original row order (transposed columns are untransposed
- *                     on the fly); exactly n_rows values.                    */
+ *                     on the fly); exactly n_rows values.
+ * A 16-byte aligned d_out takes 16-byte stores; any 4-byte aligned d_out is
+ * correct.                                                                   */
 #define FLS_ORDER_STORED 0u
 #define FLS_ORDER_ROWS 1u
 int fls_decode_column_u32(const fls_column* col, uint64_t n_rows, uint32_t order, uint32_t* d_out, void* stream);
